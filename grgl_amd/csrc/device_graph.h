@@ -1,0 +1,155 @@
+// Device-resident GRG and the traversal driver shared by the C ABI and the server.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "grgl_hip.h"
+#include "levelize.h"
+
+namespace grgl_amd {
+
+struct MisuseError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct NoMemError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct UnsupportedError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define GRGL_HIP_CHECK(expr)                                                                           \
+    do {                                                                                               \
+        hipError_t _e = (expr);                                                                        \
+        if (_e != hipSuccess) {                                                                        \
+            if (_e == hipErrorOutOfMemory) throw ::grgl_amd::NoMemError(std::string(#expr) + ": out of device memory"); \
+            throw ::grgl_amd::DeviceError(std::string(#expr) + ": " + hipGetErrorString(_e));          \
+        }                                                                                              \
+    } while (0)
+
+// Arithmetic class of a dtype: signed and unsigned integers of one width share the
+// wrapping add; bool is XOR on bytes.
+enum TypeClass : int { kF64 = 0, kF32 = 1, kW64 = 2, kW32 = 3, kW16 = 4, kW8 = 5, kX8 = 6 };
+TypeClass typeClassOf(grgl_dtype dt);
+size_t dtypeSize(grgl_dtype dt);
+
+struct RowGeometry {
+    uint32_t vb;        // bytes per lane access: 4, 8 or 16
+    uint32_t rowBytes;  // K * elemSize rounded up to vb
+    uint64_t ld;        // row stride in bytes
+    uint32_t lanes;     // rowBytes / vb
+};
+RowGeometry rowGeometry(uint32_t K, size_t elemSize);
+
+struct DeviceDirPlan {
+    uint32_t numNodes = 0, numLevels = 0, firstPulledLevel = 0, maxFanIn = 0;
+    uint64_t numEntries = 0;
+    std::vector<uint32_t> levelStart;
+    std::vector<uint32_t> fanIn;   // host: per row list length (heavy split points)
+    uint32_t* off = nullptr;
+    uint32_t* src = nullptr;
+    uint32_t* order = nullptr;     // row -> node
+    uint32_t* rank = nullptr;      // node -> row
+    uint32_t* outRow = nullptr;
+    uint32_t* missRow = nullptr;
+    uint32_t* coalByRow = nullptr;
+    uint32_t heavyFanin = 64;
+    std::vector<uint32_t> heavyCount; // per level, for heavyFanin
+    void computeHeavy();
+};
+
+struct Workspace {
+    char* ptr = nullptr;
+    size_t bytes = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    bool used = false;
+};
+
+struct Tuning {
+    uint32_t teamLanes = 0; // 0 = default
+    uint32_t useGraph = 0;
+};
+
+struct CapturedLevels {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+} // namespace grgl_amd
+
+struct grgl_graph {
+    int device = 0;
+    uint32_t N = 0, V = 0, M = 0, P = 1;
+    uint64_t E = 0;
+    bool hasMiss = false, hasCoal = false;
+    grgl_amd::DeviceDirPlan up, down;
+    uint64_t deviceBytes = 0;
+    std::mutex mu;
+    std::vector<std::unique_ptr<grgl_amd::Workspace>> workspaces; // stable addresses
+    std::vector<hipStream_t> freeStreams;
+    std::map<std::tuple<int, int, uint32_t>, grgl_amd::Tuning> tuning; // (dir, typeClass, K)
+    // Kernel timing (level kernels).
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> timingEvents;
+    std::vector<double> timingBytes;
+    double timedMs = 0.0;
+    uint64_t timedLaunches = 0;
+    double timedBytes = 0.0;
+    // Captured level sequences: (dir, typeClass, vb, tw, ld, rowBytes, readSelf, rows ptr)
+    std::map<std::tuple<int, int, uint32_t, uint32_t, uint64_t, uint32_t, int, uintptr_t>, grgl_amd::CapturedLevels> graphs;
+};
+
+namespace grgl_amd {
+
+struct TravRequest {
+    uint32_t K;
+    const void* in;   // device, K x inCols
+    void* out;        // device, K x outCols
+    uint32_t kOffset;
+};
+
+struct TravOptions {
+    bool emitAll = false;
+    bool byIndividual = false;
+    grgl_init_mode initMode = GRGL_INIT_ZERO;
+    const void* init = nullptr;  // device
+    void* miss = nullptr;        // device
+    bool accumulate = false;
+    bool orderPreserving = false;
+};
+
+// Size in bytes of the row buffer for a traversal of Ktotal stacked rows.
+size_t rowsBufferBytes(const grgl_graph* g, grgl_direction dir, uint32_t Ktotal, size_t elemSize);
+
+// Enqueue one traversal on `stream`: seed, all levels, read-out for every request.
+// `rows` must hold rowsBufferBytes(...) bytes.  Device pointers only.
+void enqueueTraversal(grgl_graph* g, grgl_dtype dt, grgl_direction dir, const std::vector<TravRequest>& reqs,
+                      uint32_t Ktotal, const TravOptions& opt, char* rows, hipStream_t stream);
+
+// Workspace cache (stream-ordered reuse).
+Workspace* acquireWorkspace(grgl_graph* g, size_t bytes, hipStream_t stream);
+void releaseWorkspace(grgl_graph* g, Workspace* ws, hipStream_t stream);
+hipStream_t acquireInternalStream(grgl_graph* g);
+void releaseInternalStream(grgl_graph* g, hipStream_t s);
+
+void setLastError(const std::string& msg);
+
+// Validation shared by grgl_hip_matmul and the server (internal_mult.h:130-158).
+void validateShapes(const grgl_graph* g, grgl_direction dir, size_t inCols, size_t rows, size_t outSize,
+                    bool emitAll, bool byIndividual);
+
+} // namespace grgl_amd

@@ -1,0 +1,989 @@
+// libgrgl_hip.so: device graph upload, the traversal driver and the C ABI of grgl_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+#include "device_graph.h"
+#include "kernels.h"
+
+#define GRGL_EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace grgl_amd {
+
+namespace {
+thread_local std::string g_lastError;
+
+template <typename T> T* uploadVec(const std::vector<T>& v, uint64_t& bytes) {
+    if (v.empty()) return nullptr;
+    T* d = nullptr;
+    GRGL_HIP_CHECK(hipMalloc(&d, v.size() * sizeof(T)));
+    GRGL_HIP_CHECK(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    bytes += v.size() * sizeof(T);
+    return d;
+}
+
+void freeDev(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+uint32_t pow2ceil(uint32_t x) {
+    uint32_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+uint32_t log2u(uint32_t x) {
+    uint32_t l = 0;
+    while ((1U << l) < x) l++;
+    return l;
+}
+
+// Transpose tile: TK = min(32, pow2ceil(K)).
+uint32_t tileShiftFor(uint32_t K) { return std::min<uint32_t>(5, log2u(pow2ceil(K))); }
+
+} // namespace
+
+void setLastError(const std::string& msg) { g_lastError = msg; }
+
+TypeClass typeClassOf(grgl_dtype dt) {
+    switch (dt) {
+    case GRGL_F64: return kF64;
+    case GRGL_F32: return kF32;
+    case GRGL_I64:
+    case GRGL_U64: return kW64;
+    case GRGL_I32:
+    case GRGL_U32: return kW32;
+    case GRGL_I16:
+    case GRGL_U16: return kW16;
+    case GRGL_I8:
+    case GRGL_U8: return kW8;
+    case GRGL_BOOL: return kX8;
+    }
+    throw MisuseError("Unsupported dtype code " + std::to_string((int)dt));
+}
+
+size_t dtypeSize(grgl_dtype dt) {
+    switch (typeClassOf(dt)) {
+    case kF64:
+    case kW64: return 8;
+    case kF32:
+    case kW32: return 4;
+    case kW16: return 2;
+    default: return 1;
+    }
+}
+
+RowGeometry rowGeometry(uint32_t K, size_t elemSize) {
+    RowGeometry r;
+    const uint64_t raw = (uint64_t)K * elemSize;
+    r.vb = raw <= 4 ? 4 : (raw <= 8 ? 8 : 16);
+    const uint64_t rb = (raw + r.vb - 1) / r.vb * r.vb;
+    if (rb > 0xffffffffULL) throw MisuseError("too many rows (K) for one traversal");
+    r.rowBytes = (uint32_t)rb;
+    r.ld = rb >= 64 ? (rb + 63) / 64 * 64 : rb;
+    r.lanes = r.rowBytes / r.vb;
+    return r;
+}
+
+void DeviceDirPlan::computeHeavy() {
+    heavyCount.assign(numLevels, 0);
+    for (uint32_t l = firstPulledLevel; l < numLevels; l++) {
+        uint32_t lo = levelStart[l], hi = levelStart[l + 1];
+        const uint32_t begin = lo;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (fanIn[mid] > heavyFanin) lo = mid + 1;
+            else hi = mid;
+        }
+        heavyCount[l] = lo - begin;
+    }
+}
+
+size_t rowsBufferBytes(const grgl_graph* g, grgl_direction dir, uint32_t Ktotal, size_t elemSize) {
+    const RowGeometry geo = rowGeometry(Ktotal, elemSize);
+    uint64_t rows = g->V;
+    if (dir == GRGL_DIRECTION_DOWN) rows += (uint64_t)g->M * (g->hasMiss ? 2 : 1);
+    return (size_t)(rows * geo.ld) + 256;
+}
+
+void validateShapes(const grgl_graph* g, grgl_direction dir, size_t inCols, size_t rows, size_t outSize,
+                    bool emitAll, bool byIndividual) {
+    // release_asserts of grg.h:1353-1354 and internal_mult.h:139 become misuse errors here.
+    if (rows == 0 || inCols == 0) throw MisuseError("matmul() requires non-zero dimensions.");
+    if (outSize % rows != 0) throw MisuseError("Output size is not a multiple of the number of input rows");
+    const size_t outCols = outSize / rows;
+    const size_t expectSampleCols = byIndividual ? g->N / g->P : g->N;
+    if (dir == GRGL_DIRECTION_DOWN) {
+        if (inCols != g->M) throw MisuseError("Input vector is not size M (numMutations)");
+        if (!emitAll && outCols != expectSampleCols)
+            throw MisuseError("Output vector is not size N (numSamples, or numIndividuals depending on flags)");
+    } else if (dir == GRGL_DIRECTION_UP) {
+        if (inCols != expectSampleCols)
+            throw MisuseError("Input vector is not size N (numSamples, or numIndividuals depending on flags)");
+        if (!emitAll && outCols != g->M) throw MisuseError("Output vector is not size M (numMutations)");
+    } else {
+        throw MisuseError("Invalid traversal direction");
+    }
+    if (emitAll && outCols != g->V) throw MisuseError("Output vector is not size numNodes");
+}
+
+// ---------------------------------------------------------------------------------------
+// Kernel dispatch
+// ---------------------------------------------------------------------------------------
+namespace {
+
+template <typename T, bool XOR> void launchPullT(uint32_t vb, uint32_t tw, dim3 grid, const PullArgs& a, hipStream_t s) {
+#define GRGL_PULL(VB, TW) hipLaunchKernelGGL((pull_level_kernel<T, VB, TW, XOR>), grid, dim3(kBlock), 0, s, a)
+    if (vb == 4) {
+        if constexpr (sizeof(T) <= 4) GRGL_PULL(4, 1);
+    } else if (vb == 8) {
+        if constexpr (sizeof(T) <= 8) GRGL_PULL(8, 1);
+    } else {
+        switch (tw) {
+        case 1: GRGL_PULL(16, 1); break;
+        case 2: GRGL_PULL(16, 2); break;
+        case 4: GRGL_PULL(16, 4); break;
+        case 8: GRGL_PULL(16, 8); break;
+        case 16: GRGL_PULL(16, 16); break;
+        case 32: GRGL_PULL(16, 32); break;
+        default: GRGL_PULL(16, 64); break;
+        }
+    }
+#undef GRGL_PULL
+}
+
+void launchPull(TypeClass tc, uint32_t vb, uint32_t tw, dim3 grid, const PullArgs& a, hipStream_t s) {
+    switch (tc) {
+    case kF64: launchPullT<double, false>(vb, tw, grid, a, s); break;
+    case kF32: launchPullT<float, false>(vb, tw, grid, a, s); break;
+    case kW64: launchPullT<uint64_t, false>(vb, tw, grid, a, s); break;
+    case kW32: launchPullT<uint32_t, false>(vb, tw, grid, a, s); break;
+    case kW16: launchPullT<uint16_t, false>(vb, tw, grid, a, s); break;
+    case kW8: launchPullT<uint8_t, false>(vb, tw, grid, a, s); break;
+    case kX8: launchPullT<uint8_t, true>(vb, tw, grid, a, s); break;
+    }
+}
+
+template <typename T, bool XOR> void launchSeedT(dim3 grid, const SeedArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL((seed_rows_kernel<T, XOR>), grid, dim3(kBlock), 0, s, a);
+}
+void launchSeed(TypeClass tc, dim3 grid, const SeedArgs& a, hipStream_t s) {
+    switch (tc) {
+    case kF64: launchSeedT<double, false>(grid, a, s); break;
+    case kF32: launchSeedT<float, false>(grid, a, s); break;
+    case kW64: launchSeedT<uint64_t, false>(grid, a, s); break;
+    case kW32: launchSeedT<uint32_t, false>(grid, a, s); break;
+    case kW16: launchSeedT<uint16_t, false>(grid, a, s); break;
+    case kW8: launchSeedT<uint8_t, false>(grid, a, s); break;
+    case kX8: launchSeedT<uint8_t, true>(grid, a, s); break;
+    }
+}
+
+template <typename T, bool XOR> void launchReadoutT(dim3 grid, const ReadoutArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL((readout_kernel<T, XOR>), grid, dim3(kBlock), 0, s, a);
+}
+void launchReadout(TypeClass tc, dim3 grid, const ReadoutArgs& a, hipStream_t s) {
+    switch (tc) {
+    case kF64: launchReadoutT<double, false>(grid, a, s); break;
+    case kF32: launchReadoutT<float, false>(grid, a, s); break;
+    case kW64: launchReadoutT<uint64_t, false>(grid, a, s); break;
+    case kW32: launchReadoutT<uint32_t, false>(grid, a, s); break;
+    case kW16: launchReadoutT<uint16_t, false>(grid, a, s); break;
+    case kW8: launchReadoutT<uint8_t, false>(grid, a, s); break;
+    case kX8: launchReadoutT<uint8_t, true>(grid, a, s); break;
+    }
+}
+
+void launchInit(TypeClass tc, const InitArgs& a, hipStream_t s) {
+    const uint64_t n = (uint64_t)(a.rowEnd - a.rowBegin) * a.K;
+    if (n == 0) return;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + kBlock - 1) / kBlock, 65536);
+    switch (tc) {
+    case kF64: hipLaunchKernelGGL((init_rows_kernel<double>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+    case kF32: hipLaunchKernelGGL((init_rows_kernel<float>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+    case kW64: hipLaunchKernelGGL((init_rows_kernel<uint64_t>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+    case kW32: hipLaunchKernelGGL((init_rows_kernel<uint32_t>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+    case kW16: hipLaunchKernelGGL((init_rows_kernel<uint16_t>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL((init_rows_kernel<uint8_t>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+    }
+}
+
+void seedRows(TypeClass tc, const void* in, uint64_t inCols, uint32_t K, char* rows, uint64_t ld, uint32_t rowBase,
+              uint32_t kOffset, uint32_t D, uint32_t per, const uint32_t* colMap, bool accumulate, hipStream_t s) {
+    if (D == 0 || K == 0) return;
+    SeedArgs a;
+    a.in = in;
+    a.inCols = inCols;
+    a.K = K;
+    a.rows = rows;
+    a.ld = ld;
+    a.rowBase = rowBase;
+    a.kOffset = kOffset;
+    a.D = D;
+    a.per = per;
+    a.colMap = colMap;
+    a.accumulate = accumulate ? 1 : 0;
+    a.tkShift = tileShiftFor(K);
+    a.normalizeBool = tc == kX8 ? 1 : 0;
+    const uint32_t TK = 1U << a.tkShift, TD = 1024U / TK;
+    dim3 grid((D + TD - 1) / TD, (K + TK - 1) / TK);
+    launchSeed(tc, grid, a, s);
+}
+
+void readoutRows(TypeClass tc, const char* rows, uint64_t ld, uint32_t kOffset, uint32_t K, const uint32_t* map,
+                 uint32_t per, uint64_t C, void* out, uint64_t outCols, bool accumulate, hipStream_t s) {
+    if (C == 0 || K == 0) return;
+    ReadoutArgs a;
+    a.rows = rows;
+    a.ld = ld;
+    a.kOffset = kOffset;
+    a.K = K;
+    a.map = map;
+    a.per = per;
+    a.C = C;
+    a.out = out;
+    a.outCols = outCols;
+    a.accumulate = accumulate ? 1 : 0;
+    a.tkShift = tileShiftFor(K);
+    const uint32_t TK = 1U << a.tkShift, TD = 1024U / TK;
+    dim3 grid((uint32_t)((C + TD - 1) / TD), (K + TK - 1) / TK);
+    launchReadout(tc, grid, a, s);
+}
+
+Tuning chooseTuning(grgl_graph* g, grgl_direction dir, TypeClass tc, uint32_t K, const RowGeometry& geo) {
+    Tuning t;
+    {
+        std::lock_guard<std::mutex> lk(g->mu);
+        auto it = g->tuning.find(std::make_tuple((int)dir, (int)tc, K));
+        if (it != g->tuning.end()) t = it->second;
+    }
+    uint32_t tw = t.teamLanes;
+    if (geo.vb != 16) {
+        tw = 1;
+    } else {
+        if (tw == 0) tw = std::min<uint32_t>(64, pow2ceil(geo.lanes));
+        tw = std::min<uint32_t>(64, pow2ceil(tw));
+    }
+    t.teamLanes = tw;
+    return t;
+}
+
+// Launch every pulled level of one direction.
+void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const RowGeometry& geo, uint32_t tw,
+                   bool readSelf, bool orderPreserving, uint32_t K, size_t elemSize, char* rows, hipStream_t stream) {
+    const uint32_t tiles = (geo.lanes + tw - 1) / tw;
+    const uint32_t teamsPerBlock = kBlock / tw;
+    for (uint32_t l = plan.firstPulledLevel; l < plan.numLevels; l++) {
+        const uint32_t begin = plan.levelStart[l], end = plan.levelStart[l + 1];
+        if (end == begin) continue;
+        const uint32_t heavy = orderPreserving ? 0 : plan.heavyCount[l];
+        const uint32_t light = end - begin - heavy;
+        PullArgs a;
+        a.rows = rows;
+        a.ld = geo.ld;
+        a.off = plan.off;
+        a.src = plan.src;
+        a.levelBegin = begin;
+        a.heavyCount = heavy;
+        a.levelEnd = end;
+        a.rowBytes = geo.rowBytes;
+        a.readSelf = readSelf ? 1 : 0;
+        dim3 grid(heavy + (light + teamsPerBlock - 1) / teamsPerBlock, tiles);
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        if (g->timing) {
+            GRGL_HIP_CHECK(hipEventCreate(&ev0));
+            GRGL_HIP_CHECK(hipEventCreate(&ev1));
+            GRGL_HIP_CHECK(hipEventRecord(ev0, stream));
+        }
+        launchPull(tc, geo.vb, tw, grid, a, stream);
+        GRGL_HIP_CHECK(hipGetLastError());
+        if (g->timing) {
+            GRGL_HIP_CHECK(hipEventRecord(ev1, stream));
+            const uint64_t nodes = end - begin;
+            uint64_t ent = 0;
+            for (uint32_t r = begin; r < end; r++) ent += plan.fanIn[r];
+            const double rowB = (double)K * (double)elemSize;
+            const double bytes = 4.0 * (double)(ent + nodes + 1) + rowB * (double)(ent + nodes + (readSelf ? nodes : 0));
+            g->timingEvents.push_back({ev0, ev1});
+            g->timingBytes.push_back(bytes);
+        }
+    }
+}
+
+// Level sequence, either launched directly or replayed from a captured hipGraph.  A
+// captured sequence bakes in the row buffer address, the row geometry and the team
+// width, so the cache key holds all of them; different workspaces get different graphs.
+void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometry& geo, const Tuning& t,
+               bool readSelf, bool orderPreserving, uint32_t K, size_t es, char* rows, hipStream_t stream) {
+    DeviceDirPlan& plan = dir == GRGL_DIRECTION_UP ? g->up : g->down;
+    if (!t.useGraph || stream == nullptr || g->timing) {
+        enqueueLevels(g, plan, tc, geo, t.teamLanes, readSelf, orderPreserving, K, es, rows, stream);
+        return;
+    }
+    const auto key = std::make_tuple((int)dir, (int)tc, geo.vb, t.teamLanes, geo.ld, geo.rowBytes,
+                                     (int)readSelf * 2 + (int)orderPreserving + 4 * (int)plan.heavyFanin,
+                                     (uintptr_t)rows);
+    hipGraphExec_t exec = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g->mu);
+        auto it = g->graphs.find(key);
+        if (it != g->graphs.end()) exec = it->second.exec;
+    }
+    if (!exec) {
+        CapturedLevels cap;
+        GRGL_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueueLevels(g, plan, tc, geo, t.teamLanes, readSelf, orderPreserving, K, es, rows, stream);
+        } catch (...) {
+            hipGraph_t broken = nullptr;
+            (void)hipStreamEndCapture(stream, &broken);
+            if (broken) (void)hipGraphDestroy(broken);
+            throw;
+        }
+        GRGL_HIP_CHECK(hipStreamEndCapture(stream, &cap.graph));
+        const hipError_t ie = hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(cap.graph);
+            GRGL_HIP_CHECK(ie);
+        }
+        exec = cap.exec;
+        std::lock_guard<std::mutex> lk(g->mu);
+        if (g->graphs.size() >= 64) {
+            // Bounded cache: drop everything not in use (graphs of idle workspaces).
+            (void)hipDeviceSynchronize();
+            for (auto& kv : g->graphs) {
+                (void)hipGraphExecDestroy(kv.second.exec);
+                (void)hipGraphDestroy(kv.second.graph);
+            }
+            g->graphs.clear();
+        }
+        g->graphs[key] = cap;
+    }
+    GRGL_HIP_CHECK(hipGraphLaunch(exec, stream));
+}
+
+} // namespace
+
+void enqueueTraversal(grgl_graph* g, grgl_dtype dt, grgl_direction dir, const std::vector<TravRequest>& reqs,
+                      uint32_t Ktotal, const TravOptions& opt, char* rows, hipStream_t stream) {
+    const TypeClass tc = typeClassOf(dt);
+    const size_t es = dtypeSize(dt);
+    const RowGeometry geo = rowGeometry(Ktotal, es);
+    DeviceDirPlan& plan = dir == GRGL_DIRECTION_UP ? g->up : g->down;
+    const bool hasInit = opt.initMode != GRGL_INIT_ZERO;
+    // 1. Node initialisation (grg.h:1369-1402).
+    if (hasInit) {
+        for (const auto& r : reqs) {
+            if (opt.initMode == GRGL_INIT_MATRIX) {
+                // init is K x numNodes; row j takes column order[j].
+                seedRows(tc, opt.init, g->V, r.K, rows, geo.ld, 0, r.kOffset, g->V, 1, plan.order, false, stream);
+            } else {
+                InitArgs a;
+                a.rows = rows;
+                a.ld = geo.ld;
+                a.rowBegin = 0;
+                a.rowEnd = g->V;
+                a.kOffset = r.kOffset;
+                a.K = r.K;
+                a.kind = opt.initMode == GRGL_INIT_VECTOR ? kInitVector : kInitXtx;
+                a.vec = opt.init;
+                a.coalByRow = plan.coalByRow;
+                launchInit(tc, a, stream);
+            }
+        }
+    } else if (dir == GRGL_DIRECTION_UP && plan.levelStart[1] > g->N) {
+        // Childless non-sample nodes of level 0 hold zero.
+        InitArgs a{};
+        a.rows = rows;
+        a.ld = geo.ld;
+        a.rowBegin = g->N;
+        a.rowEnd = plan.levelStart[1];
+        a.kOffset = 0;
+        a.K = Ktotal;
+        a.kind = kInitZero;
+        launchInit(tc, a, stream);
+    }
+    // 2. Seeds.
+    for (const auto& r : reqs) {
+        if (dir == GRGL_DIRECTION_UP) {
+            // Samples are rows 0..N-1 (grg.h:1456-1469).
+            const uint32_t per = opt.byIndividual ? g->P : 1;
+            const uint64_t inCols = opt.byIndividual ? g->N / g->P : g->N;
+            seedRows(tc, r.in, inCols, r.K, rows, geo.ld, 0, r.kOffset, g->N, per, nullptr, hasInit, stream);
+        } else {
+            // Mutation seed rows V..V+M-1; missingness seed rows V+M..V+2M-1.
+            seedRows(tc, r.in, g->M, r.K, rows, geo.ld, g->V, r.kOffset, g->M, 1, nullptr, false, stream);
+            if (g->hasMiss) {
+                if (opt.miss) {
+                    seedRows(tc, opt.miss, g->M, r.K, rows, geo.ld, g->V + g->M, r.kOffset, g->M, 1, nullptr, false,
+                             stream);
+                } else {
+                    InitArgs a{};
+                    a.rows = rows;
+                    a.ld = geo.ld;
+                    a.rowBegin = g->V + g->M;
+                    a.rowEnd = g->V + 2 * g->M;
+                    a.kOffset = r.kOffset;
+                    a.K = r.K;
+                    a.kind = kInitZero;
+                    launchInit(tc, a, stream);
+                }
+            }
+        }
+    }
+    GRGL_HIP_CHECK(hipGetLastError());
+    // 3. Levels.
+    const Tuning tuning = chooseTuning(g, dir, tc, Ktotal, geo);
+    // ORDER_PRESERVING: every list is summed by one team front to back, i.e. in the
+    // reference's order.
+    runLevels(g, dir, tc, geo, tuning, hasInit, opt.orderPreserving, Ktotal, es, rows, stream);
+    // 4. Read-out.
+    for (const auto& r : reqs) {
+        if (opt.emitAll) {
+            readoutRows(tc, rows, geo.ld, r.kOffset, r.K, plan.rank, 1, g->V, r.out, g->V, opt.accumulate, stream);
+        } else if (dir == GRGL_DIRECTION_UP) {
+            readoutRows(tc, rows, geo.ld, r.kOffset, r.K, plan.outRow, 1, g->M, r.out, g->M, opt.accumulate, stream);
+            if (opt.miss && plan.missRow) {
+                readoutRows(tc, rows, geo.ld, r.kOffset, r.K, plan.missRow, 1, g->M, opt.miss, g->M, opt.accumulate,
+                            stream);
+            }
+        } else {
+            const uint32_t per = opt.byIndividual ? g->P : 1;
+            const uint64_t C = g->N / per;
+            readoutRows(tc, rows, geo.ld, r.kOffset, r.K, plan.outRow, per, C, r.out, C, opt.accumulate, stream);
+        }
+    }
+    GRGL_HIP_CHECK(hipGetLastError());
+}
+
+Workspace* acquireWorkspace(grgl_graph* g, size_t bytes, hipStream_t stream) {
+    std::lock_guard<std::mutex> lk(g->mu);
+    Workspace* pick = nullptr;
+    for (auto& w : g->workspaces) {
+        if (w->busy || w->bytes < bytes) continue;
+        if (!w->used || w->stream == stream || hipEventQuery(w->done) == hipSuccess) {
+            pick = w.get();
+            break;
+        }
+    }
+    if (!pick) {
+        // Grow: drop an idle, finished workspace that is too small, then allocate.
+        for (auto it = g->workspaces.begin(); it != g->workspaces.end(); ++it) {
+            Workspace& w = **it;
+            if (!w.busy && w.bytes < bytes && (!w.used || hipEventQuery(w.done) == hipSuccess)) {
+                (void)hipFree(w.ptr);
+                if (w.done) (void)hipEventDestroy(w.done);
+                g->workspaces.erase(it);
+                break;
+            }
+        }
+        std::unique_ptr<Workspace> w(new Workspace());
+        const size_t sz = std::max<size_t>(bytes, 1 << 20);
+        GRGL_HIP_CHECK(hipMalloc(&w->ptr, sz));
+        w->bytes = sz;
+        const hipError_t ev = hipEventCreateWithFlags(&w->done, hipEventDisableTiming);
+        if (ev != hipSuccess) {
+            (void)hipFree(w->ptr);
+            GRGL_HIP_CHECK(ev);
+        }
+        pick = w.get();
+        g->workspaces.push_back(std::move(w));
+    }
+    pick->busy = true;
+    return pick;
+}
+
+void releaseWorkspace(grgl_graph* g, Workspace* ws, hipStream_t stream) {
+    std::lock_guard<std::mutex> lk(g->mu);
+    (void)hipEventRecord(ws->done, stream);
+    ws->stream = stream;
+    ws->used = true;
+    ws->busy = false;
+}
+
+hipStream_t acquireInternalStream(grgl_graph* g) {
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (!g->freeStreams.empty()) {
+        hipStream_t s = g->freeStreams.back();
+        g->freeStreams.pop_back();
+        return s;
+    }
+    hipStream_t s;
+    GRGL_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+void releaseInternalStream(grgl_graph* g, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g->mu);
+    g->freeStreams.push_back(s);
+}
+
+} // namespace grgl_amd
+
+using namespace grgl_amd;
+
+namespace {
+
+grgl_status translate(const std::exception& e) {
+    setLastError(e.what());
+    if (dynamic_cast<const MisuseError*>(&e)) return GRGL_ERR_MISUSE;
+    if (dynamic_cast<const NoMemError*>(&e)) return GRGL_ERR_NOMEM;
+    if (dynamic_cast<const UnsupportedError*>(&e)) return GRGL_ERR_UNSUPPORTED;
+    if (dynamic_cast<const DeviceError*>(&e)) return GRGL_ERR_DEVICE;
+    if (dynamic_cast<const std::bad_alloc*>(&e)) return GRGL_ERR_NOMEM;
+    return GRGL_ERR_MISUSE;
+}
+
+#define GRGL_API_BEGIN try {
+#define GRGL_API_END                                                                                   \
+    return GRGL_OK;                                                                                    \
+    }                                                                                                  \
+    catch (const std::exception& e) {                                                                  \
+        return translate(e);                                                                           \
+    }
+
+void uploadPlan(DeviceDirPlan& d, const DirPlan& p, uint64_t& bytes) {
+    d.numNodes = p.numNodes;
+    d.numLevels = p.numLevels;
+    d.firstPulledLevel = p.firstPulledLevel;
+    d.maxFanIn = p.maxFanIn;
+    d.numEntries = p.src.size();
+    d.levelStart = p.levelStart;
+    d.fanIn.resize(p.numNodes);
+    for (uint32_t j = 0; j < p.numNodes; j++) d.fanIn[j] = p.off[j + 1] - p.off[j];
+    d.off = uploadVec(p.off, bytes);
+    d.src = uploadVec(p.src, bytes);
+    d.order = uploadVec(p.order, bytes);
+    d.rank = uploadVec(p.rank, bytes);
+    d.outRow = uploadVec(p.outRow, bytes);
+    d.missRow = uploadVec(p.missRow, bytes);
+    d.coalByRow = uploadVec(p.coalByRow, bytes);
+    d.computeHeavy();
+}
+
+void freePlan(DeviceDirPlan& d) {
+    freeDev(d.off);
+    freeDev(d.src);
+    freeDev(d.order);
+    freeDev(d.rank);
+    freeDev(d.outRow);
+    freeDev(d.missRow);
+    freeDev(d.coalByRow);
+    d = DeviceDirPlan();
+}
+
+} // namespace
+
+GRGL_EXPORT const char* grgl_hip_last_error(void) { return g_lastError.c_str(); }
+
+GRGL_EXPORT int grgl_hip_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+GRGL_EXPORT grgl_status grgl_hip_graph_create(const grgl_graph_desc* desc, int device, grgl_graph** out) {
+    GRGL_API_BEGIN
+    if (!desc || !out) throw MisuseError("null argument");
+    *out = nullptr;
+    HostGrg h;
+    h.numSamples = desc->num_samples;
+    h.ploidy = desc->ploidy;
+    if (!desc->down_offsets) throw MisuseError("down_offsets is required");
+    h.downOff.assign(desc->down_offsets, desc->down_offsets + (size_t)desc->num_nodes + 1);
+    h.downChild.assign(desc->down_children, desc->down_children + h.downOff.back());
+    h.mutNode.assign(desc->mutation_nodes, desc->mutation_nodes + desc->num_mutations);
+    if (desc->mutation_miss_nodes) {
+        h.mutMiss.assign(desc->mutation_miss_nodes, desc->mutation_miss_nodes + desc->num_mutations);
+        bool any = false;
+        for (uint32_t x : h.mutMiss) any |= (x != kInvalidNode);
+        if (!any) h.mutMiss.clear();
+    }
+    if (desc->coal_counts) h.coal.assign(desc->coal_counts, desc->coal_counts + desc->num_nodes);
+    try {
+        validateHostGrg(h);
+    } catch (const std::runtime_error& e) {
+        throw MisuseError(e.what());
+    }
+    if ((uint64_t)h.numNodes() + 2ULL * h.numMutations() >= 0x7fffffffULL)
+        throw MisuseError("graph too large: nodes + 2*mutations must stay below 2^31");
+    DirPlan up = buildUpPlan(h);
+    DirPlan down = buildDownPlan(h);
+    int ndev = 0;
+    GRGL_HIP_CHECK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) throw MisuseError("invalid device ordinal " + std::to_string(device));
+    GRGL_HIP_CHECK(hipSetDevice(device));
+    std::unique_ptr<grgl_graph> g(new grgl_graph());
+    g->device = device;
+    g->N = h.numSamples;
+    g->V = h.numNodes();
+    g->M = h.numMutations();
+    g->P = h.ploidy;
+    g->E = h.numEdges();
+    g->hasMiss = !h.mutMiss.empty();
+    g->hasCoal = h.hasCoals();
+    if (g->hasCoal) {
+        for (uint32_t v = g->N; v < g->V; v++) {
+            if (h.coal[v] == kCoalNotSet) {
+                g->hasCoal = false;
+                break;
+            }
+        }
+    }
+    try {
+        uploadPlan(g->up, up, g->deviceBytes);
+        uploadPlan(g->down, down, g->deviceBytes);
+    } catch (...) {
+        freePlan(g->up);
+        freePlan(g->down);
+        throw;
+    }
+    *out = g.release();
+    GRGL_API_END
+}
+
+GRGL_EXPORT grgl_status grgl_hip_graph_destroy(grgl_graph* g) {
+    GRGL_API_BEGIN
+    if (!g) return GRGL_OK;
+    (void)hipSetDevice(g->device);
+    (void)hipDeviceSynchronize();
+    for (auto& kv : g->graphs) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
+    for (auto& w : g->workspaces) {
+        (void)hipFree(w->ptr);
+        if (w->done) (void)hipEventDestroy(w->done);
+    }
+    for (auto s : g->freeStreams) (void)hipStreamDestroy(s);
+    for (auto& ev : g->timingEvents) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    freePlan(g->up);
+    freePlan(g->down);
+    delete g;
+    GRGL_API_END
+}
+
+GRGL_EXPORT grgl_status grgl_hip_graph_info(const grgl_graph* g, grgl_graph_info* info) {
+    GRGL_API_BEGIN
+    if (!g || !info) throw MisuseError("null argument");
+    info->num_samples = g->N;
+    info->num_nodes = g->V;
+    info->num_mutations = g->M;
+    info->ploidy = g->P;
+    info->num_edges = g->E;
+    info->up_levels = g->up.numLevels;
+    info->down_levels = g->down.numLevels;
+    info->max_fanin_up = g->up.maxFanIn;
+    info->max_fanin_down = g->down.maxFanIn;
+    info->up_gather_entries = g->up.numEntries;
+    info->down_gather_entries = g->down.numEntries;
+    info->device_bytes = g->deviceBytes;
+    info->device = g->device;
+    GRGL_API_END
+}
+
+GRGL_EXPORT grgl_status grgl_hip_matmul(grgl_graph* g, grgl_dtype dtype, const void* input, size_t input_cols,
+                                        size_t input_rows, grgl_direction direction, void* output, size_t output_size,
+                                        int emit_all_nodes, int by_individual, const void* init,
+                                        grgl_init_mode init_mode, void* miss, uint32_t flags, void* stream) {
+    GRGL_API_BEGIN
+    if (!g) throw MisuseError("null graph");
+    const TypeClass tc = typeClassOf(dtype);
+    validateShapes(g, direction, input_cols, input_rows, output_size, emit_all_nodes != 0, by_individual != 0);
+    if (!input || !output) throw MisuseError("null input or output");
+    if (input_rows > 65536) throw MisuseError("at most 65536 input rows per call");
+    if (tc == kX8 && init_mode != GRGL_INIT_ZERO)
+        throw MisuseError("Node initialization and bit vector type cannot be mixed");
+    if (init_mode == GRGL_INIT_XTX && !g->hasCoal)
+        throw MisuseError("Coalescent counts not available for this GRG (init=XTX won't work)");
+    if ((init_mode == GRGL_INIT_VECTOR || init_mode == GRGL_INIT_MATRIX) && !init)
+        throw MisuseError("init mode requires an init buffer");
+    if (miss && emit_all_nodes) throw MisuseError("The \"miss\" parameter cannot be mixed with the \"emit_all_nodes\" parameter");
+    const uint32_t K = (uint32_t)input_rows;
+    const size_t es = dtypeSize(dtype);
+    GRGL_HIP_CHECK(hipSetDevice(g->device));
+    const bool host = (flags & GRGL_FLAG_HOST_PTRS) != 0;
+    hipStream_t s = (hipStream_t)stream;
+    bool ownStream = false;
+    if (host && s == nullptr) {
+        s = acquireInternalStream(g);
+        ownStream = true;
+    }
+    const size_t outCols = output_size / input_rows;
+    const size_t rowsBytes = rowsBufferBytes(g, direction, K, es);
+    auto align = [](size_t x) { return (x + 255) / 256 * 256; };
+    size_t total = align(rowsBytes);
+    size_t inOff = 0, outOff = 0, initOff = 0, missOff = 0;
+    const size_t inBytes = input_rows * input_cols * es, outBytes = output_size * es;
+    const size_t initBytes = init_mode == GRGL_INIT_VECTOR ? K * es : (init_mode == GRGL_INIT_MATRIX ? (size_t)K * g->V * es : 0);
+    const size_t missBytes = miss ? (size_t)K * g->M * es : 0;
+    if (host) {
+        inOff = total;
+        total += align(inBytes);
+        outOff = total;
+        total += align(outBytes);
+        initOff = total;
+        total += align(initBytes);
+        missOff = total;
+        total += align(missBytes);
+    }
+    Workspace* ws = acquireWorkspace(g, total, s);
+    try {
+        char* base = ws->ptr;
+        const void* dIn = input;
+        void* dOut = output;
+        const void* dInit = init;
+        void* dMiss = miss;
+        const bool accumulate = (flags & GRGL_FLAG_ACCUMULATE) != 0;
+        if (host) {
+            GRGL_HIP_CHECK(hipMemcpyAsync(base + inOff, input, inBytes, hipMemcpyHostToDevice, s));
+            dIn = base + inOff;
+            dOut = base + outOff;
+            if (accumulate) GRGL_HIP_CHECK(hipMemcpyAsync(dOut, output, outBytes, hipMemcpyHostToDevice, s));
+            if (initBytes) {
+                GRGL_HIP_CHECK(hipMemcpyAsync(base + initOff, init, initBytes, hipMemcpyHostToDevice, s));
+                dInit = base + initOff;
+            }
+            if (missBytes) {
+                GRGL_HIP_CHECK(hipMemcpyAsync(base + missOff, miss, missBytes, hipMemcpyHostToDevice, s));
+                dMiss = base + missOff;
+            }
+        }
+        TravOptions opt;
+        opt.emitAll = emit_all_nodes != 0;
+        opt.byIndividual = by_individual != 0;
+        opt.initMode = init_mode;
+        opt.init = dInit;
+        opt.miss = dMiss;
+        opt.accumulate = accumulate;
+        opt.orderPreserving = (flags & GRGL_FLAG_ORDER_PRESERVING) != 0;
+        std::vector<TravRequest> reqs{{K, dIn, dOut, 0}};
+        (void)outCols;
+        enqueueTraversal(g, dtype, direction, reqs, K, opt, base, s);
+        if (host) {
+            GRGL_HIP_CHECK(hipMemcpyAsync(output, dOut, outBytes, hipMemcpyDeviceToHost, s));
+            if (missBytes && direction == GRGL_DIRECTION_UP)
+                GRGL_HIP_CHECK(hipMemcpyAsync(miss, dMiss, missBytes, hipMemcpyDeviceToHost, s));
+            GRGL_HIP_CHECK(hipStreamSynchronize(s));
+        }
+    } catch (...) {
+        releaseWorkspace(g, ws, s);
+        if (ownStream) releaseInternalStream(g, s);
+        throw;
+    }
+    releaseWorkspace(g, ws, s);
+    if (ownStream) releaseInternalStream(g, s);
+    GRGL_API_END
+}
+
+GRGL_EXPORT grgl_status grgl_hip_matmul_batch(grgl_graph* g, grgl_dtype dtype, grgl_direction direction,
+                                              const grgl_request* requests, uint32_t num_requests, uint32_t flags,
+                                              void* stream) {
+    GRGL_API_BEGIN
+    if (!g) throw MisuseError("null graph");
+    if (num_requests == 0) return GRGL_OK;
+    if (!requests) throw MisuseError("null requests");
+    const size_t es = dtypeSize(dtype);
+    const size_t inCols = direction == GRGL_DIRECTION_UP ? g->N : g->M;
+    const size_t outCols = direction == GRGL_DIRECTION_UP ? g->M : g->N;
+    uint64_t Ktot = 0;
+    for (uint32_t i = 0; i < num_requests; i++) {
+        if (requests[i].rows == 0) throw MisuseError("matmul() requires non-zero dimensions.");
+        if (!requests[i].input || !requests[i].output) throw MisuseError("null input or output");
+        Ktot += requests[i].rows;
+    }
+    if (direction != GRGL_DIRECTION_UP && direction != GRGL_DIRECTION_DOWN) throw MisuseError("Invalid traversal direction");
+    if (Ktot > 65536) throw MisuseError("at most 65536 stacked rows per batch");
+    const uint32_t K = (uint32_t)Ktot;
+    GRGL_HIP_CHECK(hipSetDevice(g->device));
+    const bool host = (flags & GRGL_FLAG_HOST_PTRS) != 0;
+    hipStream_t s = (hipStream_t)stream;
+    bool ownStream = false;
+    if (host && s == nullptr) {
+        s = acquireInternalStream(g);
+        ownStream = true;
+    }
+    auto align = [](size_t x) { return (x + 255) / 256 * 256; };
+    size_t total = align(rowsBufferBytes(g, direction, K, es));
+    const size_t stageBase = total;
+    if (host) total += align((size_t)K * inCols * es) + align((size_t)K * outCols * es);
+    Workspace* ws = acquireWorkspace(g, total, s);
+    try {
+        char* base = ws->ptr;
+        std::vector<TravRequest> reqs;
+        uint32_t k0 = 0;
+        char* inStage = base + stageBase;
+        char* outStage = inStage + align((size_t)K * inCols * es);
+        const bool accumulate = (flags & GRGL_FLAG_ACCUMULATE) != 0;
+        for (uint32_t i = 0; i < num_requests; i++) {
+            const auto& r = requests[i];
+            TravRequest t{r.rows, r.input, r.output, k0};
+            if (host) {
+                char* in = inStage + (size_t)k0 * inCols * es;
+                char* out = outStage + (size_t)k0 * outCols * es;
+                GRGL_HIP_CHECK(hipMemcpyAsync(in, r.input, (size_t)r.rows * inCols * es, hipMemcpyHostToDevice, s));
+                if (accumulate)
+                    GRGL_HIP_CHECK(hipMemcpyAsync(out, r.output, (size_t)r.rows * outCols * es, hipMemcpyHostToDevice, s));
+                t.in = in;
+                t.out = out;
+            }
+            reqs.push_back(t);
+            k0 += r.rows;
+        }
+        TravOptions opt;
+        opt.accumulate = accumulate;
+        opt.orderPreserving = (flags & GRGL_FLAG_ORDER_PRESERVING) != 0;
+        enqueueTraversal(g, dtype, direction, reqs, K, opt, base, s);
+        if (host) {
+            for (uint32_t i = 0; i < num_requests; i++) {
+                GRGL_HIP_CHECK(hipMemcpyAsync(requests[i].output, reqs[i].out, (size_t)requests[i].rows * outCols * es,
+                                              hipMemcpyDeviceToHost, s));
+            }
+            GRGL_HIP_CHECK(hipStreamSynchronize(s));
+        }
+    } catch (...) {
+        releaseWorkspace(g, ws, s);
+        if (ownStream) releaseInternalStream(g, s);
+        throw;
+    }
+    releaseWorkspace(g, ws, s);
+    if (ownStream) releaseInternalStream(g, s);
+    GRGL_API_END
+}
+
+GRGL_EXPORT grgl_status grgl_hip_set_tuning(grgl_graph* g, grgl_direction direction, grgl_dtype dtype, uint32_t rows,
+                                            const grgl_tuning* tuning) {
+    GRGL_API_BEGIN
+    if (!g || !tuning) throw MisuseError("null argument");
+    if (direction != GRGL_DIRECTION_UP && direction != GRGL_DIRECTION_DOWN) throw MisuseError("Invalid traversal direction");
+    const TypeClass tc = typeClassOf(dtype);
+    if (tuning->team_lanes > 64) throw MisuseError("team_lanes must be at most 64");
+    std::lock_guard<std::mutex> lk(g->mu);
+    Tuning t;
+    t.teamLanes = tuning->team_lanes;
+    t.useGraph = tuning->use_graph;
+    g->tuning[std::make_tuple((int)direction, (int)tc, rows)] = t;
+    if (tuning->heavy_fanin) {
+        DeviceDirPlan& plan = direction == GRGL_DIRECTION_UP ? g->up : g->down;
+        plan.heavyFanin = tuning->heavy_fanin;
+        plan.computeHeavy();
+    }
+    GRGL_API_END
+}
+
+GRGL_EXPORT grgl_status grgl_hip_kernel_timing(grgl_graph* g, int enable) {
+    GRGL_API_BEGIN
+    if (!g) throw MisuseError("null graph");
+    g->timing = enable != 0;
+    GRGL_API_END
+}
+
+GRGL_EXPORT grgl_status grgl_hip_kernel_timing_read(grgl_graph* g, double* total_ms, uint64_t* launches,
+                                                    double* algorithmic_bytes, int reset) {
+    GRGL_API_BEGIN
+    if (!g) throw MisuseError("null graph");
+    GRGL_HIP_CHECK(hipSetDevice(g->device));
+    for (size_t i = 0; i < g->timingEvents.size(); i++) {
+        auto& ev = g->timingEvents[i];
+        GRGL_HIP_CHECK(hipEventSynchronize(ev.second));
+        float ms = 0.0f;
+        GRGL_HIP_CHECK(hipEventElapsedTime(&ms, ev.first, ev.second));
+        g->timedMs += ms;
+        g->timedLaunches++;
+        g->timedBytes += g->timingBytes[i];
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    g->timingEvents.clear();
+    g->timingBytes.clear();
+    if (total_ms) *total_ms = g->timedMs;
+    if (launches) *launches = g->timedLaunches;
+    if (algorithmic_bytes) *algorithmic_bytes = g->timedBytes;
+    if (reset) {
+        g->timedMs = 0.0;
+        g->timedLaunches = 0;
+        g->timedBytes = 0.0;
+    }
+    GRGL_API_END
+}
+
+GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction direction, grgl_dtype dtype, uint32_t rows,
+                                          grgl_tuning* chosen, float* best_ms) {
+    GRGL_API_BEGIN
+    if (!g) throw MisuseError("null graph");
+    if (direction != GRGL_DIRECTION_UP && direction != GRGL_DIRECTION_DOWN) throw MisuseError("Invalid traversal direction");
+    if (rows == 0 || rows > 65536) throw MisuseError("rows must be in 1..65536");
+    const TypeClass tc = typeClassOf(dtype);
+    const size_t es = dtypeSize(dtype);
+    const RowGeometry geo = rowGeometry(rows, es);
+    GRGL_HIP_CHECK(hipSetDevice(g->device));
+    const bool savedTiming = g->timing;
+    g->timing = false;
+    hipStream_t s = acquireInternalStream(g);
+    Workspace* ws = acquireWorkspace(g, rowsBufferBytes(g, direction, rows, es), s);
+    std::vector<uint32_t> tws;
+    if (geo.vb != 16) {
+        tws.push_back(1);
+    } else {
+        for (uint32_t tw = 1; tw <= std::min<uint32_t>(64, pow2ceil(geo.lanes)); tw *= 2) {
+            if ((geo.lanes + tw - 1) / tw <= 65535) tws.push_back(tw);
+        }
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    Tuning best;
+    float bestMs = 1e30f;
+    try {
+        GRGL_HIP_CHECK(hipEventCreate(&e0));
+        GRGL_HIP_CHECK(hipEventCreate(&e1));
+        GRGL_HIP_CHECK(hipMemsetAsync(ws->ptr, 0, rowsBufferBytes(g, direction, rows, es), s));
+        for (uint32_t useGraph = 0; useGraph <= 1; useGraph++) {
+            for (uint32_t tw : tws) {
+                Tuning t;
+                t.teamLanes = tw;
+                t.useGraph = useGraph;
+                runLevels(g, direction, tc, geo, t, false, false, rows, es, ws->ptr, s); // warm-up / capture
+                float trial[3];
+                for (float& ms : trial) {
+                    GRGL_HIP_CHECK(hipEventRecord(e0, s));
+                    runLevels(g, direction, tc, geo, t, false, false, rows, es, ws->ptr, s);
+                    GRGL_HIP_CHECK(hipEventRecord(e1, s));
+                    GRGL_HIP_CHECK(hipEventSynchronize(e1));
+                    GRGL_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                }
+                std::sort(trial, trial + 3);
+                if (trial[1] < bestMs) {
+                    bestMs = trial[1];
+                    best = t;
+                }
+            }
+        }
+    } catch (...) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        releaseWorkspace(g, ws, s);
+        releaseInternalStream(g, s);
+        g->timing = savedTiming;
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    releaseWorkspace(g, ws, s);
+    releaseInternalStream(g, s);
+    g->timing = savedTiming;
+    {
+        std::lock_guard<std::mutex> lk(g->mu);
+        g->tuning[std::make_tuple((int)direction, (int)tc, rows)] = best;
+    }
+    if (chosen) {
+        chosen->team_lanes = best.teamLanes;
+        chosen->heavy_fanin = (direction == GRGL_DIRECTION_UP ? g->up : g->down).heavyFanin;
+        chosen->use_graph = best.useGraph;
+    }
+    if (best_ms) *best_ms = bestMs;
+    GRGL_API_END
+}

@@ -1,0 +1,58 @@
+// Host-side immutable GRG in plain arrays: the content of the reference's CSRGRG
+// (include/grgl/grg.h:1232-1300) without the varint encoding.  Nodes 0..numSamples-1 are
+// the samples; every child ID is smaller than its parent's; children sorted ascending.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace grgl_amd {
+
+constexpr uint32_t kInvalidNode = 0x7fffffffU;    // include/grgl/grgnode.h:58
+constexpr uint32_t kCoalNotSet = 0xffffffffU;     // include/grgl/node_data.h:31
+
+struct HostGrg {
+    uint32_t numSamples = 0;
+    uint32_t ploidy = 2;
+    std::vector<uint64_t> downOff;   // numNodes + 1
+    std::vector<uint32_t> downChild;
+    std::vector<uint32_t> mutNode;   // per MutationId
+    std::vector<uint32_t> mutMiss;   // per MutationId, empty when there is no missing data
+    std::vector<double> mutPos;      // per MutationId (base-pair position)
+    std::vector<uint32_t> coal;      // per node, empty when not available
+    bool hasCoals() const { return !coal.empty(); }
+    uint32_t numNodes() const { return downOff.empty() ? 0 : (uint32_t)(downOff.size() - 1); }
+    uint64_t numEdges() const { return downOff.empty() ? 0 : downOff.back(); }
+    uint32_t numMutations() const { return (uint32_t)mutNode.size(); }
+};
+
+// Throws std::runtime_error describing the first violated invariant.
+void validateHostGrg(const HostGrg& g);
+
+struct SynthParams {
+    uint32_t numSamples = 1000;  // haplotypes
+    uint32_t numSites = 10000;   // biallelic sites (= mutations)
+    uint32_t numTrees = 1;       // independent coalescent trees, one per genome segment
+    uint32_t ploidy = 2;
+    uint64_t seed = 20261203ULL;
+    uint32_t threads = 0;        // 0 = hardware concurrency
+    double genomeLength = 1.0e8; // positions are spread over [0, genomeLength)
+};
+
+// Seeded Kingman coalescent trees (uniform pair merges, Exp(C(k,2)) inter-merge times),
+// mutations on branches with probability proportional to branch length, each tree turned
+// into GRG nodes with the reference's simplify-on-write rule (src/serialize.cpp:436-463:
+// a single-parent node is dropped unless it carries a mutation; its children move up),
+// nodes renumbered children-first.  Deterministic for a given SynthParams.
+HostGrg synthCoalescentGrg(const SynthParams& p);
+
+struct GraphStats {
+    uint64_t numEdges = 0;
+    uint32_t upLevels = 0;
+    uint32_t maxFanIn = 0;
+    std::vector<uint64_t> fanInHistogram; // bucket b counts nodes with fan-in in [2^b, 2^(b+1)), b=0 holds 0..1
+};
+GraphStats computeGraphStats(const HostGrg& g);
+
+} // namespace grgl_amd

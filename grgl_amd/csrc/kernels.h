@@ -1,0 +1,354 @@
+// gfx950 device kernels for GRG matrix multiplication.
+//
+// Data layout in HBM (one traversal):
+//   rows      node-major value buffer: row r holds the K values of one node (or one seed
+//             row), K contiguous, `ld` bytes apart.  Rows [0, numNodes) are the nodes in
+//             level order; DOWN appends one row per mutation (input seeds) and, with
+//             missing data, one per mutation for the missingness seeds.
+//   off/src   per-row gather lists (u32 row indices), level by level.
+//
+// Kernels:
+//   pull_level_kernel  one topological level: row j = [self +] sum of rows src[off[j]..].
+//                      A "team" of TW lanes (TW*VB bytes of the row) owns a node and sums
+//                      its list front to back.  Rows whose list is longer than the heavy
+//                      threshold take a whole block: entry i goes to partial (i mod 64),
+//                      each partial sums its entries in order, and the 64 partials meet in
+//                      a fixed binary tree (strides 32,16,...,1).  Both schemes do the same
+//                      arithmetic whatever TW or the K tiling is, so a request computed
+//                      alone or stacked with others gives identical bits.
+//   seed_rows_kernel   transposes a row-major K x C matrix into rows (optionally through a
+//                      column map), adding into them or overwriting.
+//   readout_kernel     transposes rows back into a row-major K x C output, summing `per`
+//                      rows per output column in order (by-individual read-out).
+//   init_rows_kernel   fills rows with 0, a per-column constant, or 2*coal counts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace grgl_amd {
+
+constexpr uint32_t kInvalidRow = 0x7fffffffU;
+constexpr int kBlock = 256;
+constexpr int kHeavyPartials = 64;
+
+template <typename T, bool XOR> __device__ __forceinline__ T combine(T a, T b) {
+    if constexpr (XOR) {
+        return (T)(a ^ b);
+    } else {
+        return (T)(a + b);
+    }
+}
+
+template <int VB> struct RawVec;
+template <> struct RawVec<4> { using type = uint32_t; };
+template <> struct RawVec<8> { using type = uint2; };
+template <> struct RawVec<16> { using type = uint4; };
+
+template <typename T, int VB> struct Pack {
+    static constexpr int N = VB / (int)sizeof(T);
+    T v[N];
+};
+
+template <typename T, int VB> __device__ __forceinline__ Pack<T, VB> load_pack(const char* p) {
+    typename RawVec<VB>::type raw = *reinterpret_cast<const typename RawVec<VB>::type*>(p);
+    Pack<T, VB> out;
+    __builtin_memcpy(&out, &raw, VB);
+    return out;
+}
+
+template <typename T, int VB> __device__ __forceinline__ void store_pack(char* p, const Pack<T, VB>& v) {
+    typename RawVec<VB>::type raw;
+    __builtin_memcpy(&raw, &v, VB);
+    *reinterpret_cast<typename RawVec<VB>::type*>(p) = raw;
+}
+
+template <typename T, int VB> __device__ __forceinline__ Pack<T, VB> zero_pack() {
+    Pack<T, VB> z;
+#pragma unroll
+    for (int i = 0; i < Pack<T, VB>::N; i++) z.v[i] = (T)0;
+    return z;
+}
+
+// Lane-wise wrapping add of the 8- or 16-bit fields of two 32-bit words: the low bits of
+// every field are added with the field's top bit cleared, so no carry crosses a field,
+// and the top bit is restored by XOR (a + b mod 2^w per field).
+template <typename T> __device__ __forceinline__ uint32_t swar_add(uint32_t x, uint32_t y) {
+    constexpr uint32_t H = sizeof(T) == 1 ? 0x80808080U : 0x80008000U;
+    return ((x & ~H) + (y & ~H)) ^ ((x ^ y) & H);
+}
+
+template <typename T, int VB, bool XOR>
+__device__ __forceinline__ void acc_pack(Pack<T, VB>& a, const Pack<T, VB>& b) {
+    if constexpr (sizeof(T) < 4) {
+        uint32_t wa[VB / 4], wb[VB / 4];
+        __builtin_memcpy(wa, &a, VB);
+        __builtin_memcpy(wb, &b, VB);
+#pragma unroll
+        for (int i = 0; i < VB / 4; i++) wa[i] = XOR ? (wa[i] ^ wb[i]) : swar_add<T>(wa[i], wb[i]);
+        __builtin_memcpy(&a, wa, VB);
+    } else {
+#pragma unroll
+        for (int i = 0; i < Pack<T, VB>::N; i++) a.v[i] = combine<T, XOR>(a.v[i], b.v[i]);
+    }
+}
+
+struct PullArgs {
+    char* rows;
+    uint64_t ld;          // row stride, bytes
+    const uint32_t* off;
+    const uint32_t* src;
+    uint32_t levelBegin;  // first row of the level
+    uint32_t heavyCount;  // rows at the front of the level handled one per block
+    uint32_t levelEnd;
+    uint32_t rowBytes;    // bytes of each row holding values (multiple of VB)
+    uint32_t readSelf;    // 1: start from the row's current value (node initialisation)
+};
+
+// Light rows: one team per row, list summed front to back, four loads in flight.
+template <typename T, int VB, int TW, bool XOR>
+__device__ __forceinline__ void pull_light(const PullArgs& a, uint32_t row, uint64_t colByte, bool active) {
+    using P = Pack<T, VB>;
+    const uint32_t b = a.off[row], e = a.off[row + 1];
+    P acc = zero_pack<T, VB>();
+    if (!active) return;
+    if (a.readSelf) acc = load_pack<T, VB>(a.rows + (uint64_t)row * a.ld + colByte);
+    uint32_t i = b;
+    for (; i + 4 <= e; i += 4) {
+        const uint32_t s0 = a.src[i], s1 = a.src[i + 1], s2 = a.src[i + 2], s3 = a.src[i + 3];
+        const P x0 = load_pack<T, VB>(a.rows + (uint64_t)s0 * a.ld + colByte);
+        const P x1 = load_pack<T, VB>(a.rows + (uint64_t)s1 * a.ld + colByte);
+        const P x2 = load_pack<T, VB>(a.rows + (uint64_t)s2 * a.ld + colByte);
+        const P x3 = load_pack<T, VB>(a.rows + (uint64_t)s3 * a.ld + colByte);
+        acc_pack<T, VB, XOR>(acc, x0);
+        acc_pack<T, VB, XOR>(acc, x1);
+        acc_pack<T, VB, XOR>(acc, x2);
+        acc_pack<T, VB, XOR>(acc, x3);
+    }
+    for (; i < e; i++) {
+        const P x = load_pack<T, VB>(a.rows + (uint64_t)a.src[i] * a.ld + colByte);
+        acc_pack<T, VB, XOR>(acc, x);
+    }
+    store_pack<T, VB>(a.rows + (uint64_t)row * a.ld + colByte, acc);
+}
+
+// Heavy rows: the whole block on one row, 64 interleaved partials, fixed reduction tree.
+template <typename T, int VB, int TW, bool XOR>
+__device__ __forceinline__ void pull_heavy(const PullArgs& a, uint32_t row, uint64_t colByte, bool active,
+                                           uint32_t team, uint32_t lane) {
+    using P = Pack<T, VB>;
+    constexpr int NT = kBlock / TW;                          // teams in the block
+    constexpr int PPT = NT >= kHeavyPartials ? 1 : kHeavyPartials / NT; // partials per team
+    constexpr int ACTIVE_TEAMS = NT >= kHeavyPartials ? kHeavyPartials : NT;
+    __shared__ char lds[ACTIVE_TEAMS * TW * VB];
+    const uint32_t b = a.off[row], e = a.off[row + 1];
+    P part[PPT];
+#pragma unroll
+    for (int q = 0; q < PPT; q++) part[q] = zero_pack<T, VB>();
+    constexpr int A = ACTIVE_TEAMS;                          // A * PPT == 64 always
+    const bool working = active && team < (uint32_t)A;
+    if (working) {
+        // Entry b + i belongs to partial (i mod 64); team t owns partials t + q*A.
+        // Walking i = t, t+A, t+2A, ... visits partial q = (step mod PPT), each partial's
+        // entries in increasing order.
+        uint32_t i = b + team;
+        for (; i + (uint32_t)(PPT - 1) * A < e; i += (uint32_t)PPT * A) {
+            P x[PPT];
+#pragma unroll
+            for (int q = 0; q < PPT; q++) x[q] = load_pack<T, VB>(a.rows + (uint64_t)a.src[i + q * A] * a.ld + colByte);
+#pragma unroll
+            for (int q = 0; q < PPT; q++) acc_pack<T, VB, XOR>(part[q], x[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < PPT; q++) {
+            if (i + (uint32_t)q * A < e) {
+                const P x = load_pack<T, VB>(a.rows + (uint64_t)a.src[i + q * A] * a.ld + colByte);
+                acc_pack<T, VB, XOR>(part[q], x);
+            }
+        }
+        // In-register tree steps: strides 32 .. A pair partials held by the same team
+        // (partial t + q*A with partial t + (q + s/A)*A).
+#pragma unroll
+        for (int s = kHeavyPartials / 2; s >= A; s /= 2) {
+#pragma unroll
+            for (int q = 0; q < s / A; q++) acc_pack<T, VB, XOR>(part[q], part[q + s / A]);
+        }
+        store_pack<T, VB>(lds + ((uint64_t)team * TW + lane) * VB, part[0]);
+    }
+    __syncthreads();
+    // Remaining tree steps across teams through LDS: strides A/2 .. 1.
+    for (int s = A / 2; s >= 1; s /= 2) {
+        if (working && team < (uint32_t)s) {
+            P mine = load_pack<T, VB>(lds + ((uint64_t)team * TW + lane) * VB);
+            const P other = load_pack<T, VB>(lds + ((uint64_t)(team + s) * TW + lane) * VB);
+            acc_pack<T, VB, XOR>(mine, other);
+            store_pack<T, VB>(lds + ((uint64_t)team * TW + lane) * VB, mine);
+        }
+        __syncthreads();
+    }
+    if (working && team == 0) {
+        P res = load_pack<T, VB>(lds + (uint64_t)lane * VB);
+        if (a.readSelf) {
+            P self = load_pack<T, VB>(a.rows + (uint64_t)row * a.ld + colByte);
+            acc_pack<T, VB, XOR>(self, res);
+            res = self;
+        }
+        store_pack<T, VB>(a.rows + (uint64_t)row * a.ld + colByte, res);
+    }
+}
+
+template <typename T, int VB, int TW, bool XOR>
+__global__ __launch_bounds__(kBlock) void pull_level_kernel(PullArgs a) {
+    const uint32_t lane = threadIdx.x % TW;
+    const uint32_t team = threadIdx.x / TW;
+    const uint64_t colByte = ((uint64_t)blockIdx.y * TW + lane) * VB;
+    const bool active = colByte < a.rowBytes;
+    if (blockIdx.x < a.heavyCount) {
+        pull_heavy<T, VB, TW, XOR>(a, a.levelBegin + blockIdx.x, colByte, active, team, lane);
+    } else {
+        const uint32_t row = a.levelBegin + a.heavyCount + (blockIdx.x - a.heavyCount) * (kBlock / TW) + team;
+        if (row < a.levelEnd) pull_light<T, VB, TW, XOR>(a, row, colByte, active);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Transposes between row-major K x C matrices and node rows, through a 1024-element LDS
+// tile of TK (k) x TD (columns), TK = 2^tkShift <= 32, TD = 1024 / TK.
+// ---------------------------------------------------------------------------------------
+struct SeedArgs {
+    const void* in;       // K x inCols, row-major
+    uint64_t inCols;
+    uint32_t K;
+    char* rows;
+    uint64_t ld;          // bytes
+    uint32_t rowBase;
+    uint32_t kOffset;     // first column of this request inside the row
+    uint32_t D;           // destination rows
+    uint32_t per;         // destination row d reads column d / per (no map)
+    const uint32_t* colMap; // or column colMap[d]
+    uint32_t accumulate;
+    uint32_t tkShift;
+    uint32_t normalizeBool;
+};
+
+template <typename T, bool XOR> __global__ __launch_bounds__(kBlock) void seed_rows_kernel(SeedArgs a) {
+    __shared__ T tile[1024 + 32];
+    const uint32_t TK = 1U << a.tkShift, tdShift = 10 - a.tkShift, TD = 1U << tdShift;
+    const uint64_t d0 = (uint64_t)blockIdx.x * TD;
+    const uint32_t k0 = blockIdx.y * TK;
+    const T* in = (const T*)a.in;
+    for (uint32_t e = threadIdx.x; e < 1024; e += kBlock) {
+        const uint32_t k = e >> tdShift, d = e & (TD - 1);
+        const uint64_t gd = d0 + d;
+        const uint32_t gk = k0 + k;
+        if (gk < a.K && gd < a.D) {
+            const uint64_t c = a.colMap ? (uint64_t)a.colMap[gd] : gd / a.per;
+            T v = in[(uint64_t)gk * a.inCols + c];
+            if (a.normalizeBool) v = (T)(v != (T)0);
+            tile[k * (TD + 1) + d] = v;
+        }
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < 1024; e += kBlock) {
+        const uint32_t d = e >> a.tkShift, k = e & (TK - 1);
+        const uint64_t gd = d0 + d;
+        const uint32_t gk = k0 + k;
+        if (gk < a.K && gd < a.D) {
+            T* dst = (T*)(a.rows + (uint64_t)(a.rowBase + gd) * a.ld) + a.kOffset + gk;
+            const T v = tile[k * (TD + 1) + d];
+            *dst = a.accumulate ? combine<T, XOR>(*dst, v) : v;
+        }
+    }
+}
+
+struct ReadoutArgs {
+    const char* rows;
+    uint64_t ld;          // bytes
+    uint32_t kOffset;
+    uint32_t K;
+    const uint32_t* map;  // C * per row indices (kInvalidRow = no contribution)
+    uint32_t per;
+    uint64_t C;
+    void* out;            // K x outCols, row-major
+    uint64_t outCols;
+    uint32_t accumulate;
+    uint32_t tkShift;
+};
+
+template <typename T, bool XOR> __global__ __launch_bounds__(kBlock) void readout_kernel(ReadoutArgs a) {
+    __shared__ T tile[1024 + 32];
+    const uint32_t TK = 1U << a.tkShift, tdShift = 10 - a.tkShift, TD = 1U << tdShift;
+    const uint64_t c0 = (uint64_t)blockIdx.x * TD;
+    const uint32_t k0 = blockIdx.y * TK;
+    T* out = (T*)a.out;
+    T acc[1024 / kBlock];
+    bool any[1024 / kBlock];
+#pragma unroll
+    for (int i = 0; i < 1024 / kBlock; i++) {
+        const uint32_t e = threadIdx.x + i * kBlock;
+        const uint32_t k = e >> tdShift, c = e & (TD - 1);
+        any[i] = false;
+        acc[i] = (T)0;
+        if (a.accumulate && k0 + k < a.K && c0 + c < a.C) acc[i] = out[(uint64_t)(k0 + k) * a.outCols + c0 + c];
+    }
+    for (uint32_t j = 0; j < a.per; j++) {
+        for (uint32_t e = threadIdx.x; e < 1024; e += kBlock) {
+            const uint32_t c = e >> a.tkShift, k = e & (TK - 1);
+            const uint64_t gc = c0 + c;
+            const uint32_t gk = k0 + k;
+            T v = (T)0;
+            if (gk < a.K && gc < a.C) {
+                const uint32_t r = a.map[gc * a.per + j];
+                if (r != kInvalidRow) v = *((const T*)(a.rows + (uint64_t)r * a.ld) + a.kOffset + gk);
+            }
+            tile[k * (TD + 1) + c] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 1024 / kBlock; i++) {
+            const uint32_t e = threadIdx.x + i * kBlock;
+            const uint32_t k = e >> tdShift, c = e & (TD - 1);
+            if (k0 + k < a.K && c0 + c < a.C && a.map[(c0 + c) * a.per + j] != kInvalidRow) {
+                acc[i] = combine<T, XOR>(acc[i], tile[k * (TD + 1) + c]);
+                any[i] = true;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 1024 / kBlock; i++) {
+        const uint32_t e = threadIdx.x + i * kBlock;
+        const uint32_t k = e >> tdShift, c = e & (TD - 1);
+        if (k0 + k < a.K && c0 + c < a.C && (any[i] || !a.accumulate)) {
+            out[(uint64_t)(k0 + k) * a.outCols + c0 + c] = acc[i];
+        }
+    }
+}
+
+enum InitKind : uint32_t { kInitZero = 0, kInitVector = 1, kInitXtx = 3 };
+
+struct InitArgs {
+    char* rows;
+    uint64_t ld;        // bytes
+    uint32_t rowBegin;
+    uint32_t rowEnd;
+    uint32_t kOffset;
+    uint32_t K;
+    uint32_t kind;
+    const void* vec;            // K values (kInitVector)
+    const uint32_t* coalByRow;  // per row (kInitXtx)
+};
+
+template <typename T> __global__ __launch_bounds__(kBlock) void init_rows_kernel(InitArgs a) {
+    const uint64_t n = (uint64_t)(a.rowEnd - a.rowBegin) * a.K;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t r = a.rowBegin + (uint32_t)(i / a.K);
+        const uint32_t k = (uint32_t)(i % a.K);
+        T v = (T)0;
+        if (a.kind == kInitVector) v = ((const T*)a.vec)[k];
+        else if (a.kind == kInitXtx) v = (T)(a.coalByRow[r] * 2U);
+        *((T*)(a.rows + (uint64_t)r * a.ld) + a.kOffset + k) = v;
+    }
+}
+
+} // namespace grgl_amd

@@ -1,0 +1,497 @@
+// _grgl_amd: pybind11 bindings with the pygrgl surface for graph matrix multiplication.
+//
+// Mirrors src/python/_grgl.cpp of the reference for this path: matmul (:187-241, :1159-1230),
+// dispatchMult (:100-185), dot_product (:67-98, :1141-1157), TraversalDirection (:750-757),
+// the GRG properties callers use (:354-460, :499-560, :656-710) and the module constants
+// (:1266-1269).  All arithmetic runs on the GPU through the C ABI in grgl_hip.h; there is
+// no CPU fallback: without a usable device every matmul raises RuntimeError.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <stdexcept>
+
+#include "grgl_hip.h"
+#include "host_graph.h"
+#include "levelize.h"
+
+namespace py = pybind11;
+using namespace grgl_amd;
+
+namespace {
+
+constexpr size_t kNoUpEdges = std::numeric_limits<size_t>::max(); // include/grgl/grg.h:50
+
+#define API_CHECK(cond, msg)                                                                           \
+    do {                                                                                               \
+        if (!(cond)) {                                                                                 \
+            std::ostringstream _ss;                                                                    \
+            _ss << msg;                                                                                \
+            throw std::runtime_error(_ss.str());                                                       \
+        }                                                                                              \
+    } while (0)
+
+void checkStatus(grgl_status st) {
+    if (st != GRGL_OK) throw std::runtime_error(grgl_hip_last_error());
+}
+
+int g_device = 0;
+
+// Traversal direction with the reference's numeric values (include/grgl/visitor.h:26).
+enum class Direction : int { DOWN = 1, UP = 2 };
+
+struct PyGRG {
+    HostGrg h;
+    std::mutex mu;
+    std::map<int, grgl_graph*> dev;
+    bool upBuilt = false;
+    std::vector<uint64_t> upOff;
+    std::vector<uint32_t> upPar;
+    std::vector<std::pair<uint32_t, uint32_t>> nodeMut; // (node, mutId) sorted, invalid first
+
+    ~PyGRG() {
+        for (auto& kv : dev) grgl_hip_graph_destroy(kv.second);
+    }
+
+    grgl_graph* deviceGraph() {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = dev.find(g_device);
+        if (it != dev.end()) return it->second;
+        grgl_graph_desc d{};
+        d.num_samples = h.numSamples;
+        d.num_nodes = h.numNodes();
+        d.ploidy = h.ploidy;
+        d.num_mutations = h.numMutations();
+        d.down_offsets = h.downOff.data();
+        d.down_children = h.downChild.data();
+        d.mutation_nodes = h.mutNode.data();
+        d.mutation_miss_nodes = h.mutMiss.empty() ? nullptr : h.mutMiss.data();
+        d.coal_counts = h.coal.empty() ? nullptr : h.coal.data();
+        grgl_graph* g = nullptr;
+        checkStatus(grgl_hip_graph_create(&d, g_device, &g));
+        dev[g_device] = g;
+        return g;
+    }
+
+    void buildUp() {
+        std::lock_guard<std::mutex> lk(mu);
+        if (upBuilt) return;
+        const uint32_t V = h.numNodes();
+        upOff.assign(V + 1, 0);
+        for (uint32_t c : h.downChild) upOff[c + 1]++;
+        for (uint32_t v = 0; v < V; v++) upOff[v + 1] += upOff[v];
+        upPar.resize(h.numEdges());
+        std::vector<uint64_t> cur(upOff.begin(), upOff.end() - 1);
+        for (uint32_t p = 0; p < V; p++)
+            for (uint64_t e = h.downOff[p]; e < h.downOff[p + 1]; e++) upPar[cur[h.downChild[e]]++] = p;
+        nodeMut.clear();
+        for (uint32_t m = 0; m < h.numMutations(); m++) nodeMut.push_back({h.mutNode[m], m});
+        std::sort(nodeMut.begin(), nodeMut.end(), [](const auto& a, const auto& b) {
+            const bool ai = a.first == kInvalidNode, bi = b.first == kInvalidNode;
+            if (ai != bi) return ai;
+            return a < b;
+        });
+        upBuilt = true;
+    }
+};
+using PyGRGPtr = std::shared_ptr<PyGRG>;
+
+grgl_dtype dtypeOf(py::handle input, bool& ok) {
+    ok = true;
+    if (py::isinstance<py::array_t<double>>(input)) return GRGL_F64;
+    if (py::isinstance<py::array_t<float>>(input)) return GRGL_F32;
+    if (py::isinstance<py::array_t<int64_t>>(input)) return GRGL_I64;
+    if (py::isinstance<py::array_t<int32_t>>(input)) return GRGL_I32;
+    if (py::isinstance<py::array_t<int16_t>>(input)) return GRGL_I16;
+    if (py::isinstance<py::array_t<int8_t>>(input)) return GRGL_I8;
+    if (py::isinstance<py::array_t<uint64_t>>(input)) return GRGL_U64;
+    if (py::isinstance<py::array_t<uint32_t>>(input)) return GRGL_U32;
+    if (py::isinstance<py::array_t<uint16_t>>(input)) return GRGL_U16;
+    if (py::isinstance<py::array_t<uint8_t>>(input)) return GRGL_U8;
+    if (py::isinstance<py::array_t<bool>>(input)) return GRGL_BOOL;
+    ok = false;
+    return GRGL_F64;
+}
+
+py::dtype numpyDtype(grgl_dtype dt) {
+    switch (dt) {
+    case GRGL_F64: return py::dtype::of<double>();
+    case GRGL_F32: return py::dtype::of<float>();
+    case GRGL_I64: return py::dtype::of<int64_t>();
+    case GRGL_I32: return py::dtype::of<int32_t>();
+    case GRGL_I16: return py::dtype::of<int16_t>();
+    case GRGL_I8: return py::dtype::of<int8_t>();
+    case GRGL_U64: return py::dtype::of<uint64_t>();
+    case GRGL_U32: return py::dtype::of<uint32_t>();
+    case GRGL_U16: return py::dtype::of<uint16_t>();
+    case GRGL_U8: return py::dtype::of<uint8_t>();
+    default: return py::dtype::of<bool>();
+    }
+}
+
+bool sameDtype(py::handle h, grgl_dtype dt) {
+    bool ok = false;
+    const grgl_dtype got = dtypeOf(h, ok);
+    return ok && got == dt;
+}
+
+// dispatchMult (src/python/_grgl.cpp:100-185): zeroed result, init/miss parsing, GIL released.
+py::array dispatchMult(PyGRG& grg, grgl_dtype dt, const py::array& arr, size_t rows, size_t inCols, size_t outCols,
+                       Direction direction, bool emitAllNodes, bool byIndividual, py::handle init, py::handle miss,
+                       uint32_t extraFlags) {
+    const size_t outSize = rows * outCols;
+    py::array result(numpyDtype(dt), std::vector<py::ssize_t>{(py::ssize_t)rows, (py::ssize_t)outCols});
+    std::memset(result.mutable_data(), 0, outSize * result.itemsize());
+
+    py::array initArr;
+    const void* initPtr = nullptr;
+    grgl_init_mode initMode = GRGL_INIT_ZERO;
+    if (init.is_none()) {
+        ;
+    } else if (py::isinstance<py::str>(init)) {
+        const std::string strval = init.cast<std::string>();
+        API_CHECK(strval == "xtx", "Unexpected init value: " << strval);
+        initMode = GRGL_INIT_XTX;
+    } else {
+        initArr = py::array::ensure(init, py::array::c_style | py::array::forcecast);
+        API_CHECK(sameDtype(init, dt), "The init matrix must match the dtype of the input matrix. Got: "
+                                           << std::string(py::str(initArr.dtype())));
+        initPtr = initArr.data();
+        if (initArr.ndim() == 1) {
+            API_CHECK((size_t)initArr.shape(0) == rows,
+                      "If init has a single dimension, it must match the number of rows in the input matrix");
+            initMode = GRGL_INIT_VECTOR;
+        }
+        if (initArr.ndim() == 2) {
+            API_CHECK((size_t)initArr.shape(0) == rows && (size_t)initArr.shape(1) == grg.h.numNodes(),
+                      "If init is a matrix, it must match the dimensions ROW x NODES");
+            initMode = GRGL_INIT_MATRIX;
+        }
+    }
+
+    py::array missArr;
+    void* missPtr = nullptr;
+    if (!miss.is_none()) {
+        API_CHECK(!emitAllNodes, "The \"miss\" parameter cannot be mixed with the \"emit_all_nodes\" parameter");
+        missArr = py::array::ensure(miss, py::array::c_style | py::array::forcecast);
+        API_CHECK(sameDtype(miss, dt), "The \"miss\" input must be a matrix with dtype matching the input matrix. Got: "
+                                           << std::string(py::str(missArr.dtype())));
+        API_CHECK(missArr.ndim() == 2, "\"miss\" must be a two-dimension numpy array (matrix). ndim=" << missArr.ndim());
+        const size_t missRows = missArr.shape(0), missCols = missArr.shape(1);
+        API_CHECK(missRows == rows, "\"miss\" has " << missRows << " rows, but must match the input and output matrices ("
+                                                    << rows << ")");
+        if (direction == Direction::DOWN) {
+            API_CHECK(missCols == inCols,
+                      "The \"miss\" matrix must match the number of columns in the input matrix. Got: " << missCols);
+        } else {
+            API_CHECK(missCols == outCols,
+                      "The \"miss\" matrix must match the number of columns in the output matrix. Got: " << missCols);
+        }
+        missPtr = missArr.mutable_data();
+    }
+
+    grgl_graph* g = grg.deviceGraph();
+    const void* in = arr.data();
+    void* out = result.mutable_data();
+    grgl_status st;
+    {
+        py::gil_scoped_release release;
+        st = grgl_hip_matmul(g, dt, in, inCols, rows, (grgl_direction)direction, out, outSize, emitAllNodes ? 1 : 0,
+                             byIndividual ? 1 : 0, initPtr, initMode, missPtr,
+                             GRGL_FLAG_HOST_PTRS | GRGL_FLAG_ACCUMULATE | extraFlags, nullptr);
+    }
+    checkStatus(st);
+    return result;
+}
+
+// matMul (src/python/_grgl.cpp:187-241).
+py::array matMul(PyGRGPtr& grg, py::handle input, Direction direction, bool emitAllNodes, bool byIndividual,
+                 py::handle init, py::handle miss, bool orderPreserving) {
+    py::array arr = py::array::ensure(input, py::array::c_style | py::array::forcecast);
+    API_CHECK(arr, "matmul() requires a numpy array as input.");
+    API_CHECK(arr.ndim() == 2, "matmul() only support two dimensional numpy arrays as input.");
+    const size_t rows = arr.shape(0), cols = arr.shape(1);
+    API_CHECK(rows != 0 && cols != 0, "matmul() requires non-zero dimensions.");
+    const size_t numSamples = byIndividual ? grg->h.numSamples / grg->h.ploidy : grg->h.numSamples;
+    const size_t outCols =
+        emitAllNodes ? grg->h.numNodes() : (direction == Direction::DOWN ? numSamples : grg->h.numMutations());
+    bool ok = false;
+    const grgl_dtype dt = dtypeOf(input, ok);
+    API_CHECK(ok, "Unsupported numpy dtype: " << std::string(py::str(arr.dtype())));
+    if (dt == GRGL_BOOL) API_CHECK(init.is_none(), "init=... is not supported with boolean dtype");
+    return dispatchMult(*grg, dt, arr, rows, cols, outCols, direction, emitAllNodes, byIndividual, init, miss,
+                        orderPreserving ? GRGL_FLAG_ORDER_PRESERVING : 0U);
+}
+
+// dotProduct (src/python/_grgl.cpp:67-98).
+py::array dotProduct(PyGRGPtr& grg, py::handle input, Direction direction) {
+    py::array arr = py::array::ensure(input, py::array::c_style | py::array::forcecast);
+    API_CHECK(arr, "dot_product() requires a numpy array as input.");
+    API_CHECK(arr.ndim() == 1, "dot_product() only support one-dimensional numpy arrays as input.");
+    const size_t cols = arr.shape(0);
+    API_CHECK(cols != 0, "dot_product() requires non-empty array input.");
+    const size_t outCols = direction == Direction::DOWN ? grg->h.numSamples : grg->h.numMutations();
+    bool ok = false;
+    const grgl_dtype dt = dtypeOf(input, ok);
+    API_CHECK(ok && (dt == GRGL_F64 || dt == GRGL_F32 || dt == GRGL_I64 || dt == GRGL_I32),
+              "Unsupported numpy dtype: " << std::string(py::str(arr.dtype())));
+    py::array result(numpyDtype(dt), std::vector<py::ssize_t>{(py::ssize_t)outCols});
+    std::memset(result.mutable_data(), 0, outCols * result.itemsize());
+    grgl_graph* g = grg->deviceGraph();
+    const void* in = arr.data();
+    void* out = result.mutable_data();
+    grgl_status st;
+    {
+        py::gil_scoped_release release;
+        st = grgl_hip_matmul(g, dt, in, cols, 1, (grgl_direction)direction, out, outCols, 0, 0, nullptr, GRGL_INIT_ZERO,
+                             nullptr, GRGL_FLAG_HOST_PTRS | GRGL_FLAG_ACCUMULATE, nullptr);
+    }
+    checkStatus(st);
+    return result;
+}
+
+template <typename T> std::vector<T> toVec(py::handle h, const char* what) {
+    py::array_t<T, py::array::c_style | py::array::forcecast> a = py::array_t<T, py::array::c_style | py::array::forcecast>::ensure(h);
+    API_CHECK(a, what << " must be array-like");
+    API_CHECK(a.ndim() == 1, what << " must be one-dimensional");
+    return std::vector<T>(a.data(), a.data() + a.size());
+}
+
+template <typename T> py::array_t<T> toArray(const std::vector<T>& v) {
+    py::array_t<T> a((py::ssize_t)v.size());
+    if (!v.empty()) std::memcpy(a.mutable_data(), v.data(), v.size() * sizeof(T));
+    return a;
+}
+
+PyGRGPtr fromArrays(uint32_t numSamples, uint32_t ploidy, py::handle downOffsets, py::handle downChildren,
+                    py::handle mutationNodes, py::handle mutationMiss, py::handle coalCounts, py::handle positions) {
+    auto g = std::make_shared<PyGRG>();
+    g->h.numSamples = numSamples;
+    g->h.ploidy = ploidy;
+    g->h.downOff = toVec<uint64_t>(downOffsets, "down_offsets");
+    g->h.downChild = toVec<uint32_t>(downChildren, "down_children");
+    g->h.mutNode = toVec<uint32_t>(mutationNodes, "mutation_nodes");
+    if (!mutationMiss.is_none()) g->h.mutMiss = toVec<uint32_t>(mutationMiss, "mutation_miss_nodes");
+    if (!coalCounts.is_none()) g->h.coal = toVec<uint32_t>(coalCounts, "coal_counts");
+    if (!positions.is_none()) g->h.mutPos = toVec<double>(positions, "positions");
+    API_CHECK(!g->h.downOff.empty(), "down_offsets must have num_nodes + 1 entries");
+    validateHostGrg(g->h);
+    return g;
+}
+
+} // namespace
+
+PYBIND11_MODULE(_grgl_amd, m) {
+    m.doc() = "MI355X-native GRG matrix multiplication (pygrgl-compatible surface)";
+
+    py::enum_<Direction>(m, "TraversalDirection")
+        .value("DOWN", Direction::DOWN, "Traverse the graph \"down\" edges.")
+        .value("UP", Direction::UP, "Traverse the graph via \"up\" edges.")
+        .export_values();
+
+    py::class_<PyGRG, PyGRGPtr>(m, "GRG")
+        .def_property_readonly("num_samples", [](PyGRG& g) { return g.h.numSamples; })
+        .def_property_readonly("num_individuals", [](PyGRG& g) { return g.h.numSamples / g.h.ploidy; })
+        .def_property_readonly("ploidy", [](PyGRG& g) { return g.h.ploidy; })
+        .def_property_readonly("shape",
+                               [](PyGRG& g) { return std::make_pair(g.h.numSamples / g.h.ploidy, g.h.numMutations()); })
+        .def_property_readonly("num_nodes", [](PyGRG& g) { return g.h.numNodes(); })
+        .def_property_readonly("num_edges", [](PyGRG& g) { return g.h.numEdges(); })
+        .def_property_readonly("num_mutations", [](PyGRG& g) { return g.h.numMutations(); })
+        .def_property_readonly("has_missing_data", [](PyGRG& g) { return !g.h.mutMiss.empty(); })
+        .def_property_readonly("has_individual_coals", [](PyGRG& g) { return g.h.hasCoals(); })
+        .def_property_readonly("nodes_are_ordered", [](PyGRG&) { return true; })
+        .def_property_readonly("is_mutable", [](PyGRG&) { return false; })
+        .def("is_sample", [](PyGRG& g, uint32_t n) { return n < g.h.numSamples; }, py::arg("node_id"))
+        .def("num_down_edges",
+             [](PyGRG& g, uint32_t n) {
+                 API_CHECK(n < g.h.numNodes(), "Index out of bounds (CSR data)");
+                 return (size_t)(g.h.downOff[n + 1] - g.h.downOff[n]);
+             },
+             py::arg("node_id"))
+        .def("get_down_edges",
+             [](PyGRG& g, uint32_t n) {
+                 API_CHECK(n < g.h.numNodes(), "Index out of bounds (CSR data)");
+                 return std::vector<uint32_t>(g.h.downChild.begin() + g.h.downOff[n],
+                                              g.h.downChild.begin() + g.h.downOff[n + 1]);
+             },
+             py::arg("node_id"))
+        .def("num_up_edges",
+             [](PyGRG& g, uint32_t n) {
+                 g.buildUp();
+                 API_CHECK(n < g.h.numNodes(), "Index out of bounds (CSR data)");
+                 return (size_t)(g.upOff[n + 1] - g.upOff[n]);
+             },
+             py::arg("node_id"))
+        .def("get_up_edges",
+             [](PyGRG& g, uint32_t n) {
+                 g.buildUp();
+                 API_CHECK(n < g.h.numNodes(), "Index out of bounds (CSR data)");
+                 return std::vector<uint32_t>(g.upPar.begin() + g.upOff[n], g.upPar.begin() + g.upOff[n + 1]);
+             },
+             py::arg("node_id"))
+        .def("get_mutation_node_pairs",
+             [](PyGRG& g) {
+                 std::vector<std::pair<uint32_t, uint32_t>> r;
+                 for (uint32_t m = 0; m < g.h.numMutations(); m++) r.push_back({m, g.h.mutNode[m]});
+                 return r;
+             })
+        .def("get_mutation_node_miss",
+             [](PyGRG& g) {
+                 std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> r;
+                 for (uint32_t m = 0; m < g.h.numMutations(); m++)
+                     r.push_back({m, g.h.mutNode[m], g.h.mutMiss.empty() ? kInvalidNode : g.h.mutMiss[m]});
+                 return r;
+             })
+        .def("get_mutations_for_node",
+             [](PyGRG& g, uint32_t n) {
+                 g.buildUp();
+                 std::vector<uint32_t> r;
+                 auto it = std::lower_bound(g.nodeMut.begin(), g.nodeMut.end(), std::make_pair(n, 0U),
+                                            [](const auto& a, const auto& b) {
+                                                const bool ai = a.first == kInvalidNode, bi = b.first == kInvalidNode;
+                                                if (ai != bi) return ai;
+                                                return a < b;
+                                            });
+                 for (; it != g.nodeMut.end() && it->first == n; ++it) r.push_back(it->second);
+                 return r;
+             },
+             py::arg("node_id"))
+        .def("get_num_individual_coals",
+             [](PyGRG& g, uint32_t n) {
+                 if (n < g.h.numSamples) return 0U;
+                 if (g.h.coal.empty() || n >= g.h.numNodes()) return kCoalNotSet;
+                 return g.h.coal[n];
+             },
+             py::arg("node_id"))
+        .def("get_mutation_position", [](PyGRG& g, uint32_t m) {
+            API_CHECK(m < g.h.numMutations(), "Invalid MutationID: " << m);
+            return g.h.mutPos.empty() ? (double)m : g.h.mutPos[m];
+        })
+        .def("arrays",
+             [](PyGRG& g) {
+                 py::dict d;
+                 d["num_samples"] = g.h.numSamples;
+                 d["ploidy"] = g.h.ploidy;
+                 d["down_offsets"] = toArray(g.h.downOff);
+                 d["down_children"] = toArray(g.h.downChild);
+                 d["mutation_nodes"] = toArray(g.h.mutNode);
+                 d["mutation_miss_nodes"] = g.h.mutMiss.empty() ? py::object(py::none()) : py::object(toArray(g.h.mutMiss));
+                 d["coal_counts"] = g.h.coal.empty() ? py::object(py::none()) : py::object(toArray(g.h.coal));
+                 d["positions"] = toArray(g.h.mutPos);
+                 return d;
+             },
+             "The graph as plain arrays (down-edge CSR, mutation map, coalescence counts).")
+        .def("graph_stats",
+             [](PyGRG& g) {
+                 const GraphStats st = computeGraphStats(g.h);
+                 py::dict d;
+                 d["num_nodes"] = g.h.numNodes();
+                 d["num_edges"] = st.numEdges;
+                 d["up_levels"] = st.upLevels;
+                 d["max_fan_in"] = st.maxFanIn;
+                 d["fan_in_histogram"] = st.fanInHistogram;
+                 return d;
+             })
+        .def("device_info",
+             [](PyGRG& g) {
+                 grgl_graph_info info{};
+                 checkStatus(grgl_hip_graph_info(g.deviceGraph(), &info));
+                 py::dict d;
+                 d["num_nodes"] = info.num_nodes;
+                 d["num_edges"] = info.num_edges;
+                 d["up_levels"] = info.up_levels;
+                 d["down_levels"] = info.down_levels;
+                 d["max_fanin_up"] = info.max_fanin_up;
+                 d["max_fanin_down"] = info.max_fanin_down;
+                 d["up_gather_entries"] = info.up_gather_entries;
+                 d["down_gather_entries"] = info.down_gather_entries;
+                 d["device_bytes"] = info.device_bytes;
+                 d["device"] = info.device;
+                 return d;
+             })
+        .def("device_handle", [](PyGRG& g) { return (uintptr_t)g.deviceGraph(); },
+             "Address of the device-resident graph (grgl_graph*) for C-ABI callers.")
+        .def("__str__",
+             [](PyGRG& g) {
+                 std::ostringstream ss;
+                 ss << "<[grgl_amd.GRG] mutations=" << g.h.numMutations() << ", individuals=" << g.h.numSamples / g.h.ploidy
+                    << ", ploidy=" << g.h.ploidy << ", mutable=No>";
+                 return ss.str();
+             })
+        .def("__repr__", [](PyGRG& g) {
+            std::ostringstream ss;
+            ss << "<[grgl_amd.GRG] mutations=" << g.h.numMutations() << ", individuals=" << g.h.numSamples / g.h.ploidy
+               << ", ploidy=" << g.h.ploidy << ", mutable=No>";
+            return ss.str();
+        });
+
+    m.def("matmul", &matMul, py::arg("grg"), py::arg("input"), py::arg("direction"), py::arg("emit_all_nodes") = false,
+          py::arg("by_individual") = false, py::arg("init") = py::none(), py::arg("miss") = py::none(),
+          py::arg("order_preserving") = false,
+          "pygrgl.matmul on the GPU: V(KxN)·X for UP, W(KxM)·X^T for DOWN (see the reference docstring, "
+          "src/python/_grgl.cpp:1167-1229).  order_preserving=True sums every node in the reference's order, "
+          "so float results match the CPU path bit for bit.");
+    m.def("dot_product", &dotProduct, py::arg("grg"), py::arg("input"), py::arg("direction"),
+          "DEPRECATED, use matmul instead.  1-D form of matmul.");
+
+    m.def("grg_from_arrays", &fromArrays, py::arg("num_samples"), py::arg("ploidy"), py::arg("down_offsets"),
+          py::arg("down_children"), py::arg("mutation_nodes"), py::arg("mutation_miss_nodes") = py::none(),
+          py::arg("coal_counts") = py::none(), py::arg("positions") = py::none(),
+          "Build an immutable, ordered GRG from a down-edge CSR and the mutation->node map.");
+    m.def(
+        "grg_from_coalescent",
+        [](uint32_t numSamples, uint32_t numSites, uint32_t numTrees, uint32_t ploidy, uint64_t seed, uint32_t threads) {
+            SynthParams p;
+            p.numSamples = numSamples;
+            p.numSites = numSites;
+            p.numTrees = numTrees;
+            p.ploidy = ploidy;
+            p.seed = seed;
+            p.threads = threads;
+            auto g = std::make_shared<PyGRG>();
+            {
+                py::gil_scoped_release release;
+                g->h = synthCoalescentGrg(p);
+            }
+            return g;
+        },
+        py::arg("num_samples"), py::arg("num_sites"), py::arg("num_trees") = 1, py::arg("ploidy") = 2,
+        py::arg("seed") = 20261203ULL, py::arg("threads") = 0,
+        "Seeded synthetic GRG: Kingman coalescent trees (one per genome segment), mutations on branches with "
+        "probability proportional to branch length, simplified as the reference writer does.");
+
+    m.def(
+        "_level_plan",
+        [](PyGRGPtr& g, Direction dir) {
+            const DirPlan p = dir == Direction::UP ? buildUpPlan(g->h) : buildDownPlan(g->h);
+            py::dict d;
+            d["num_levels"] = p.numLevels;
+            d["first_pulled_level"] = p.firstPulledLevel;
+            d["level_start"] = toArray(p.levelStart);
+            d["order"] = toArray(p.order);
+            d["rank"] = toArray(p.rank);
+            d["off"] = toArray(p.off);
+            d["src"] = toArray(p.src);
+            d["out_row"] = toArray(p.outRow);
+            d["miss_row"] = toArray(p.missRow);
+            d["coal_by_row"] = toArray(p.coalByRow);
+            return d;
+        },
+        py::arg("grg"), py::arg("direction"),
+        "The level-bucketed gather plan the device library builds for one direction (for inspection).");
+
+    m.def("set_device", [](int d) { g_device = d; }, py::arg("device"));
+    m.def("get_device", []() { return g_device; });
+    m.def("device_count", []() { return grgl_hip_device_count(); });
+
+    m.attr("INVALID_NODE") = kInvalidNode;
+    m.attr("COAL_COUNT_NOT_SET") = kCoalNotSet;
+    m.attr("NO_UP_EDGES") = kNoUpEdges;
+}

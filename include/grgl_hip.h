@@ -1,0 +1,226 @@
+/*
+ * grgl_hip.h — C ABI of the MI355X (gfx950) GRG matrix multiplication library
+ * (libgrgl_hip.so).
+ *
+ * Plain C: no HIP, torch or C++ types cross this boundary, no exceptions escape it.
+ * Every entry point returns a grgl_status; on failure grgl_hip_last_error() returns a
+ * thread-local message.  Streams are passed as `void*` holding a hipStream_t.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   grgl_hip_graph_create   <- convertGRGToGPUGRG (include/grgl/cuda/gpu_grg.h:747-749,
+ *                              src/cuda/gpu_grg.cu:143-186): uploads an ordered GRG
+ *                              (a CSRGRG, include/grgl/grg.h:1232-1300) to the device.
+ *   grgl_hip_matmul         <- GRG::matrixMultiplication<IOType,NodeValueType,useBitVector>
+ *                              (include/grgl/grg.h:656-667, :1342-1517), which backs
+ *                              pygrgl.matmul (src/python/_grgl.cpp:100-241, :1159-1230) and
+ *                              GPUGRG::matrixMultiplication (src/cuda/cuda_matmul.cu:240-408).
+ *   grgl_hip_matmul_batch   <- no single reference counterpart: several matrixMultiplication
+ *                              calls on one graph and direction fused into one traversal.
+ *   grgl_hip_server_*       <- no reference counterpart (the reference serves concurrent
+ *                              callers by releasing the GIL, src/python/_grgl.cpp:169-183).
+ */
+#ifndef GRGL_HIP_H
+#define GRGL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* include/grgl/grgnode.h:57-58 */
+#define GRGL_INVALID_NODE 0x7fffffffU
+/* include/grgl/node_data.h:31 */
+#define GRGL_COAL_COUNT_NOT_SET 0xffffffffU
+
+typedef enum grgl_status {
+    GRGL_OK = 0,
+    GRGL_ERR_MISUSE = 1,      /* ApiMisuseFailure (include/grgl/common.h:45-49) */
+    GRGL_ERR_DEVICE = 2,      /* HIP runtime or launch failure */
+    GRGL_ERR_NOMEM = 3,       /* device allocation failed or HBM budget exceeded */
+    GRGL_ERR_UNSUPPORTED = 4, /* valid request this build does not implement */
+    GRGL_ERR_BUSY = 5,        /* server queue full */
+    GRGL_ERR_NOT_READY = 6,   /* request still in flight */
+    GRGL_ERR_CANCELLED = 7    /* request cancelled before it ran */
+} grgl_status;
+
+/* include/grgl/visitor.h:26 — same numeric values */
+typedef enum grgl_direction { GRGL_DIRECTION_DOWN = 1, GRGL_DIRECTION_UP = 2 } grgl_direction;
+
+/* The dtypes pygrgl.matmul dispatches on (src/python/_grgl.cpp:205-240). */
+typedef enum grgl_dtype {
+    GRGL_F64 = 0, GRGL_F32 = 1, GRGL_I64 = 2, GRGL_I32 = 3, GRGL_I16 = 4, GRGL_I8 = 5,
+    GRGL_U64 = 6, GRGL_U32 = 7, GRGL_U16 = 8, GRGL_U8 = 9,
+    GRGL_BOOL = 10 /* GF(2): one byte per value, XOR (include/grgl/internal_mult.h:111-128) */
+} grgl_dtype;
+
+/* GRG::NodeInitEnum (include/grgl/grg.h:646-651) — same numeric values */
+typedef enum grgl_init_mode {
+    GRGL_INIT_ZERO = 0,
+    GRGL_INIT_VECTOR = 1, /* init[k] added to every node, row k */
+    GRGL_INIT_MATRIX = 2, /* init[k * num_nodes + node] */
+    GRGL_INIT_XTX = 3     /* 2 * individual coalescence count of the node */
+} grgl_init_mode;
+
+/* grgl_hip_matmul flags */
+#define GRGL_FLAG_HOST_PTRS 0x1U        /* input/output/init/miss are host memory; the call is synchronous */
+#define GRGL_FLAG_ACCUMULATE 0x2U       /* output (and UP miss) += result, as the reference does; else = result */
+#define GRGL_FLAG_ORDER_PRESERVING 0x4U /* every node summed sequentially in the reference's order, so float
+                                           results are bit-identical to the CPU path (slower on high fan-in) */
+
+/*
+ * An immutable, ordered GRG in plain arrays — the content of a CSRGRG:
+ * nodes 0..num_samples-1 are the samples; every child ID is smaller than its parent's ID;
+ * each node's children are sorted ascending (include/grgl/csr_storage.h:580).
+ * The library copies what it needs; the caller keeps ownership of these arrays.
+ */
+typedef struct grgl_graph_desc {
+    uint32_t num_samples;
+    uint32_t num_nodes;
+    uint32_t ploidy;
+    uint32_t num_mutations;
+    const uint64_t* down_offsets;        /* num_nodes + 1 */
+    const uint32_t* down_children;       /* down_offsets[num_nodes] entries */
+    const uint32_t* mutation_nodes;      /* per MutationId: node or GRGL_INVALID_NODE */
+    const uint32_t* mutation_miss_nodes; /* per MutationId: missingness node or invalid; NULL if none */
+    const uint32_t* coal_counts;         /* per node (samples ignored); NULL if not available */
+} grgl_graph_desc;
+
+typedef struct grgl_graph grgl_graph;
+
+typedef struct grgl_graph_info {
+    uint32_t num_samples;
+    uint32_t num_nodes;
+    uint32_t num_mutations;
+    uint32_t ploidy;
+    uint64_t num_edges;
+    uint32_t up_levels;        /* topological levels of the UP (samples -> mutations) sweep */
+    uint32_t down_levels;      /* topological levels of the DOWN sweep */
+    uint32_t max_fanin_up;     /* largest per-node gather list, UP */
+    uint32_t max_fanin_down;   /* largest per-node gather list, DOWN (parents + seeds) */
+    uint64_t up_gather_entries;   /* total gather-list entries, UP */
+    uint64_t down_gather_entries; /* total gather-list entries, DOWN */
+    uint64_t device_bytes;     /* resident index memory on the device */
+    int device;
+} grgl_graph_info;
+
+grgl_status grgl_hip_graph_create(const grgl_graph_desc* desc, int device, grgl_graph** out);
+grgl_status grgl_hip_graph_destroy(grgl_graph* graph);
+grgl_status grgl_hip_graph_info(const grgl_graph* graph, grgl_graph_info* info);
+
+/*
+ * GRG::matrixMultiplication on the device, argument for argument
+ * (include/grgl/grg.h:656-667), plus dtype, flags and a stream:
+ *   input        input_rows (K) x input_cols, row-major.  UP: input_cols = N (or N/ploidy
+ *                with by_individual).  DOWN: input_cols = M.
+ *   output       output_size = K x output_cols, row-major.  UP: M columns, DOWN: N (or
+ *                N/ploidy), emit_all_nodes: num_nodes.
+ *   init         K values (VECTOR) or K x num_nodes (MATRIX); NULL otherwise.
+ *   miss         K x M.  DOWN: an input added into each mutation's missingness node.
+ *                UP: an output, accumulated like `output`.
+ * Without GRGL_FLAG_HOST_PTRS every pointer is device memory and the work is enqueued on
+ * `stream` (a hipStream_t; NULL = the device's null stream) without a host sync.
+ */
+grgl_status grgl_hip_matmul(grgl_graph* graph, grgl_dtype dtype, const void* input, size_t input_cols,
+                            size_t input_rows, grgl_direction direction, void* output, size_t output_size,
+                            int emit_all_nodes, int by_individual, const void* init, grgl_init_mode init_mode,
+                            void* miss, uint32_t flags, void* stream);
+
+/*
+ * Several requests on one graph, one direction and one dtype carried by ONE traversal:
+ * their rows are stacked (sum of rows <= 65536) and share every edge read.  Each request
+ * keeps its own input/output buffers and gets the same result as a grgl_hip_matmul call
+ * (bit-identical for integers and for GRGL_FLAG_ORDER_PRESERVING; floats otherwise within
+ * summation-order rounding).  No init/miss/emit_all/by_individual here.
+ */
+typedef struct grgl_request {
+    uint32_t rows;
+    const void* input;  /* rows x input_cols */
+    void* output;       /* rows x output_cols */
+} grgl_request;
+
+grgl_status grgl_hip_matmul_batch(grgl_graph* graph, grgl_dtype dtype, grgl_direction direction,
+                                  const grgl_request* requests, uint32_t num_requests, uint32_t flags,
+                                  void* stream);
+
+/* Tuning of the level kernels for (graph, direction, dtype, rows).  0 = library default. */
+typedef struct grgl_tuning {
+    uint32_t team_lanes;     /* lanes that share one node's row slice: 1..64, power of two */
+    uint32_t heavy_fanin;    /* gather lists longer than this are split over a whole block */
+    uint32_t use_graph;      /* 1 = replay the level sequence as a captured hipGraph */
+} grgl_tuning;
+
+grgl_status grgl_hip_set_tuning(grgl_graph* graph, grgl_direction direction, grgl_dtype dtype, uint32_t rows,
+                                const grgl_tuning* tuning);
+/* Time candidate tunings on the device and keep the fastest; writes it to *chosen. */
+grgl_status grgl_hip_autotune(grgl_graph* graph, grgl_direction direction, grgl_dtype dtype, uint32_t rows,
+                              grgl_tuning* chosen, float* best_ms);
+
+/*
+ * Per-launch timing of the level (pull-gather) kernels.  When enabled, each level launch
+ * of subsequent calls is bracketed by HIP events on the stream it runs on; the call below
+ * syncs and reports the summed kernel milliseconds, launch count and the algorithmic
+ * bytes those launches moved (4 B per gather-list entry and per node offset, plus one
+ * row read per entry and one row write per node).
+ */
+grgl_status grgl_hip_kernel_timing(grgl_graph* graph, int enable);
+grgl_status grgl_hip_kernel_timing_read(grgl_graph* graph, double* total_ms, uint64_t* launches,
+                                        double* algorithmic_bytes, int reset);
+
+/* ----------------------------------------------------------------------------------------
+ * Serving: a resident-graph request loop with continuous batching.  Requests on the same
+ * (graph, direction, dtype) that are queued when a stream slot frees up are stacked into
+ * one traversal; admission is bounded by an HBM budget for in-flight work.
+ * Host-memory requests; results are written to the caller's output buffer before the
+ * request completes.
+ * -------------------------------------------------------------------------------------- */
+typedef struct grgl_server grgl_server;
+
+typedef struct grgl_server_config {
+    int device;
+    uint32_t num_streams;      /* concurrent traversals on the device (slots); default 2 */
+    uint32_t max_batch_rows;   /* stacked rows per traversal; default 512 */
+    uint64_t hbm_budget_bytes; /* device bytes for in-flight batches; default 8 GiB */
+    uint32_t max_queue;        /* queued requests before submit returns BUSY; default 4096 */
+    uint32_t flags;            /* GRGL_FLAG_ORDER_PRESERVING allowed */
+} grgl_server_config;
+
+typedef struct grgl_server_stats {
+    uint64_t submitted;
+    uint64_t completed;
+    uint64_t rejected;
+    uint64_t cancelled;
+    uint64_t batches;
+    uint64_t rows_processed;
+    uint64_t edge_rhs;          /* sum over batches of gather entries x rows */
+    uint64_t hbm_in_use;        /* bytes held by in-flight batches now */
+    uint64_t hbm_peak;
+    uint32_t queue_depth;
+    double latency_p50_ms;      /* submit -> completion, over completed requests */
+    double latency_p99_ms;
+    double latency_mean_ms;
+} grgl_server_stats;
+
+grgl_status grgl_hip_server_create(const grgl_server_config* config, grgl_server** out);
+grgl_status grgl_hip_server_add_graph(grgl_server* server, grgl_graph* graph, uint32_t* graph_id);
+grgl_status grgl_hip_server_submit(grgl_server* server, uint32_t graph_id, grgl_dtype dtype,
+                                   grgl_direction direction, uint32_t rows, const void* host_input,
+                                   void* host_output, uint64_t* request_id);
+/* GRGL_OK when done (result written), GRGL_ERR_NOT_READY while queued or running. */
+grgl_status grgl_hip_server_poll(grgl_server* server, uint64_t request_id, double* latency_ms);
+grgl_status grgl_hip_server_wait(grgl_server* server, uint64_t request_id, double* latency_ms);
+/* Cancels a request that has not started; GRGL_ERR_NOT_READY if it is already running. */
+grgl_status grgl_hip_server_cancel(grgl_server* server, uint64_t request_id);
+grgl_status grgl_hip_server_stats(grgl_server* server, grgl_server_stats* stats, int reset_latencies);
+grgl_status grgl_hip_server_destroy(grgl_server* server);
+
+/* Thread-local message for the last non-OK status returned on this thread. */
+const char* grgl_hip_last_error(void);
+/* Number of visible devices (0 when no GPU is present). */
+int grgl_hip_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRGL_HIP_H */

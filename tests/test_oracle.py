@@ -1,0 +1,235 @@
+"""Pins the oracle (oracle/grg_oracle.c) before anything is checked against it.
+
+Each pin is chosen so a plausible mistake in the restatement fails it:
+  * golden vectors from the reference's own unit tests (test/unit/test_grg.cpp);
+  * allele counts from the reference's end-to-end expect files (UP with ones);
+  * a dense brute-force V @ X on reachability-built X (catches dropped terms, wrong
+    direction, wrong by-individual pairing);
+  * the adjoint identity sum(UP(V) * W) == sum(V * DOWN(W)) in int64;
+  * bit-exact agreement with the reference's own matmul compiled from its sources
+    (oracle/_ref), for every dtype and option — this pins the float summation order.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import (INVALID, bits_equal, dense_x, depth3_bin_tree, direct_map_grg, make_input, random_grg,
+                     sample8_grg, vcf_fixture)
+
+UP, DOWN = oracle.UP, oracle.DOWN
+ALL_DTYPES = [np.float64, np.float32, np.int64, np.int32, np.int16, np.int8, np.uint64, np.uint32, np.uint16,
+              np.uint8, np.bool_]
+
+needs_ref = pytest.mark.skipif(not oracle.ref_available(), reason="reference library oracle/_ref not built here")
+
+
+def og(arrays):
+    return oracle.OracleGraph(arrays)
+
+
+# --- golden vectors: test/unit/test_grg.cpp ---------------------------------------------
+
+def test_dot_product_good():
+    """test_grg.cpp:41-89: mutations on nodes 1, 5, 6."""
+    g = og(depth3_bin_tree([1, 5, 6]))
+    down = g.matmul(np.array([[1.0, 2.0, 1.0]]), DOWN)
+    assert down.tolist() == [[1.0, 2.0, 3.0, 3.0]]
+    up = g.matmul(np.ones((1, 4)), UP)
+    assert up.tolist() == [[1.0, 2.0, 4.0]]
+
+
+def test_matrix_mult_good():
+    """test_grg.cpp:91-124: K=2 in both directions."""
+    g = og(depth3_bin_tree([1, 5, 6]))
+    down = g.matmul(np.array([[1.0, 2.0, 1.0], [0.0, 1.0, 1.0]]), DOWN)
+    assert down.ravel().tolist() == [1.0, 2.0, 3.0, 3.0, 1.0, 1.0, 2.0, 2.0]
+    v = np.ones((2, 4))
+    v[1] /= 4.0
+    up = g.matmul(v, UP)
+    np.testing.assert_allclose(up.ravel(), [1.0, 2.0, 4.0, 0.25, 0.5, 1.0], atol=1e-3)
+
+
+def test_matmul_bad():
+    """test_grg.cpp:126-161: wrong sizes raise ApiMisuseFailure."""
+    g = og(depth3_bin_tree([1, 5]))
+    with pytest.raises(oracle.OracleMisuse):
+        g.matmul(np.zeros((1, 4)), DOWN)
+    with pytest.raises(oracle.OracleMisuse):
+        g.matmul(np.zeros((1, 3)), UP)
+    with pytest.raises(oracle.OracleMisuse):
+        g.matmul(np.zeros((9, 1)), UP)
+
+
+def test_matmul_xtx():
+    """test_grg.cpp:258-292: XTX init on depth3BinTree gives {8,4,4}; fails without coals."""
+    with pytest.raises(oracle.OracleMisuse):
+        og(depth3_bin_tree([6, 5, 4])).matmul(np.ones((1, 4), np.int64), UP, init="xtx")
+    # calculateMissingCoals: individual {0,1} coalesces at node 4, {2,3} at node 5.
+    g = og(depth3_bin_tree([6, 5, 4], coal=[0, 0, 0, 0, 1, 1, 0]))
+    assert g.matmul(np.ones((1, 4), np.int64), UP, init="xtx").tolist() == [[8, 4, 4]]
+
+
+def test_change_samples_values():
+    """test_grg.cpp:333-367: input {11,41,37,73} UP gives {162,110,52}."""
+    g = og(depth3_bin_tree([6, 5, 4]))
+    assert g.matmul(np.array([[11, 41, 37, 73]], np.int64), UP).tolist() == [[162, 110, 52]]
+
+
+def test_sample8_reachability():
+    """common_grgs.h:35-58: every mutation's UP(ones) equals its reachable sample count."""
+    arrays = sample8_grg(list(range(8, 14)))
+    g = og(arrays)
+    up = g.matmul(np.ones((1, 8), np.int32), UP)[0]
+    assert up.tolist() == dense_x(arrays).sum(axis=0).astype(int).tolist() == [2, 3, 3, 6, 4, 7]
+
+
+# --- end-to-end expect files: test/endtoend/expect/*.txt ----------------------------------
+
+def test_vcf_allele_counts():
+    """UP(ones) = ALT COUNT of test-200-samples.freq.txt (test_basic.py:71-96)."""
+    fx = vcf_fixture()
+    g = og(direct_map_grg(fx))
+    counts = g.matmul(np.ones((1, g.num_samples), np.int32), UP)[0]
+    np.testing.assert_array_equal(counts, fx["expect_alt_count"])
+    dots = g.matmul(np.ones((1, g.num_samples)), UP)[0]
+    np.testing.assert_array_equal(dots, fx["expect_alt_count"].astype(np.float64))
+
+
+def test_vcf_zygosity_by_individual():
+    """by_individual UP(ones) = Aa + 2*aa of test-200-samples.zygosity.txt (test_basic.py:134-149)."""
+    fx = vcf_fixture()
+    g = og(direct_map_grg(fx))
+    counts = g.matmul(np.ones((1, g.num_individuals), np.int64), UP, by_individual=True)[0]
+    np.testing.assert_array_equal(counts, fx["expect_Aa"] + 2 * fx["expect_aa"])
+    # DOWN by individual: per-individual alt-allele load equals the dense product.
+    w = np.ones((1, g.num_mutations), np.int64)
+    load = g.matmul(w, DOWN, by_individual=True)[0]
+    np.testing.assert_array_equal(load, dense_x(direct_map_grg(fx), by_individual=True).sum(axis=1).astype(np.int64))
+
+
+# --- closed forms and invariants ----------------------------------------------------------
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("by_individual", [False, True])
+def test_dense_bruteforce(seed, by_individual):
+    arrays = random_grg(seed, num_samples=40, num_internal=120, heavy=(35,), num_mutations=150)
+    g = og(arrays)
+    X = dense_x(arrays, by_individual)
+    rng = np.random.default_rng(seed)
+    cols_up = g.num_individuals if by_individual else g.num_samples
+    v = rng.standard_normal((5, cols_up))
+    np.testing.assert_allclose(g.matmul(v, UP, by_individual=by_individual), v @ X, rtol=1e-12, atol=1e-12)
+    w = rng.standard_normal((5, g.num_mutations))
+    np.testing.assert_allclose(g.matmul(w, DOWN, by_individual=by_individual), w @ X.T, rtol=1e-12, atol=1e-12)
+
+
+def test_dense_bruteforce_generator_tree():
+    import grgl_amd
+
+    arrays = grgl_amd.grg_from_coalescent(200, 800, 2, seed=7).arrays()
+    g = og(arrays)
+    X = dense_x(arrays)
+    assert g.matmul(np.ones((1, 200), np.int64), UP)[0].tolist() == X.sum(axis=0).astype(int).tolist()
+    w = np.random.default_rng(0).standard_normal((3, g.num_mutations))
+    np.testing.assert_allclose(g.matmul(w, DOWN), w @ X.T, rtol=1e-12, atol=1e-11)
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_adjoint_identity(seed):
+    arrays = random_grg(seed, num_samples=100, num_internal=400, heavy=(90, 70), num_mutations=600)
+    g = og(arrays)
+    rng = np.random.default_rng(seed)
+    v = rng.integers(-50, 50, (4, g.num_samples)).astype(np.int64)
+    w = rng.integers(-50, 50, (4, g.num_mutations)).astype(np.int64)
+    assert int((g.matmul(v, UP) * w).sum()) == int((v * g.matmul(w, DOWN)).sum())
+
+
+def test_emit_all_nodes_matches_subtree_sums():
+    arrays = random_grg(9, num_samples=30, num_internal=60, num_mutations=40)
+    g = og(arrays)
+    v = np.arange(1, 31, dtype=np.int64)[None, :]
+    allnodes = g.matmul(v, UP, emit_all_nodes=True)[0]
+    off, child = arrays["down_offsets"], arrays["down_children"]
+    expect = np.zeros(g.num_nodes, np.int64)
+    expect[:30] = v[0]
+    for n in range(30, g.num_nodes):
+        expect[n] = sum(expect[c] for c in child[off[n]:off[n + 1]])
+    np.testing.assert_array_equal(allnodes, expect)
+
+
+def test_unmapped_mutations_stay_zero():
+    arrays = depth3_bin_tree([1, INVALID, 6])
+    g = og(arrays)
+    assert g.matmul(np.ones((1, 4), np.int32), UP).tolist() == [[1, 0, 4]]
+    assert g.matmul(np.array([[1, 5, 1]], np.int32), DOWN).tolist() == [[1, 2, 1, 1]]
+
+
+# --- bit-exact against the reference's own matmul -----------------------------------------
+
+GRAPHS = {
+    "random": lambda: random_grg(11, num_samples=60, num_internal=300, heavy=(200, 90), num_mutations=500),
+    "random_miss_coal": lambda: random_grg(12, num_samples=48, num_internal=150, num_mutations=300, with_miss=True,
+                                           with_coal=True),
+    "vcf": lambda: direct_map_grg(vcf_fixture()),
+}
+
+
+@needs_ref
+@pytest.mark.parametrize("graph", list(GRAPHS))
+@pytest.mark.parametrize("dtype", ALL_DTYPES)
+def test_oracle_equals_reference(graph, dtype):
+    arrays = GRAPHS[graph]()
+    g = og(arrays)
+    rng = np.random.default_rng(hash((graph, np.dtype(dtype).name)) % 2**32)
+    for direction in (UP, DOWN):
+        for by_ind in (False, True):
+            cols = g.num_mutations if direction == DOWN else (g.num_individuals if by_ind else g.num_samples)
+            for K in (1, 3, 8):
+                x = make_input(rng, dtype, (K, cols))
+                assert bits_equal(g.matmul(x, direction, by_individual=by_ind),
+                                  g.ref_matmul(x, direction, by_individual=by_ind))
+        x = make_input(rng, dtype, (2, g.num_mutations if direction == DOWN else g.num_samples))
+        assert bits_equal(g.matmul(x, direction, emit_all_nodes=True), g.ref_matmul(x, direction, emit_all_nodes=True))
+
+
+@needs_ref
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.int64, np.int32, np.int8])
+def test_oracle_equals_reference_init_and_miss(dtype):
+    arrays = GRAPHS["random_miss_coal"]()
+    g = og(arrays)
+    rng = np.random.default_rng(5)
+    K = 3
+    for direction in (UP, DOWN):
+        cols = g.num_mutations if direction == DOWN else g.num_samples
+        x = make_input(rng, dtype, (K, cols))
+        vec = make_input(rng, dtype, (K,))
+        mat = make_input(rng, dtype, (K, g.num_nodes))
+        for init in (vec, mat, "xtx"):
+            assert bits_equal(g.matmul(x, direction, init=init), g.ref_matmul(x, direction, init=init))
+        miss_in = make_input(rng, dtype, (K, g.num_mutations))
+        m1, m2 = miss_in.copy(), miss_in.copy()
+        a = g.matmul(x, direction, miss=m1)
+        b = g.ref_matmul(x, direction, miss=m2)
+        assert bits_equal(a, b)
+        assert bits_equal(m1, m2)
+
+
+@needs_ref
+def test_reference_rejects_like_oracle():
+    g = og(depth3_bin_tree([1, 5, 6]))
+    for x, d in ((np.zeros((1, 4)), DOWN), (np.zeros((1, 3)), UP)):
+        with pytest.raises(oracle.OracleMisuse):
+            g.ref_matmul(x, d)
+        with pytest.raises(oracle.OracleMisuse):
+            g.matmul(x, d)
+
+
+def test_faithful_mode_matches_plain():
+    arrays = random_grg(21, num_samples=50, num_internal=200, heavy=(60,), num_mutations=300)
+    g = og(arrays)
+    rng = np.random.default_rng(3)
+    for dt in (np.float64, np.float32, np.int64, np.int32):
+        for d in (UP, DOWN):
+            x = make_input(rng, dt, (4, g.num_mutations if d == DOWN else g.num_samples))
+            assert bits_equal(g.faithful_matmul(x, d), g.matmul(x, d))
