@@ -19,8 +19,8 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 ARCH = os.environ.get("GRGL_AMD_ARCH", "gfx950")
 
-HIP_SOURCES = ["grgl_hip.hip", "server.cpp", "levelize.cpp", "host_graph.cpp"]
-PY_SOURCES = ["pygrgl_module.cpp", "synth.cpp", "host_graph.cpp", "levelize.cpp"]
+HIP_SOURCES = ["grgl_hip.hip", "server.cpp", "levelize.cpp", "host_graph.cpp", "stacking.cpp"]
+PY_SOURCES = ["pygrgl_module.cpp", "synth.cpp", "host_graph.cpp", "levelize.cpp", "stacking.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")

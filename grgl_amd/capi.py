@@ -121,6 +121,7 @@ class ServerConfig(ctypes.Structure):
         ("hbm_budget_bytes", ctypes.c_uint64),
         ("max_queue", ctypes.c_uint32),
         ("flags", ctypes.c_uint32),
+        ("batch_window_us", ctypes.c_uint32),
     ]
 
 
@@ -318,3 +319,76 @@ class Graph:
         check(lib().grgl_hip_kernel_timing_read(self.handle, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b),
                                                 int(reset)))
         return ms.value, n.value, b.value
+
+
+class Server:
+    """grgl_hip_server_*: resident graphs, continuous batching, HBM-budget admission.
+
+    Requests are host numpy arrays; each submit returns (request_id, output array) and the
+    output is filled when wait()/poll() reports completion.  Inputs and outputs are kept
+    alive by this object until the request finishes.
+    """
+
+    def __init__(self, device=0, num_streams=2, max_batch_rows=512, hbm_budget_bytes=0, max_queue=0, flags=0,
+                 batch_window_us=0):
+        cfg = ServerConfig(device, num_streams, max_batch_rows, hbm_budget_bytes, max_queue, flags, batch_window_us)
+        h = ctypes.c_void_p()
+        check(lib().grgl_hip_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self._graphs = []
+        self._live = {}
+
+    def add_graph(self, graph):
+        gid = ctypes.c_uint32()
+        check(lib().grgl_hip_server_add_graph(self.handle, graph.handle, ctypes.byref(gid)))
+        self._graphs.append(graph)
+        return gid.value
+
+    def submit(self, graph_id, x, direction):
+        x = np.ascontiguousarray(x)
+        g = self._graphs[graph_id]
+        rows = x.shape[0]
+        out_cols = g.num_mutations if direction == UP else g.num_samples
+        out = np.zeros((rows, out_cols), dtype=x.dtype)
+        rid = ctypes.c_uint64()
+        check(lib().grgl_hip_server_submit(self.handle, graph_id, NUMPY_TO_DTYPE[x.dtype], direction, rows,
+                                           x.ctypes.data, out.ctypes.data, ctypes.byref(rid)))
+        self._live[rid.value] = (x, out)
+        return rid.value, out
+
+    def _finish(self, rid, fn):
+        lat = ctypes.c_double()
+        st = fn(self.handle, rid, ctypes.byref(lat))
+        if st == ERR_NOT_READY:
+            return None
+        self._live.pop(rid, None)
+        check(st)
+        return lat.value
+
+    def wait(self, rid):
+        """Blocks until done; returns the latency in ms (raises GrglError on failure)."""
+        return self._finish(rid, lib().grgl_hip_server_wait)
+
+    def poll(self, rid):
+        """Latency in ms when done, None while queued or running."""
+        return self._finish(rid, lib().grgl_hip_server_poll)
+
+    def cancel(self, rid):
+        check(lib().grgl_hip_server_cancel(self.handle, rid))
+
+    def stats(self, reset=False):
+        s = ServerStats()
+        check(lib().grgl_hip_server_stats(self.handle, ctypes.byref(s), int(reset)))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.handle:
+            check(lib().grgl_hip_server_destroy(self.handle))
+            self.handle = None
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -14,6 +14,7 @@
 
 #include "grgl_hip.h"
 #include "levelize.h"
+#include "stacking.h"
 
 namespace grgl_amd {
 

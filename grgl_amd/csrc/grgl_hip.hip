@@ -40,8 +40,6 @@ uint32_t log2u(uint32_t x) {
     return l;
 }
 
-// Transpose tile: TK = min(32, pow2ceil(K)).
-uint32_t tileShiftFor(uint32_t K) { return std::min<uint32_t>(5, log2u(pow2ceil(K))); }
 
 } // namespace
 
@@ -82,7 +80,9 @@ RowGeometry rowGeometry(uint32_t K, size_t elemSize) {
     const uint64_t rb = (raw + r.vb - 1) / r.vb * r.vb;
     if (rb > 0xffffffffULL) throw MisuseError("too many rows (K) for one traversal");
     r.rowBytes = (uint32_t)rb;
-    r.ld = rb >= 64 ? (rb + 63) / 64 * 64 : rb;
+    // Row starts aligned so that the 256-byte slices a team reads never straddle an extra
+    // cache line: 256-byte alignment for wide rows, 64 bytes for medium ones.
+    r.ld = rb >= 256 ? (rb + 255) / 256 * 256 : (rb >= 64 ? (rb + 63) / 64 * 64 : rb);
     r.lanes = r.rowBytes / r.vb;
     return r;
 }
@@ -166,33 +166,41 @@ void launchPull(TypeClass tc, uint32_t vb, uint32_t tw, dim3 grid, const PullArg
     }
 }
 
-template <typename T, bool XOR> void launchSeedT(dim3 grid, const SeedArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL((seed_rows_kernel<T, XOR>), grid, dim3(kBlock), 0, s, a);
+template <typename T, bool XOR> void launchSeedT(dim3 grid, const SeedArgs& a, bool vr, bool vm, hipStream_t s) {
+    constexpr int V = 16 / (int)sizeof(T);
+    if (vr && vm) hipLaunchKernelGGL((seed_rows_kernel<T, XOR, V, V>), grid, dim3(kBlock), 0, s, a);
+    else if (vr) hipLaunchKernelGGL((seed_rows_kernel<T, XOR, V, 1>), grid, dim3(kBlock), 0, s, a);
+    else if (vm) hipLaunchKernelGGL((seed_rows_kernel<T, XOR, 1, V>), grid, dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL((seed_rows_kernel<T, XOR, 1, 1>), grid, dim3(kBlock), 0, s, a);
 }
-void launchSeed(TypeClass tc, dim3 grid, const SeedArgs& a, hipStream_t s) {
+void launchSeed(TypeClass tc, dim3 grid, const SeedArgs& a, bool vr, bool vm, hipStream_t s) {
     switch (tc) {
-    case kF64: launchSeedT<double, false>(grid, a, s); break;
-    case kF32: launchSeedT<float, false>(grid, a, s); break;
-    case kW64: launchSeedT<uint64_t, false>(grid, a, s); break;
-    case kW32: launchSeedT<uint32_t, false>(grid, a, s); break;
-    case kW16: launchSeedT<uint16_t, false>(grid, a, s); break;
-    case kW8: launchSeedT<uint8_t, false>(grid, a, s); break;
-    case kX8: launchSeedT<uint8_t, true>(grid, a, s); break;
+    case kF64: launchSeedT<double, false>(grid, a, vr, vm, s); break;
+    case kF32: launchSeedT<float, false>(grid, a, vr, vm, s); break;
+    case kW64: launchSeedT<uint64_t, false>(grid, a, vr, vm, s); break;
+    case kW32: launchSeedT<uint32_t, false>(grid, a, vr, vm, s); break;
+    case kW16: launchSeedT<uint16_t, false>(grid, a, vr, vm, s); break;
+    case kW8: launchSeedT<uint8_t, false>(grid, a, vr, vm, s); break;
+    case kX8: launchSeedT<uint8_t, true>(grid, a, vr, vm, s); break;
     }
 }
 
-template <typename T, bool XOR> void launchReadoutT(dim3 grid, const ReadoutArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL((readout_kernel<T, XOR>), grid, dim3(kBlock), 0, s, a);
+template <typename T, bool XOR> void launchReadoutT(dim3 grid, const ReadoutArgs& a, bool vr, bool vm, hipStream_t s) {
+    constexpr int V = 16 / (int)sizeof(T);
+    if (vr && vm) hipLaunchKernelGGL((readout_kernel<T, XOR, V, V>), grid, dim3(kBlock), 0, s, a);
+    else if (vr) hipLaunchKernelGGL((readout_kernel<T, XOR, V, 1>), grid, dim3(kBlock), 0, s, a);
+    else if (vm) hipLaunchKernelGGL((readout_kernel<T, XOR, 1, V>), grid, dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL((readout_kernel<T, XOR, 1, 1>), grid, dim3(kBlock), 0, s, a);
 }
-void launchReadout(TypeClass tc, dim3 grid, const ReadoutArgs& a, hipStream_t s) {
+void launchReadout(TypeClass tc, dim3 grid, const ReadoutArgs& a, bool vr, bool vm, hipStream_t s) {
     switch (tc) {
-    case kF64: launchReadoutT<double, false>(grid, a, s); break;
-    case kF32: launchReadoutT<float, false>(grid, a, s); break;
-    case kW64: launchReadoutT<uint64_t, false>(grid, a, s); break;
-    case kW32: launchReadoutT<uint32_t, false>(grid, a, s); break;
-    case kW16: launchReadoutT<uint16_t, false>(grid, a, s); break;
-    case kW8: launchReadoutT<uint8_t, false>(grid, a, s); break;
-    case kX8: launchReadoutT<uint8_t, true>(grid, a, s); break;
+    case kF64: launchReadoutT<double, false>(grid, a, vr, vm, s); break;
+    case kF32: launchReadoutT<float, false>(grid, a, vr, vm, s); break;
+    case kW64: launchReadoutT<uint64_t, false>(grid, a, vr, vm, s); break;
+    case kW32: launchReadoutT<uint32_t, false>(grid, a, vr, vm, s); break;
+    case kW16: launchReadoutT<uint16_t, false>(grid, a, vr, vm, s); break;
+    case kW8: launchReadoutT<uint8_t, false>(grid, a, vr, vm, s); break;
+    case kX8: launchReadoutT<uint8_t, true>(grid, a, vr, vm, s); break;
     }
 }
 
@@ -210,8 +218,24 @@ void launchInit(TypeClass tc, const InitArgs& a, hipStream_t s) {
     }
 }
 
-void seedRows(TypeClass tc, const void* in, uint64_t inCols, uint32_t K, char* rows, uint64_t ld, uint32_t rowBase,
-              uint32_t kOffset, uint32_t D, uint32_t per, const uint32_t* colMap, bool accumulate, hipStream_t s) {
+bool aligned16(const void* p) { return ((uintptr_t)p & 15U) == 0; }
+
+// Tile height: TK = min(64, pow2ceil(K)); TC = tile elements / TK.
+uint32_t tileShift(uint32_t K) { return std::min<uint32_t>(6, log2u(pow2ceil(K))); }
+
+uint32_t tileElems(size_t es) { return es == 8 ? 2048U : 4096U; }
+
+// Row-side 16-byte accesses need a 16-byte aligned row stride and request offset, and a
+// request at least 16 bytes wide: only such requests own whole 16-byte slots (stackOffsets
+// rounds their slots up), so a vector running past K touches padding, never another
+// request's columns.
+bool rowVectorOk(uint64_t ld, uint32_t kOffset, uint32_t K, size_t es) {
+    return ld % 16 == 0 && ((uint64_t)kOffset * es) % 16 == 0 && (uint64_t)K * es >= 16;
+}
+
+void seedRows(TypeClass tc, size_t es, const void* in, uint64_t inCols, uint32_t K, char* rows, uint64_t ld,
+              uint32_t rowBase, uint32_t kOffset, uint32_t D, uint32_t per, const uint32_t* colMap, bool accumulate,
+              hipStream_t s) {
     if (D == 0 || K == 0) return;
     SeedArgs a;
     a.in = in;
@@ -225,15 +249,18 @@ void seedRows(TypeClass tc, const void* in, uint64_t inCols, uint32_t K, char* r
     a.per = per;
     a.colMap = colMap;
     a.accumulate = accumulate ? 1 : 0;
-    a.tkShift = tileShiftFor(K);
+    a.tkShift = tileShift(K);
     a.normalizeBool = tc == kX8 ? 1 : 0;
-    const uint32_t TK = 1U << a.tkShift, TD = 1024U / TK;
-    dim3 grid((D + TD - 1) / TD, (K + TK - 1) / TK);
-    launchSeed(tc, grid, a, s);
+    const bool vr = rowVectorOk(ld, kOffset, K, es);
+    const bool vm = colMap == nullptr && per == 1 && (inCols * es) % 16 == 0 && aligned16(in);
+    const uint32_t TK = 1U << a.tkShift, TC = tileElems(es) / TK;
+    dim3 grid((D + TC - 1) / TC, (K + TK - 1) / TK);
+    launchSeed(tc, grid, a, vr, vm, s);
 }
 
-void readoutRows(TypeClass tc, const char* rows, uint64_t ld, uint32_t kOffset, uint32_t K, const uint32_t* map,
-                 uint32_t per, uint64_t C, void* out, uint64_t outCols, bool accumulate, hipStream_t s) {
+void readoutRows(TypeClass tc, size_t es, const char* rows, uint64_t ld, uint32_t kOffset, uint32_t K,
+                 const uint32_t* map, uint32_t per, uint64_t C, void* out, uint64_t outCols, bool accumulate,
+                 hipStream_t s) {
     if (C == 0 || K == 0) return;
     ReadoutArgs a;
     a.rows = rows;
@@ -246,10 +273,12 @@ void readoutRows(TypeClass tc, const char* rows, uint64_t ld, uint32_t kOffset, 
     a.out = out;
     a.outCols = outCols;
     a.accumulate = accumulate ? 1 : 0;
-    a.tkShift = tileShiftFor(K);
-    const uint32_t TK = 1U << a.tkShift, TD = 1024U / TK;
-    dim3 grid((uint32_t)((C + TD - 1) / TD), (K + TK - 1) / TK);
-    launchReadout(tc, grid, a, s);
+    a.tkShift = tileShift(K);
+    const bool vr = rowVectorOk(ld, kOffset, K, es);
+    const bool vm = (outCols * es) % 16 == 0 && aligned16(out);
+    const uint32_t TK = 1U << a.tkShift, TC = tileElems(es) / TK;
+    dim3 grid((uint32_t)((C + TC - 1) / TC), (K + TK - 1) / TK);
+    launchReadout(tc, grid, a, vr, vm, s);
 }
 
 Tuning chooseTuning(grgl_graph* g, grgl_direction dir, TypeClass tc, uint32_t K, const RowGeometry& geo) {
@@ -263,7 +292,9 @@ Tuning chooseTuning(grgl_graph* g, grgl_direction dir, TypeClass tc, uint32_t K,
     if (geo.vb != 16) {
         tw = 1;
     } else {
-        if (tw == 0) tw = std::min<uint32_t>(64, pow2ceil(geo.lanes));
+        // Default: 16 lanes (256-byte row slices) at most.  Blocks are dispatched K-tile
+        // by K-tile, so narrower slices keep each tile's gathered rows cache resident.
+        if (tw == 0) tw = std::min<uint32_t>(16, pow2ceil(geo.lanes));
         tw = std::min<uint32_t>(64, pow2ceil(tw));
     }
     t.teamLanes = tw;
@@ -378,7 +409,7 @@ void enqueueTraversal(grgl_graph* g, grgl_dtype dt, grgl_direction dir, const st
         for (const auto& r : reqs) {
             if (opt.initMode == GRGL_INIT_MATRIX) {
                 // init is K x numNodes; row j takes column order[j].
-                seedRows(tc, opt.init, g->V, r.K, rows, geo.ld, 0, r.kOffset, g->V, 1, plan.order, false, stream);
+                seedRows(tc, es, opt.init, g->V, r.K, rows, geo.ld, 0, r.kOffset, g->V, 1, plan.order, false, stream);
             } else {
                 InitArgs a;
                 a.rows = rows;
@@ -411,13 +442,13 @@ void enqueueTraversal(grgl_graph* g, grgl_dtype dt, grgl_direction dir, const st
             // Samples are rows 0..N-1 (grg.h:1456-1469).
             const uint32_t per = opt.byIndividual ? g->P : 1;
             const uint64_t inCols = opt.byIndividual ? g->N / g->P : g->N;
-            seedRows(tc, r.in, inCols, r.K, rows, geo.ld, 0, r.kOffset, g->N, per, nullptr, hasInit, stream);
+            seedRows(tc, es, r.in, inCols, r.K, rows, geo.ld, 0, r.kOffset, g->N, per, nullptr, hasInit, stream);
         } else {
             // Mutation seed rows V..V+M-1; missingness seed rows V+M..V+2M-1.
-            seedRows(tc, r.in, g->M, r.K, rows, geo.ld, g->V, r.kOffset, g->M, 1, nullptr, false, stream);
+            seedRows(tc, es, r.in, g->M, r.K, rows, geo.ld, g->V, r.kOffset, g->M, 1, nullptr, false, stream);
             if (g->hasMiss) {
                 if (opt.miss) {
-                    seedRows(tc, opt.miss, g->M, r.K, rows, geo.ld, g->V + g->M, r.kOffset, g->M, 1, nullptr, false,
+                    seedRows(tc, es, opt.miss, g->M, r.K, rows, geo.ld, g->V + g->M, r.kOffset, g->M, 1, nullptr, false,
                              stream);
                 } else {
                     InitArgs a{};
@@ -442,17 +473,17 @@ void enqueueTraversal(grgl_graph* g, grgl_dtype dt, grgl_direction dir, const st
     // 4. Read-out.
     for (const auto& r : reqs) {
         if (opt.emitAll) {
-            readoutRows(tc, rows, geo.ld, r.kOffset, r.K, plan.rank, 1, g->V, r.out, g->V, opt.accumulate, stream);
+            readoutRows(tc, es, rows, geo.ld, r.kOffset, r.K, plan.rank, 1, g->V, r.out, g->V, opt.accumulate, stream);
         } else if (dir == GRGL_DIRECTION_UP) {
-            readoutRows(tc, rows, geo.ld, r.kOffset, r.K, plan.outRow, 1, g->M, r.out, g->M, opt.accumulate, stream);
+            readoutRows(tc, es, rows, geo.ld, r.kOffset, r.K, plan.outRow, 1, g->M, r.out, g->M, opt.accumulate, stream);
             if (opt.miss && plan.missRow) {
-                readoutRows(tc, rows, geo.ld, r.kOffset, r.K, plan.missRow, 1, g->M, opt.miss, g->M, opt.accumulate,
+                readoutRows(tc, es, rows, geo.ld, r.kOffset, r.K, plan.missRow, 1, g->M, opt.miss, g->M, opt.accumulate,
                             stream);
             }
         } else {
             const uint32_t per = opt.byIndividual ? g->P : 1;
             const uint64_t C = g->N / per;
-            readoutRows(tc, rows, geo.ld, r.kOffset, r.K, plan.outRow, per, C, r.out, C, opt.accumulate, stream);
+            readoutRows(tc, es, rows, geo.ld, r.kOffset, r.K, plan.outRow, per, C, r.out, C, opt.accumulate, stream);
         }
     }
     GRGL_HIP_CHECK(hipGetLastError());
@@ -791,12 +822,14 @@ GRGL_EXPORT grgl_status grgl_hip_matmul_batch(grgl_graph* g, grgl_dtype dtype, g
     const size_t es = dtypeSize(dtype);
     const size_t inCols = direction == GRGL_DIRECTION_UP ? g->N : g->M;
     const size_t outCols = direction == GRGL_DIRECTION_UP ? g->M : g->N;
-    uint64_t Ktot = 0;
+    std::vector<uint32_t> rowsOf(num_requests);
     for (uint32_t i = 0; i < num_requests; i++) {
         if (requests[i].rows == 0) throw MisuseError("matmul() requires non-zero dimensions.");
         if (!requests[i].input || !requests[i].output) throw MisuseError("null input or output");
-        Ktot += requests[i].rows;
+        rowsOf[i] = requests[i].rows;
     }
+    uint64_t Ktot = 0;
+    const std::vector<uint32_t> offsets = stackOffsets(rowsOf, es, Ktot);
     if (direction != GRGL_DIRECTION_UP && direction != GRGL_DIRECTION_DOWN) throw MisuseError("Invalid traversal direction");
     if (Ktot > 65536) throw MisuseError("at most 65536 stacked rows per batch");
     const uint32_t K = (uint32_t)Ktot;
@@ -822,6 +855,7 @@ GRGL_EXPORT grgl_status grgl_hip_matmul_batch(grgl_graph* g, grgl_dtype dtype, g
         const bool accumulate = (flags & GRGL_FLAG_ACCUMULATE) != 0;
         for (uint32_t i = 0; i < num_requests; i++) {
             const auto& r = requests[i];
+            k0 = offsets[i];
             TravRequest t{r.rows, r.input, r.output, k0};
             if (host) {
                 char* in = inStage + (size_t)k0 * inCols * es;
@@ -833,7 +867,6 @@ GRGL_EXPORT grgl_status grgl_hip_matmul_batch(grgl_graph* g, grgl_dtype dtype, g
                 t.out = out;
             }
             reqs.push_back(t);
-            k0 += r.rows;
         }
         TravOptions opt;
         opt.accumulate = accumulate;

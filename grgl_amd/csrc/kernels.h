@@ -212,9 +212,17 @@ __global__ __launch_bounds__(kBlock) void pull_level_kernel(PullArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Transposes between row-major K x C matrices and node rows, through a 1024-element LDS
-// tile of TK (k) x TD (columns), TK = 2^tkShift <= 32, TD = 1024 / TK.
+// Transposes between row-major K x C matrices and node rows, through an LDS tile of
+// TK (k) x TC (columns) elements, TK = 2^tkShift <= 64 and TK * TC = kTileElems<T>.
+// VR: elements per access on the node-row side (contiguous in k), VM: elements per
+// access on the matrix side (contiguous in columns); 1 = scalar, 16/sizeof(T) = 16 B.
+// The host picks the vector forms only when addresses, strides and offsets are 16-byte
+// aligned.  A 16-byte row-side access may run past K into the row's padding (requests
+// start 16-byte aligned, so that padding belongs to no other request); matrix-side
+// accesses never cross the last column.
 // ---------------------------------------------------------------------------------------
+template <typename T> constexpr int kTileElems() { return sizeof(T) == 8 ? 2048 : 4096; }
+
 struct SeedArgs {
     const void* in;       // K x inCols, row-major
     uint64_t inCols;
@@ -231,31 +239,71 @@ struct SeedArgs {
     uint32_t normalizeBool;
 };
 
-template <typename T, bool XOR> __global__ __launch_bounds__(kBlock) void seed_rows_kernel(SeedArgs a) {
-    __shared__ T tile[1024 + 32];
-    const uint32_t TK = 1U << a.tkShift, tdShift = 10 - a.tkShift, TD = 1U << tdShift;
-    const uint64_t d0 = (uint64_t)blockIdx.x * TD;
+template <typename T, bool XOR, int VR, int VM>
+__global__ __launch_bounds__(kBlock) void seed_rows_kernel(SeedArgs a) {
+    constexpr int TE = kTileElems<T>();
+    __shared__ T tile[TE + 64];
+    const uint32_t TK = 1U << a.tkShift;
+    const uint32_t TC = (uint32_t)TE >> a.tkShift;
+    const uint64_t d0 = (uint64_t)blockIdx.x * TC;
     const uint32_t k0 = blockIdx.y * TK;
     const T* in = (const T*)a.in;
-    for (uint32_t e = threadIdx.x; e < 1024; e += kBlock) {
-        const uint32_t k = e >> tdShift, d = e & (TD - 1);
-        const uint64_t gd = d0 + d;
+    // Load: k-major, VM columns per access.
+    const uint32_t vecPerK = TC / VM;
+    for (uint32_t e = threadIdx.x; e < TK * vecPerK; e += kBlock) {
+        const uint32_t k = e / vecPerK, dv = (e % vecPerK) * VM;
         const uint32_t gk = k0 + k;
-        if (gk < a.K && gd < a.D) {
-            const uint64_t c = a.colMap ? (uint64_t)a.colMap[gd] : gd / a.per;
-            T v = in[(uint64_t)gk * a.inCols + c];
-            if (a.normalizeBool) v = (T)(v != (T)0);
-            tile[k * (TD + 1) + d] = v;
+        const uint64_t gd = d0 + dv;
+        T v[VM];
+#pragma unroll
+        for (int i = 0; i < VM; i++) v[i] = (T)0;
+        if (gk < a.K) {
+            if constexpr (VM > 1) {
+                if (gd + VM <= a.D) {
+                    typename RawVec<16>::type raw = *reinterpret_cast<const typename RawVec<16>::type*>(
+                        in + (uint64_t)gk * a.inCols + gd);
+                    __builtin_memcpy(v, &raw, 16);
+                } else {
+                    for (int i = 0; i < VM; i++)
+                        if (gd + i < a.D) v[i] = in[(uint64_t)gk * a.inCols + gd + i];
+                }
+            } else if (gd < a.D) {
+                const uint64_t c = a.colMap ? (uint64_t)a.colMap[gd] : gd / a.per;
+                v[0] = in[(uint64_t)gk * a.inCols + c];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < VM; i++) {
+            T x = v[i];
+            if (a.normalizeBool) x = (T)(x != (T)0);
+            tile[k * (TC + 1) + dv + i] = x;
         }
     }
     __syncthreads();
-    for (uint32_t e = threadIdx.x; e < 1024; e += kBlock) {
-        const uint32_t d = e >> a.tkShift, k = e & (TK - 1);
+    // Store: row-major over destination rows, VR k-values per access.
+    const uint32_t vecPerD = (TK + VR - 1) / VR;
+    for (uint32_t e = threadIdx.x; e < TC * vecPerD; e += kBlock) {
+        const uint32_t d = e / vecPerD, k = (e % vecPerD) * VR;
         const uint64_t gd = d0 + d;
         const uint32_t gk = k0 + k;
-        if (gk < a.K && gd < a.D) {
-            T* dst = (T*)(a.rows + (uint64_t)(a.rowBase + gd) * a.ld) + a.kOffset + gk;
-            const T v = tile[k * (TD + 1) + d];
+        if (gd >= a.D || gk >= a.K) continue;
+        T* dst = (T*)(a.rows + (uint64_t)(a.rowBase + gd) * a.ld) + a.kOffset + gk;
+        if constexpr (VR > 1) {
+            T v[VR];
+#pragma unroll
+            for (int i = 0; i < VR; i++) v[i] = (k + i < TK) ? tile[(k + i) * (TC + 1) + d] : (T)0;
+            if (a.accumulate) {
+                T old[VR];
+                typename RawVec<16>::type raw = *reinterpret_cast<const typename RawVec<16>::type*>(dst);
+                __builtin_memcpy(old, &raw, 16);
+#pragma unroll
+                for (int i = 0; i < VR; i++) v[i] = combine<T, XOR>(old[i], v[i]);
+            }
+            typename RawVec<16>::type raw;
+            __builtin_memcpy(&raw, v, 16);
+            *reinterpret_cast<typename RawVec<16>::type*>(dst) = raw;
+        } else {
+            const T v = tile[k * (TC + 1) + d];
             *dst = a.accumulate ? combine<T, XOR>(*dst, v) : v;
         }
     }
@@ -275,53 +323,97 @@ struct ReadoutArgs {
     uint32_t tkShift;
 };
 
-template <typename T, bool XOR> __global__ __launch_bounds__(kBlock) void readout_kernel(ReadoutArgs a) {
-    __shared__ T tile[1024 + 32];
-    const uint32_t TK = 1U << a.tkShift, tdShift = 10 - a.tkShift, TD = 1U << tdShift;
-    const uint64_t c0 = (uint64_t)blockIdx.x * TD;
+template <typename T, bool XOR, int VR, int VM>
+__global__ __launch_bounds__(kBlock) void readout_kernel(ReadoutArgs a) {
+    constexpr int TE = kTileElems<T>();
+    constexpr int OWN = TE / kBlock / VM; // matrix-side vectors owned per thread
+    __shared__ T tile[TE + 64];
+    const uint32_t TK = 1U << a.tkShift;
+    const uint32_t TC = (uint32_t)TE >> a.tkShift;
+    const uint64_t c0 = (uint64_t)blockIdx.x * TC;
     const uint32_t k0 = blockIdx.y * TK;
     T* out = (T*)a.out;
-    T acc[1024 / kBlock];
-    bool any[1024 / kBlock];
+    const uint32_t vecPerK = TC / VM;
+    T acc[OWN][VM];
+    bool any[OWN][VM];
 #pragma unroll
-    for (int i = 0; i < 1024 / kBlock; i++) {
-        const uint32_t e = threadIdx.x + i * kBlock;
-        const uint32_t k = e >> tdShift, c = e & (TD - 1);
-        any[i] = false;
-        acc[i] = (T)0;
-        if (a.accumulate && k0 + k < a.K && c0 + c < a.C) acc[i] = out[(uint64_t)(k0 + k) * a.outCols + c0 + c];
+    for (int o = 0; o < OWN; o++) {
+        const uint32_t e = threadIdx.x + o * kBlock;
+        const uint32_t k = e / vecPerK, c = (e % vecPerK) * VM;
+#pragma unroll
+        for (int i = 0; i < VM; i++) {
+            acc[o][i] = (T)0;
+            any[o][i] = false;
+        }
+        if (a.accumulate && k < TK && k0 + k < a.K) {
+            const T* src = out + (uint64_t)(k0 + k) * a.outCols + c0 + c;
+            for (int i = 0; i < VM; i++)
+                if (c0 + c + i < a.C) acc[o][i] = src[i];
+        }
     }
+    const uint32_t vecPerC = (TK + VR - 1) / VR;
     for (uint32_t j = 0; j < a.per; j++) {
-        for (uint32_t e = threadIdx.x; e < 1024; e += kBlock) {
-            const uint32_t c = e >> a.tkShift, k = e & (TK - 1);
+        // Load: one row slice per column, VR k-values per access.
+        for (uint32_t e = threadIdx.x; e < TC * vecPerC; e += kBlock) {
+            const uint32_t c = e / vecPerC, k = (e % vecPerC) * VR;
             const uint64_t gc = c0 + c;
             const uint32_t gk = k0 + k;
-            T v = (T)0;
-            if (gk < a.K && gc < a.C) {
+            T v[VR];
+#pragma unroll
+            for (int i = 0; i < VR; i++) v[i] = (T)0;
+            if (gc < a.C && gk < a.K) {
                 const uint32_t r = a.map[gc * a.per + j];
-                if (r != kInvalidRow) v = *((const T*)(a.rows + (uint64_t)r * a.ld) + a.kOffset + gk);
+                if (r != kInvalidRow) {
+                    const T* src = (const T*)(a.rows + (uint64_t)r * a.ld) + a.kOffset + gk;
+                    if constexpr (VR > 1) {
+                        typename RawVec<16>::type raw = *reinterpret_cast<const typename RawVec<16>::type*>(src);
+                        __builtin_memcpy(v, &raw, 16);
+                    } else {
+                        v[0] = *src;
+                    }
+                }
             }
-            tile[k * (TD + 1) + c] = v;
+#pragma unroll
+            for (int i = 0; i < VR; i++)
+                if (k + i < TK) tile[(k + i) * (TC + 1) + c] = v[i];
         }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 1024 / kBlock; i++) {
-            const uint32_t e = threadIdx.x + i * kBlock;
-            const uint32_t k = e >> tdShift, c = e & (TD - 1);
-            if (k0 + k < a.K && c0 + c < a.C && a.map[(c0 + c) * a.per + j] != kInvalidRow) {
-                acc[i] = combine<T, XOR>(acc[i], tile[k * (TD + 1) + c]);
-                any[i] = true;
+        for (int o = 0; o < OWN; o++) {
+            const uint32_t e = threadIdx.x + o * kBlock;
+            const uint32_t k = e / vecPerK, c = (e % vecPerK) * VM;
+            if (k < TK && k0 + k < a.K) {
+#pragma unroll
+                for (int i = 0; i < VM; i++) {
+                    const uint64_t gc = c0 + c + i;
+                    if (gc < a.C && a.map[gc * a.per + j] != kInvalidRow) {
+                        acc[o][i] = combine<T, XOR>(acc[o][i], tile[k * (TC + 1) + c + i]);
+                        any[o][i] = true;
+                    }
+                }
             }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 1024 / kBlock; i++) {
-        const uint32_t e = threadIdx.x + i * kBlock;
-        const uint32_t k = e >> tdShift, c = e & (TD - 1);
-        if (k0 + k < a.K && c0 + c < a.C && (any[i] || !a.accumulate)) {
-            out[(uint64_t)(k0 + k) * a.outCols + c0 + c] = acc[i];
+    for (int o = 0; o < OWN; o++) {
+        const uint32_t e = threadIdx.x + o * kBlock;
+        const uint32_t k = e / vecPerK, c = (e % vecPerK) * VM;
+        if (k >= TK || k0 + k >= a.K) continue;
+        T* dst = out + (uint64_t)(k0 + k) * a.outCols + c0 + c;
+        bool full = c0 + c + VM <= a.C;
+#pragma unroll
+        for (int i = 0; i < VM; i++) full = full && (any[o][i] || !a.accumulate);
+        if constexpr (VM > 1) {
+            if (full) {
+                typename RawVec<16>::type raw;
+                __builtin_memcpy(&raw, acc[o], 16);
+                *reinterpret_cast<typename RawVec<16>::type*>(dst) = raw;
+                continue;
+            }
         }
+        for (int i = 0; i < VM; i++)
+            if (c0 + c + i < a.C && (any[o][i] || !a.accumulate)) dst[i] = acc[o][i];
     }
 }
 

@@ -20,6 +20,7 @@
 #include "grgl_hip.h"
 #include "host_graph.h"
 #include "levelize.h"
+#include "stacking.h"
 
 namespace py = pybind11;
 using namespace grgl_amd;
@@ -486,6 +487,15 @@ PYBIND11_MODULE(_grgl_amd, m) {
         },
         py::arg("grg"), py::arg("direction"),
         "The level-bucketed gather plan the device library builds for one direction (for inspection).");
+
+    m.def(
+        "_stack_offsets",
+        [](const std::vector<uint32_t>& rows, size_t elemSize) {
+            uint64_t total = 0;
+            std::vector<uint32_t> off = stackOffsets(rows, elemSize, total);
+            return std::make_pair(off, total);
+        },
+        py::arg("rows"), py::arg("elem_size"), "Stacked-row offsets the batch path assigns (for inspection).");
 
     m.def("set_device", [](int d) { g_device = d; }, py::arg("device"));
     m.def("get_device", []() { return g_device; });

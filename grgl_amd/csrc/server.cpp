@@ -58,6 +58,7 @@ struct Slot {
     size_t budgetBytes = 0;
     grgl_graph* graph = nullptr;
     std::vector<std::shared_ptr<ServerRequest>> reqs;
+    std::vector<uint32_t> offsets; // first stacked row of each request
     size_t inBytes = 0;   // per stacked row
     size_t outBytes = 0;  // per stacked row
     uint64_t edgeRhs = 0;
@@ -119,13 +120,12 @@ bool Server::pollSlots() {
         const hipError_t q = hipEventQuery(s.done);
         if (q == hipErrorNotReady) continue;
         const auto now = Clock::now();
-        const size_t outCols0 = 0;
-        (void)outCols0;
-        size_t k0 = 0;
         uint32_t Ktot = 0;
-        for (auto& r : s.reqs) Ktot += r->rows;
+        for (size_t i = 0; i < s.reqs.size(); i++) Ktot = std::max(Ktot, s.offsets[i] + s.reqs[i]->rows);
         char* pinnedOut = s.pinned + alignUp(s.inBytes * Ktot);
-        for (auto& r : s.reqs) {
+        for (size_t i = 0; i < s.reqs.size(); i++) {
+            auto& r = s.reqs[i];
+            const size_t k0 = s.offsets[i];
             if (q == hipSuccess) {
                 std::memcpy(r->hostOut, pinnedOut + k0 * s.outBytes, (size_t)r->rows * s.outBytes);
                 r->state = ReqState::Done;
@@ -133,13 +133,12 @@ bool Server::pollSlots() {
                 r->state = ReqState::Failed;
                 r->error = hipGetErrorString(q);
             }
-            k0 += r->rows;
             r->latencyMs = std::chrono::duration<double, std::milli>(now - r->submitted).count();
             latencies.push_back(r->latencyMs);
             stats.completed++;
         }
         stats.batches++;
-        stats.rows_processed += Ktot;
+        for (auto& r : s.reqs) stats.rows_processed += r->rows;
         stats.edge_rhs += s.edgeRhs;
         hbmInUse -= s.budgetBytes;
         s.budgetBytes = 0;
@@ -167,15 +166,22 @@ bool Server::launchBatch(Slot& s, std::unique_lock<std::mutex>& lk) {
         return alignUp(rowsBufferBytes(g, head->dir, K, es)) + alignUp(K * inCols * es) + alignUp(K * outCols * es);
     };
     std::vector<std::shared_ptr<ServerRequest>> picked;
+    std::vector<uint32_t> offsets, rowsOf;
     uint32_t K = 0;
     const uint64_t budgetLeft = cfg.hbm_budget_bytes > hbmInUse ? cfg.hbm_budget_bytes - hbmInUse : 0;
     for (auto& r : queue) {
         if (r->state != ReqState::Queued) continue;
         if (r->graphId != head->graphId || r->dtype != head->dtype || r->dir != head->dir) continue;
-        if (!picked.empty() && K + r->rows > cfg.max_batch_rows) break;
-        if (batchBytes(K + r->rows) > budgetLeft) break;
+        rowsOf.push_back(r->rows);
+        uint64_t total = 0;
+        std::vector<uint32_t> trial = stackOffsets(rowsOf, es, total);
+        if ((!picked.empty() && total > cfg.max_batch_rows) || batchBytes((uint32_t)total) > budgetLeft) {
+            rowsOf.pop_back();
+            break;
+        }
         picked.push_back(r);
-        K += r->rows;
+        offsets = std::move(trial);
+        K = (uint32_t)total;
     }
     if (picked.empty()) {
         // The head alone does not fit what is left of the budget.
@@ -224,12 +230,13 @@ bool Server::launchBatch(Slot& s, std::unique_lock<std::mutex>& lk) {
     char* devOut = devIn + alignUp(K * inCols * es);
     char* pinIn = s.pinned;
     std::vector<TravRequest> treqs;
-    uint32_t k0 = 0;
-    for (auto& r : picked) {
+    for (size_t i = 0; i < picked.size(); i++) {
+        auto& r = picked[i];
+        const uint32_t k0 = offsets[i];
         std::memcpy(pinIn + (size_t)k0 * inCols * es, r->hostIn, (size_t)r->rows * inCols * es);
         treqs.push_back({r->rows, devIn + (size_t)k0 * inCols * es, devOut + (size_t)k0 * outCols * es, k0});
-        k0 += r->rows;
     }
+    s.offsets = offsets;
     try {
         GRGL_HIP_CHECK(hipMemcpyAsync(devIn, pinIn, (size_t)K * inCols * es, hipMemcpyHostToDevice, s.stream));
         TravOptions opt;
@@ -264,15 +271,25 @@ void Server::run() {
             for (auto& s : slots) busy |= s.busy;
             if (!busy) break;
         }
+        // Batching window: hold a free slot until the oldest queued request has waited
+        // batch_window_us, so later arrivals can share its traversal.
+        bool windowOpen = false;
+        Clock::time_point windowEnd;
+        if (!queue.empty() && cfg.batch_window_us > 0 && !stop) {
+            windowEnd = queue.front()->submitted + std::chrono::microseconds(cfg.batch_window_us);
+            windowOpen = Clock::now() < windowEnd;
+        }
         for (auto& s : slots) {
-            if (s.busy || stop) continue;
+            if (s.busy || stop || windowOpen) continue;
             if (queue.empty()) break;
             progressed |= launchBatch(s, lk);
         }
         bool busy = false;
         for (auto& s : slots) busy |= s.busy;
         if (!progressed) {
-            if (busy) {
+            if (windowOpen) {
+                cvWork.wait_until(lk, windowEnd);
+            } else if (busy) {
                 // Batches in flight: check again shortly (event completion is polled).
                 cvWork.wait_for(lk, std::chrono::microseconds(50));
             } else {

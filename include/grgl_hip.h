@@ -129,10 +129,11 @@ grgl_status grgl_hip_matmul(grgl_graph* graph, grgl_dtype dtype, const void* inp
 
 /*
  * Several requests on one graph, one direction and one dtype carried by ONE traversal:
- * their rows are stacked (sum of rows <= 65536) and share every edge read.  Each request
- * keeps its own input/output buffers and gets the same result as a grgl_hip_matmul call
- * (bit-identical for integers and for GRGL_FLAG_ORDER_PRESERVING; floats otherwise within
- * summation-order rounding).  No init/miss/emit_all/by_individual here.
+ * their rows are stacked (each request starting on a 16-byte boundary of the row; stacked
+ * rows <= 65536) and share every edge read.  Each request keeps its own input/output
+ * buffers and gets bit-for-bit the result of a grgl_hip_matmul call with the same flags:
+ * every column is summed by the same arithmetic whatever else is stacked beside it.
+ * No init/miss/emit_all/by_individual here.
  */
 typedef struct grgl_request {
     uint32_t rows;
@@ -184,6 +185,8 @@ typedef struct grgl_server_config {
     uint64_t hbm_budget_bytes; /* device bytes for in-flight batches; default 8 GiB */
     uint32_t max_queue;        /* queued requests before submit returns BUSY; default 4096 */
     uint32_t flags;            /* GRGL_FLAG_ORDER_PRESERVING allowed */
+    uint32_t batch_window_us;  /* a free slot waits up to this long after the oldest queued request
+                                  arrived, to stack more requests into its batch; default 0 */
 } grgl_server_config;
 
 typedef struct grgl_server_stats {

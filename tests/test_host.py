@@ -191,3 +191,26 @@ def test_pygrgl_properties():
 def test_no_gpu_fails_loudly():
     g = grgl_amd.grg_from_coalescent(100, 200)
     _expect_runtime_error(lambda: grgl_amd.matmul(g, np.ones((1, 100)), grgl_amd.UP))
+
+
+@pytest.mark.parametrize("es", [1, 2, 4, 8])
+def test_stacked_requests_never_share_a_vector(es):
+    """Batched requests: a request >= 16 bytes wide starts 16-byte aligned and owns whole
+    16-byte slots, so its vector accesses past its last row touch no other request."""
+    rng = np.random.default_rng(es)
+    stack = grgl_amd._grgl_amd._stack_offsets
+    off, total = stack([1, 2, 4, 8, 16, 32, 64, 128, 256], 4)
+    assert total == 511
+    for _ in range(200):
+        rows = [int(x) for x in rng.choice([1, 2, 3, 5, 8, 17, 31, 64, 100], size=int(rng.integers(1, 12)))]
+        off, total = stack(rows, es)
+        v = max(1, 16 // es)
+        used = np.zeros(total + 64, dtype=np.int64) - 1
+        for i, (o, r) in enumerate(zip(off, rows)):
+            span = r
+            if r * es >= 16:
+                assert o % v == 0
+                span = -(-r // v) * v
+            assert np.all(used[o:o + span] == -1), "overlapping slots"
+            used[o:o + span] = i
+        assert total == max(o + r for o, r in zip(off, rows))
