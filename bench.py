@@ -30,6 +30,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# The headline metric this benchmark reports (edge*RHS/s of the batched traversal, with
+# the HBM roofline fraction in "roofline" and served p50/p99 vs the plain path in "serving").
+METRIC = "GRG matmul edge\u00b7RHS/s + HBM GB/s frac; served p50/p99 vs plain, 1 GPU"
 K_LADDER = [1, 2, 4, 8, 16, 32, 64, 128, 256]
 SEED = 20261203
 
@@ -85,11 +88,161 @@ def cpu_baseline(arrays, E, budget_s=12.0):
     dt = time.perf_counter() - t0
     return {
         "value": E * K * done / dt,
-        "unit": "edge*RHS/s",
+        "unit": "edge\u00b7RHS/s",
         "cores": 1,
         "kind": kind,
         "sample": f"{done} single-thread f32 K=16 traversals (UP and DOWN alternating) of the same graph, "
                   f"{dt:.1f} s",
+    }
+
+
+SERVE_GRAPHS = [(10_000, 50_000, 10), (100_000, 500_000, 20), (200_000, 1_000_000, 50)]
+SERVE_K = [1, 8, 32, 128]
+SERVE_P = [0.4, 0.3, 0.2, 0.1]
+
+
+def serve_benchmark(num_requests=2000, load=0.6):
+    """Resident graphs, mixed requests: continuous batching vs one request per traversal.
+
+    Requests pick a graph uniformly, K from {1,8,32,128} (p .4/.3/.2/.1), a direction
+    uniformly and f32 (70%) or i32 (30%).  Inputs and outputs are device buffers (one
+    input/output pair per graph, direction and dtype, sized for K=128; every request
+    reads rows [0, K) and writes rows [0, K) of them, so concurrent requests of one key
+    write identical values).  Scenarios:
+      burst   all requests queued at once: throughput (requests/s, edge*RHS/s);
+      poisson arrivals at `load` x the plain path's burst throughput: p50/p99 latency
+              (arrival -> result written), measured by the server for the served path and
+              by the caller around each synchronous call for the plain path.
+    """
+    import numpy as np
+    import torch
+
+    import grgl_amd
+    from grgl_amd import capi
+
+    sp = torch.cuda.current_stream().cuda_stream
+    rng = np.random.default_rng(SEED + 5)
+    graphs = []
+    for n, m, t in SERVE_GRAPHS:
+        g = grgl_amd.grg_from_coalescent(n, m, t, seed=SEED + n)
+        cg = capi.Graph(g.arrays())
+        graphs.append(cg)
+    bufs = {}
+    for gi, cg in enumerate(graphs):
+        for d in (capi.UP, capi.DOWN):
+            cin, cout = (cg.num_samples, cg.num_mutations) if d == capi.UP else (cg.num_mutations, cg.num_samples)
+            for cdt, tdt in ((capi.F32, torch.float32), (capi.I32, torch.int32)):
+                x = torch.randn((128, cin), device="cuda").to(tdt)
+                y = torch.zeros((128, cout), device="cuda", dtype=tdt)
+                bufs[(gi, d, cdt)] = (x, y, cin, cout)
+    edges = [cg.info()["num_edges"] for cg in graphs]
+    reqs = []
+    for _ in range(num_requests):
+        gi = int(rng.integers(0, len(graphs)))
+        K = int(rng.choice(SERVE_K, p=SERVE_P))
+        d = capi.UP if rng.random() < 0.5 else capi.DOWN
+        cdt = capi.F32 if rng.random() < 0.7 else capi.I32
+        reqs.append((gi, K, d, cdt))
+    edge_rhs = sum(edges[gi] * K for gi, K, _, _ in reqs)
+
+    def plain_one(r):
+        gi, K, d, cdt = r
+        x, y, cin, cout = bufs[(gi, d, cdt)]
+        graphs[gi].matmul_ptr(cdt, x.data_ptr(), cin, K, d, y.data_ptr(), K * cout, stream=sp)
+
+    # Warm every (graph, direction, dtype, K) shape once on both paths.
+    for gi in range(len(graphs)):
+        for d in (capi.UP, capi.DOWN):
+            for cdt in (capi.F32, capi.I32):
+                for K in SERVE_K:
+                    plain_one((gi, K, d, cdt))
+    torch.cuda.synchronize()
+    srv = capi.Server(num_streams=2, max_batch_rows=512, hbm_budget_bytes=48 << 30)
+    gids = [srv.add_graph(cg) for cg in graphs]
+
+    def served_submit(r):
+        gi, K, d, cdt = r
+        x, y, cin, cout = bufs[(gi, d, cdt)]
+        return srv.submit_device(gids[gi], cdt, d, K, x.data_ptr(), y.data_ptr())
+
+    for r in reqs[:64]:
+        srv.wait(served_submit(r))
+    batches0 = srv.stats(reset=True)["batches"]
+
+    # Burst: plain.
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for r in reqs:
+        plain_one(r)
+    torch.cuda.synchronize()
+    plain_burst = time.perf_counter() - t
+    # Burst: served.
+    t = time.perf_counter()
+    ids = [served_submit(r) for r in reqs]
+    for rid in ids:
+        srv.wait(rid)
+    served_burst = time.perf_counter() - t
+    st_burst = srv.stats(reset=True)
+    batches1 = st_burst["batches"]
+    plain_rate = num_requests / plain_burst
+    rate = load * plain_rate
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=num_requests))
+
+    def spin_until(t0, at):
+        while True:
+            now = time.perf_counter() - t0
+            if now >= at:
+                return
+            if at - now > 2e-4:
+                time.sleep((at - now) / 2)
+
+    # Poisson: plain (FIFO, one at a time).
+    lat = []
+    t0 = time.perf_counter()
+    for r, at in zip(reqs, arrivals):
+        spin_until(t0, at)
+        plain_one(r)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t0 - at) * 1e3)
+    plain_poisson_wall = time.perf_counter() - t0
+    # Poisson: served.
+    t0 = time.perf_counter()
+    ids = []
+    for r, at in zip(reqs, arrivals):
+        spin_until(t0, at)
+        ids.append(served_submit(r))
+    for rid in ids:
+        srv.wait(rid)
+    served_poisson_wall = time.perf_counter() - t0
+    st = srv.stats(reset=True)
+    srv.close()
+    for cg in graphs:
+        cg.close()
+    lat = np.array(lat)
+    return {
+        "workload": "3 resident graphs (10k x 50k / 100k x 500k / 200k x 1M), K in {1,8,32,128} p .4/.3/.2/.1, "
+                    "direction uniform, 70% f32 / 30% i32, device-resident request buffers",
+        "requests": num_requests,
+        "burst": {
+            "plain_req_per_s": round(plain_rate, 1),
+            "served_req_per_s": round(num_requests / served_burst, 1),
+            "plain_edge_rhs_per_s": edge_rhs / plain_burst,
+            "served_edge_rhs_per_s": edge_rhs / served_burst,
+            "speedup": round(plain_burst / served_burst, 3),
+            "served_batches": batches1 - batches0,
+            "served_p50_ms": round(st_burst["latency_p50_ms"], 3),
+            "served_p99_ms": round(st_burst["latency_p99_ms"], 3),
+        },
+        "poisson": {
+            "offered_req_per_s": round(rate, 1),
+            "plain_p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "plain_p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "served_p50_ms": round(st["latency_p50_ms"], 3),
+            "served_p99_ms": round(st["latency_p99_ms"], 3),
+            "plain_wall_s": round(plain_poisson_wall, 3),
+            "served_wall_s": round(served_poisson_wall, 3),
+            "served_batches": st["batches"] - batches1,
+        },
     }
 
 
@@ -104,6 +257,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-request-per-traversal sweep")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving (mixed request stream) scenario")
+    ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -244,14 +399,19 @@ def main():
         "launches": k_launches,
         "avg_launch_us": round(k_ms / max(1, k_launches) * 1e3, 2),
     }
+    serving = None
+    if not args.no_serving:
+        log("serving scenario ...")
+        serving = serve_benchmark(args.serve_requests)
+        log(f"serving: {json.dumps(serving)}")
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU baseline ...")
         cpu = cpu_baseline(arrays, E)
     result = {
-        "metric": "GRG matmul edge*RHS/s (batched K-ladder, UP+DOWN, f32+i32)",
+        "metric": METRIC,
         "value": value,
-        "unit": "edge*RHS/s",
+        "unit": "edge\u00b7RHS/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -278,6 +438,7 @@ def main():
         "plain": plain,
         "batched_vs_plain_f32": round(plain_ms_total / batched_f32_ms, 3) if plain_ms_total else None,
         "roofline": roofline,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)

@@ -63,6 +63,7 @@ EXPORTED_SYMBOLS = [
     "grgl_hip_server_create",
     "grgl_hip_server_add_graph",
     "grgl_hip_server_submit",
+    "grgl_hip_server_submit_device",
     "grgl_hip_server_poll",
     "grgl_hip_server_wait",
     "grgl_hip_server_cancel",
@@ -187,6 +188,8 @@ def lib():
         L.grgl_hip_server_add_graph.argtypes = [vp, vp, ctypes.POINTER(u32)]
         L.grgl_hip_server_submit.restype = ci
         L.grgl_hip_server_submit.argtypes = [vp, u32, ci, ci, u32, vp, vp, ctypes.POINTER(u64)]
+        L.grgl_hip_server_submit_device.restype = ci
+        L.grgl_hip_server_submit_device.argtypes = [vp, u32, ci, ci, u32, vp, vp, ctypes.POINTER(u64)]
         L.grgl_hip_server_poll.restype = ci
         L.grgl_hip_server_poll.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_double)]
         L.grgl_hip_server_wait.restype = ci
@@ -355,6 +358,13 @@ class Server:
                                            x.ctypes.data, out.ctypes.data, ctypes.byref(rid)))
         self._live[rid.value] = (x, out)
         return rid.value, out
+
+    def submit_device(self, graph_id, dtype, direction, rows, in_ptr, out_ptr):
+        """Device buffers (e.g. torch tensors' data_ptr()); they must outlive the request."""
+        rid = ctypes.c_uint64()
+        check(lib().grgl_hip_server_submit_device(self.handle, graph_id, dtype, direction, rows, in_ptr, out_ptr,
+                                                  ctypes.byref(rid)))
+        return rid.value
 
     def _finish(self, rid, fn):
         lat = ctypes.c_double()

@@ -41,6 +41,7 @@ struct ServerRequest {
     uint32_t rows = 0;
     const void* hostIn = nullptr;
     void* hostOut = nullptr;
+    bool device = false;          // in/out are device buffers used in place
     Clock::time_point submitted;
     ReqState state = ReqState::Queued;
     double latencyMs = 0.0;
@@ -127,7 +128,7 @@ bool Server::pollSlots() {
             auto& r = s.reqs[i];
             const size_t k0 = s.offsets[i];
             if (q == hipSuccess) {
-                std::memcpy(r->hostOut, pinnedOut + k0 * s.outBytes, (size_t)r->rows * s.outBytes);
+                if (!r->device) std::memcpy(r->hostOut, pinnedOut + k0 * s.outBytes, (size_t)r->rows * s.outBytes);
                 r->state = ReqState::Done;
             } else {
                 r->state = ReqState::Failed;
@@ -162,8 +163,12 @@ bool Server::launchBatch(Slot& s, std::unique_lock<std::mutex>& lk) {
     const size_t es = dtypeSize(head->dtype);
     const size_t inCols = head->dir == GRGL_DIRECTION_UP ? g->N : g->M;
     const size_t outCols = head->dir == GRGL_DIRECTION_UP ? g->M : g->N;
+    // Device-buffer requests need no staging; the budget counts the row buffer plus staging
+    // for the rows of host requests (charged as if every stacked row were staged).
+    bool hostRows = false;
     auto batchBytes = [&](uint32_t K) {
-        return alignUp(rowsBufferBytes(g, head->dir, K, es)) + alignUp(K * inCols * es) + alignUp(K * outCols * es);
+        const size_t staging = hostRows ? alignUp(K * inCols * es) + alignUp(K * outCols * es) : 0;
+        return alignUp(rowsBufferBytes(g, head->dir, K, es)) + staging;
     };
     std::vector<std::shared_ptr<ServerRequest>> picked;
     std::vector<uint32_t> offsets, rowsOf;
@@ -173,10 +178,13 @@ bool Server::launchBatch(Slot& s, std::unique_lock<std::mutex>& lk) {
         if (r->state != ReqState::Queued) continue;
         if (r->graphId != head->graphId || r->dtype != head->dtype || r->dir != head->dir) continue;
         rowsOf.push_back(r->rows);
+        const bool savedHost = hostRows;
+        hostRows = hostRows || !r->device;
         uint64_t total = 0;
         std::vector<uint32_t> trial = stackOffsets(rowsOf, es, total);
         if ((!picked.empty() && total > cfg.max_batch_rows) || batchBytes((uint32_t)total) > budgetLeft) {
             rowsOf.pop_back();
+            hostRows = savedHost;
             break;
         }
         picked.push_back(r);
@@ -185,6 +193,7 @@ bool Server::launchBatch(Slot& s, std::unique_lock<std::mutex>& lk) {
     }
     if (picked.empty()) {
         // The head alone does not fit what is left of the budget.
+        hostRows = !head->device;
         if (batchBytes(head->rows) > cfg.hbm_budget_bytes) {
             head->state = ReqState::Failed;
             head->error = "request exceeds the server's HBM budget";
@@ -200,7 +209,7 @@ bool Server::launchBatch(Slot& s, std::unique_lock<std::mutex>& lk) {
     try {
         GRGL_HIP_CHECK(hipSetDevice(cfg.device));
         growDevice(s, bytes);
-        growPinned(s, alignUp(K * inCols * es) + alignUp(K * outCols * es));
+        if (hostRows) growPinned(s, alignUp(K * inCols * es) + alignUp(K * outCols * es));
     } catch (const std::exception& e) {
         for (auto& r : picked) {
             r->state = ReqState::Failed;
@@ -230,20 +239,28 @@ bool Server::launchBatch(Slot& s, std::unique_lock<std::mutex>& lk) {
     char* devOut = devIn + alignUp(K * inCols * es);
     char* pinIn = s.pinned;
     std::vector<TravRequest> treqs;
+    bool anyHost = false;
     for (size_t i = 0; i < picked.size(); i++) {
         auto& r = picked[i];
         const uint32_t k0 = offsets[i];
-        std::memcpy(pinIn + (size_t)k0 * inCols * es, r->hostIn, (size_t)r->rows * inCols * es);
-        treqs.push_back({r->rows, devIn + (size_t)k0 * inCols * es, devOut + (size_t)k0 * outCols * es, k0});
+        if (r->device) {
+            treqs.push_back({r->rows, r->hostIn, r->hostOut, k0});
+        } else {
+            anyHost = true;
+            std::memcpy(pinIn + (size_t)k0 * inCols * es, r->hostIn, (size_t)r->rows * inCols * es);
+            treqs.push_back({r->rows, devIn + (size_t)k0 * inCols * es, devOut + (size_t)k0 * outCols * es, k0});
+        }
     }
     s.offsets = offsets;
     try {
-        GRGL_HIP_CHECK(hipMemcpyAsync(devIn, pinIn, (size_t)K * inCols * es, hipMemcpyHostToDevice, s.stream));
+        if (anyHost)
+            GRGL_HIP_CHECK(hipMemcpyAsync(devIn, pinIn, (size_t)K * inCols * es, hipMemcpyHostToDevice, s.stream));
         TravOptions opt;
         opt.orderPreserving = (cfg.flags & GRGL_FLAG_ORDER_PRESERVING) != 0;
         enqueueTraversal(g, head->dtype, head->dir, treqs, K, opt, s.dev, s.stream);
-        GRGL_HIP_CHECK(hipMemcpyAsync(s.pinned + alignUp(K * inCols * es), devOut, (size_t)K * outCols * es,
-                                      hipMemcpyDeviceToHost, s.stream));
+        if (anyHost)
+            GRGL_HIP_CHECK(hipMemcpyAsync(s.pinned + alignUp(K * inCols * es), devOut, (size_t)K * outCols * es,
+                                          hipMemcpyDeviceToHost, s.stream));
         GRGL_HIP_CHECK(hipEventRecord(s.done, s.stream));
     } catch (const std::exception& e) {
         // Wait for whatever was enqueued, then fail the batch.
@@ -386,9 +403,9 @@ GRGL_EXPORT grgl_status grgl_hip_server_add_graph(grgl_server* server, grgl_grap
     return GRGL_OK;
 }
 
-GRGL_EXPORT grgl_status grgl_hip_server_submit(grgl_server* server, uint32_t graph_id, grgl_dtype dtype,
-                                               grgl_direction direction, uint32_t rows, const void* host_input,
-                                               void* host_output, uint64_t* request_id) {
+namespace {
+grgl_status submitImpl(grgl_server* server, uint32_t graph_id, grgl_dtype dtype, grgl_direction direction,
+                       uint32_t rows, const void* host_input, void* host_output, bool device, uint64_t* request_id) {
     if (!server || !request_id || !host_input || !host_output) return fail(GRGL_ERR_MISUSE, "null argument");
     Server* srv = S(server);
     try {
@@ -416,6 +433,7 @@ GRGL_EXPORT grgl_status grgl_hip_server_submit(grgl_server* server, uint32_t gra
         r->rows = rows;
         r->hostIn = host_input;
         r->hostOut = host_output;
+        r->device = device;
         r->submitted = Clock::now();
         srv->queue.push_back(r);
         srv->byId[r->id] = r;
@@ -423,6 +441,20 @@ GRGL_EXPORT grgl_status grgl_hip_server_submit(grgl_server* server, uint32_t gra
     }
     srv->cvWork.notify_one();
     return GRGL_OK;
+}
+} // namespace
+
+GRGL_EXPORT grgl_status grgl_hip_server_submit(grgl_server* server, uint32_t graph_id, grgl_dtype dtype,
+                                               grgl_direction direction, uint32_t rows, const void* host_input,
+                                               void* host_output, uint64_t* request_id) {
+    return submitImpl(server, graph_id, dtype, direction, rows, host_input, host_output, false, request_id);
+}
+
+GRGL_EXPORT grgl_status grgl_hip_server_submit_device(grgl_server* server, uint32_t graph_id, grgl_dtype dtype,
+                                                      grgl_direction direction, uint32_t rows,
+                                                      const void* device_input, void* device_output,
+                                                      uint64_t* request_id) {
+    return submitImpl(server, graph_id, dtype, direction, rows, device_input, device_output, true, request_id);
 }
 
 GRGL_EXPORT grgl_status grgl_hip_server_poll(grgl_server* server, uint64_t request_id, double* latency_ms) {

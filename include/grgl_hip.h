@@ -210,6 +210,12 @@ grgl_status grgl_hip_server_add_graph(grgl_server* server, grgl_graph* graph, ui
 grgl_status grgl_hip_server_submit(grgl_server* server, uint32_t graph_id, grgl_dtype dtype,
                                    grgl_direction direction, uint32_t rows, const void* host_input,
                                    void* host_output, uint64_t* request_id);
+/* Same, with device buffers (any device allocation on the server's device): no staging
+ * copies; the traversal reads the input and writes the output in place.  Both buffers
+ * must stay valid until the request completes. */
+grgl_status grgl_hip_server_submit_device(grgl_server* server, uint32_t graph_id, grgl_dtype dtype,
+                                          grgl_direction direction, uint32_t rows, const void* device_input,
+                                          void* device_output, uint64_t* request_id);
 /* GRGL_OK when done (result written), GRGL_ERR_NOT_READY while queued or running. */
 grgl_status grgl_hip_server_poll(grgl_server* server, uint64_t request_id, double* latency_ms);
 grgl_status grgl_hip_server_wait(grgl_server* server, uint64_t request_id, double* latency_ms);

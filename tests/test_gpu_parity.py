@@ -270,3 +270,14 @@ def test_config3_scale_properties(gpu):
     lhs = int((grgl_amd.matmul(g, v, UP) * w).sum())
     rhs = int((v * grgl_amd.matmul(g, w, DOWN)).sum())
     assert lhs == rhs
+
+
+def test_cpp_shim_reference_unit_tests(gpu, tmp_path):
+    """The reference's C++ matmul unit tests through include/grgl_hip.hpp on the device."""
+    import subprocess
+
+    from test_host import _build_shim_test
+
+    exe = _build_shim_test(tmp_path)
+    proc = subprocess.run([str(exe)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert proc.returncode == 0, proc.stdout

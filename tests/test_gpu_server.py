@@ -111,3 +111,31 @@ def test_cancel_and_queue_full(graphs):
     st = srv.stats()
     assert st["cancelled"] == cancelled and st["rejected"] == busy
     srv.close()
+
+
+def test_device_submit_zero_copy_equals_alone(graphs):
+    import torch
+
+    srv = capi.Server(num_streams=2, max_batch_rows=512, batch_window_us=20000)
+    ids = [srv.add_graph(cg) for _, cg, _ in graphs]
+    rng = np.random.default_rng(9)
+    subs = []
+    for _ in range(40):
+        gi = int(rng.integers(0, 2))
+        _, cg, og = graphs[gi]
+        d = capi.UP if rng.random() < 0.5 else capi.DOWN
+        K = int(rng.choice([1, 3, 8, 32, 100]))
+        dt = np.float32 if rng.random() < 0.5 else np.int32
+        cin, cout = (cg.num_samples, cg.num_mutations) if d == capi.UP else (cg.num_mutations, cg.num_samples)
+        x = make_input(rng, dt, (K, cin))
+        xd = torch.from_numpy(x).cuda()
+        yd = torch.zeros((K, cout), dtype=xd.dtype, device="cuda")
+        torch.cuda.synchronize()
+        rid = srv.submit_device(ids[gi], capi.NUMPY_TO_DTYPE[np.dtype(dt)], d, K, xd.data_ptr(), yd.data_ptr())
+        subs.append((rid, x, xd, yd, gi, d))
+    for rid, x, xd, yd, gi, d in subs:
+        srv.wait(rid)
+        _, cg, og = graphs[gi]
+        assert bits_equal(yd.cpu().numpy(), cg.matmul_host(x, d))
+    assert srv.stats()["batches"] < 40
+    srv.close()

@@ -214,3 +214,19 @@ def test_stacked_requests_never_share_a_vector(es):
             assert np.all(used[o:o + span] == -1), "overlapping slots"
             used[o:o + span] = i
         assert total == max(o + r for o, r in zip(off, rows))
+
+
+def _build_shim_test(tmp_path):
+    exe = tmp_path / "shim_matmul_test"
+    lib_dir = os.path.join(ROOT, "grgl_amd")
+    import subprocess
+
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+                    os.path.join(ROOT, "tests", "cpp", "shim_matmul_test.cpp"), f"-L{lib_dir}", "-lgrgl_hip",
+                    f"-Wl,-rpath,{lib_dir}", "-o", str(exe)], check=True)
+    return exe
+
+
+def test_cpp_shim_compiles_and_links(tmp_path):
+    """include/grgl_hip.hpp (GRG::matMul / matrixMultiplication surface) builds against the library."""
+    assert _build_shim_test(tmp_path).exists()
