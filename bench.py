@@ -246,6 +246,82 @@ def serve_benchmark(num_requests=2000, load=0.6):
     }
 
 
+def plain_at_scale():
+    """One request per traversal on 100,000 haplotypes x 500,000 sites (20 trees):
+    K = 1 and 16, FP64 and FP32, UP and DOWN (ms per traversal, edge*RHS/s, algorithmic GB/s)."""
+    import torch
+
+    import grgl_amd
+    from grgl_amd import capi
+
+    g = grgl_amd.grg_from_coalescent(100_000, 500_000, 20, seed=SEED)
+    cg = capi.Graph(g.arrays())
+    info = cg.info()
+    V, E, N, M = info["num_nodes"], info["num_edges"], info["num_samples"], info["num_mutations"]
+    sp = torch.cuda.current_stream().cuda_stream
+    out = {"workload": "100,000 haplotypes x 500,000 sites, 20 trees", "nodes": V, "edges": E}
+    for tdt, cdt, es, name in ((torch.float64, capi.F64, 8, "f64"), (torch.float32, capi.F32, 4, "f32")):
+        for d, dn in ((capi.UP, "UP"), (capi.DOWN, "DOWN")):
+            cin, cout = (N, M) if d == capi.UP else (M, N)
+            for K in (1, 16):
+                x = torch.randn((K, cin), device="cuda", dtype=tdt)
+                y = torch.zeros((K, cout), device="cuda", dtype=tdt)
+                cg.autotune(d, cdt, K)
+                cg.matmul_ptr(cdt, x.data_ptr(), cin, K, d, y.data_ptr(), y.numel(), stream=sp)
+                torch.cuda.synchronize()
+                reps = 20
+                t = time.perf_counter()
+                for _ in range(reps):
+                    cg.matmul_ptr(cdt, x.data_ptr(), cin, K, d, y.data_ptr(), y.numel(), stream=sp)
+                torch.cuda.synchronize()
+                ms = (time.perf_counter() - t) / reps * 1e3
+                out[f"{name}_{dn}_K{K}"] = {"ms": round(ms, 4), "edge_rhs_per_s": E * K / (ms / 1e3),
+                                            "alg_GBps": round(algorithmic_bytes(V, E, K, es, cin, cout) / (ms / 1e3) / 1e9, 1)}
+    cg.close()
+    return out
+
+
+def pca_benchmark():
+    """Standardized PCA: 50,000 diploid individuals (100,000 haplotypes) x 300,000 sites,
+    15 segment trees; randomized SVD, block 20 (10 + 10 oversampling), 5 power iterations;
+    FP32 traversals, FP64 statistics and QR on the GPU.  Errors against the FP64 CPU oracle
+    running the same algorithm with the same random block."""
+    import numpy as np
+    import torch
+
+    import grgl_amd
+    from grgl_amd import capi
+    from grgl_amd.pca import DeviceStdOperator, eigen_errors, randomized_pca
+
+    g = grgl_amd.grg_from_coalescent(100_000, 300_000, 15, seed=SEED + 4)
+    arrays = g.arrays()
+    cg = capi.Graph(arrays)
+    op = DeviceStdOperator(cg)
+    randomized_pca(op, k=10)  # warm-up
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    e, U = randomized_pca(op, k=10)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t
+    out = {"workload": "50,000 diploids (100,000 haplotypes) x 300,000 sites, 15 trees; k=10, block 20, "
+                       "5 power iterations", "gpu_ms": round(gpu_s * 1e3, 2), "traversals": 12,
+           "top10_eigenvalues": [float(x) for x in e]}
+    try:
+        from oracle.pca_oracle import OracleStdOperator
+        from oracle.pca_oracle import randomized_pca as oracle_pca
+
+        t = time.perf_counter()
+        re, rU = oracle_pca(OracleStdOperator(arrays), k=10)
+        out["cpu_oracle_fp64_s"] = round(time.perf_counter() - t, 2)
+        rel, ang = eigen_errors(e, U, re, rU)
+        out["max_eigenvalue_rel_err"] = float(rel.max())
+        out["max_eigenvector_angle_rad"] = float(ang.max())
+    except ImportError:
+        pass
+    cg.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +335,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving (mixed request stream) scenario")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--no-pca", action="store_true", help="skip the standardized PCA scenario")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -404,6 +481,15 @@ def main():
         log("serving scenario ...")
         serving = serve_benchmark(args.serve_requests)
         log(f"serving: {json.dumps(serving)}")
+    at_scale = None
+    if not args.no_plain:
+        log("plain path at scale (100k x 500k) ...")
+        at_scale = plain_at_scale()
+    pca = None
+    if not args.no_pca:
+        log("PCA scenario ...")
+        pca = pca_benchmark()
+        log(f"pca: {json.dumps(pca)}")
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU baseline ...")
@@ -438,7 +524,9 @@ def main():
         "plain": plain,
         "batched_vs_plain_f32": round(plain_ms_total / batched_f32_ms, 3) if plain_ms_total else None,
         "roofline": roofline,
+        "plain_100k_x_500k": at_scale,
         "serving": serving,
+        "pca": pca,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)

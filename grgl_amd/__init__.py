@@ -30,12 +30,14 @@ try:
         get_device,
         grg_from_arrays,
         grg_from_coalescent,
+        load_immutable_grg,
         matmul,
+        save_grg,
         set_device,
     )
 except ImportError as exc:  # pragma: no cover - exercised only when the build is missing
     raise ImportError(
-        "grgl_amd native extension is not built (run `python -m grgl_amd.build`): " + str(exc)
+        "grgl_amd native extension is not built (run `python grgl_amd/build.py`): " + str(exc)
     ) from exc
 
 DOWN = TraversalDirection.DOWN
@@ -54,6 +56,8 @@ __all__ = [
     "get_device",
     "grg_from_arrays",
     "grg_from_coalescent",
+    "load_immutable_grg",
     "matmul",
+    "save_grg",
     "set_device",
 ]

@@ -1,6 +1,6 @@
 """Build the native parts of grgl_amd in-tree.
 
-    python -m grgl_amd.build          # libgrgl_hip.so (gfx950) + _grgl_amd extension
+    python grgl_amd/build.py          # libgrgl_hip.so (gfx950) + _grgl_amd extension
 
 libgrgl_hip.so holds the gfx950 kernels and the C ABI of include/grgl_hip.h (hipcc);
 _grgl_amd*.so is the pybind11 module with the pygrgl surface (g++), linked against
@@ -20,7 +20,7 @@ INCLUDE = os.path.join(ROOT, "include")
 ARCH = os.environ.get("GRGL_AMD_ARCH", "gfx950")
 
 HIP_SOURCES = ["grgl_hip.hip", "server.cpp", "levelize.cpp", "host_graph.cpp", "stacking.cpp"]
-PY_SOURCES = ["pygrgl_module.cpp", "synth.cpp", "host_graph.cpp", "levelize.cpp", "stacking.cpp"]
+PY_SOURCES = ["pygrgl_module.cpp", "synth.cpp", "host_graph.cpp", "levelize.cpp", "stacking.cpp", "grg_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")

@@ -21,6 +21,16 @@ struct HostGrg {
     std::vector<uint32_t> mutMiss;   // per MutationId, empty when there is no missing data
     std::vector<double> mutPos;      // per MutationId (base-pair position)
     std::vector<uint32_t> coal;      // per node, empty when not available
+    // Mutation metadata carried through .grg files (include/grgl/mutation.h:248-261):
+    // allele storage string (REF then ALT, no delimiter), ALT start index, time field.
+    std::vector<std::string> mutAlleles; // empty: "A" -> "G" for every mutation
+    std::vector<uint16_t> mutAltIndex;
+    std::vector<uint32_t> mutTime;
+    std::vector<std::string> populations;
+    std::vector<uint16_t> popIds;        // per sample, empty without populations
+    std::vector<std::string> individualIds;
+    uint64_t rangeStart = 0, rangeEnd = 0;
+    bool phased = true;
     bool hasCoals() const { return !coal.empty(); }
     uint32_t numNodes() const { return downOff.empty() ? 0 : (uint32_t)(downOff.size() - 1); }
     uint64_t numEdges() const { return downOff.empty() ? 0 : downOff.back(); }
@@ -54,5 +64,11 @@ struct GraphStats {
     std::vector<uint64_t> fanInHistogram; // bucket b counts nodes with fan-in in [2^b, 2^(b+1)), b=0 holds 0..1
 };
 GraphStats computeGraphStats(const HostGrg& g);
+
+// .grg files (format v5.x of the reference, src/serialize.cpp:518-669 and :697-890).
+// Reading accepts versions 5.0-5.3 (u32 node index up to 5.1, u64 after); writing produces
+// version 5.3 without simplification (the graph is stored as it is).
+HostGrg readGrgFile(const std::string& path);
+void writeGrgFile(const HostGrg& g, const std::string& path);
 
 } // namespace grgl_amd

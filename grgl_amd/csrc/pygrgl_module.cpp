@@ -103,6 +103,14 @@ struct PyGRG {
 };
 using PyGRGPtr = std::shared_ptr<PyGRG>;
 
+// A mutation's public fields (include/grgl/mutation.h): position, ALT, REF, time.
+struct PyMutation {
+    uint64_t position = 0;
+    std::string allele;
+    std::string ref_allele;
+    uint32_t time = 0;
+};
+
 grgl_dtype dtypeOf(py::handle input, bool& ok) {
     ok = true;
     if (py::isinstance<py::array_t<double>>(input)) return GRGL_F64;
@@ -271,7 +279,9 @@ template <typename T> py::array_t<T> toArray(const std::vector<T>& v) {
 }
 
 PyGRGPtr fromArrays(uint32_t numSamples, uint32_t ploidy, py::handle downOffsets, py::handle downChildren,
-                    py::handle mutationNodes, py::handle mutationMiss, py::handle coalCounts, py::handle positions) {
+                    py::handle mutationNodes, py::handle mutationMiss, py::handle coalCounts, py::handle positions,
+                    py::handle refAlleles, py::handle altAlleles, py::handle populations, py::handle populationIds,
+                    py::handle individualIds) {
     auto g = std::make_shared<PyGRG>();
     g->h.numSamples = numSamples;
     g->h.ploidy = ploidy;
@@ -282,6 +292,23 @@ PyGRGPtr fromArrays(uint32_t numSamples, uint32_t ploidy, py::handle downOffsets
     if (!coalCounts.is_none()) g->h.coal = toVec<uint32_t>(coalCounts, "coal_counts");
     if (!positions.is_none()) g->h.mutPos = toVec<double>(positions, "positions");
     API_CHECK(!g->h.downOff.empty(), "down_offsets must have num_nodes + 1 entries");
+    if (!refAlleles.is_none() || !altAlleles.is_none()) {
+        const auto refs = refAlleles.cast<std::vector<std::string>>();
+        const auto alts = altAlleles.cast<std::vector<std::string>>();
+        API_CHECK(refs.size() == g->h.numMutations() && alts.size() == g->h.numMutations(),
+                  "ref_alleles and alt_alleles need one entry per mutation");
+        for (size_t i = 0; i < refs.size(); i++) {
+            API_CHECK(refs[i].size() <= 0xffff, "REF allele length exceeded maximum of 65535");
+            g->h.mutAlleles.push_back(refs[i] + alts[i]);
+            g->h.mutAltIndex.push_back((uint16_t)refs[i].size());
+        }
+    }
+    if (!populations.is_none()) g->h.populations = populations.cast<std::vector<std::string>>();
+    if (!populationIds.is_none()) g->h.popIds = toVec<uint16_t>(populationIds, "population_ids");
+    if (!individualIds.is_none()) {
+        g->h.individualIds = individualIds.cast<std::vector<std::string>>();
+        API_CHECK(g->h.individualIds.size() == numSamples / ploidy, "individual_ids needs one entry per individual");
+    }
     validateHostGrg(g->h);
     return g;
 }
@@ -295,6 +322,15 @@ PYBIND11_MODULE(_grgl_amd, m) {
         .value("DOWN", Direction::DOWN, "Traverse the graph \"down\" edges.")
         .value("UP", Direction::UP, "Traverse the graph via \"up\" edges.")
         .export_values();
+
+    py::class_<PyMutation>(m, "Mutation")
+        .def_readonly("position", &PyMutation::position)
+        .def_readonly("allele", &PyMutation::allele)
+        .def_readonly("ref_allele", &PyMutation::ref_allele)
+        .def_readonly("time", &PyMutation::time)
+        .def("__repr__", [](const PyMutation& x) {
+            return "Mutation(" + std::to_string(x.position) + ", " + x.ref_allele + ">" + x.allele + ")";
+        });
 
     py::class_<PyGRG, PyGRGPtr>(m, "GRG")
         .def_property_readonly("num_samples", [](PyGRG& g) { return g.h.numSamples; })
@@ -375,6 +411,30 @@ PYBIND11_MODULE(_grgl_amd, m) {
             API_CHECK(m < g.h.numMutations(), "Invalid MutationID: " << m);
             return g.h.mutPos.empty() ? (double)m : g.h.mutPos[m];
         })
+        .def("get_mutation_by_id",
+             [](PyGRG& g, uint32_t m) {
+                 API_CHECK(m < g.h.numMutations(), "Invalid MutationID: " << m);
+                 PyMutation r;
+                 r.position = g.h.mutPos.empty() ? (uint64_t)m : (uint64_t)g.h.mutPos[m];
+                 const std::string s = g.h.mutAlleles.empty() ? std::string("AG") : g.h.mutAlleles[m];
+                 const size_t alt = g.h.mutAltIndex.empty() ? 1 : std::min<size_t>(g.h.mutAltIndex[m], s.size());
+                 r.ref_allele = s.substr(0, alt);
+                 r.allele = s.substr(alt);
+                 r.time = g.h.mutTime.empty() ? 0x3FFFFFFFU : g.h.mutTime[m];
+                 return r;
+             },
+             py::arg("mut_id"))
+        .def_property_readonly("specified_bp_range",
+                               [](PyGRG& g) { return std::make_pair(g.h.rangeStart, g.h.rangeEnd); })
+        .def_property_readonly("num_populations", [](PyGRG& g) { return g.h.populations.size(); })
+        .def("get_populations", [](PyGRG& g) { return g.h.populations; })
+        .def_property_readonly("has_individual_ids", [](PyGRG& g) { return !g.h.individualIds.empty(); })
+        .def("get_individual_id",
+             [](PyGRG& g, uint32_t i) {
+                 API_CHECK(i < g.h.individualIds.size(), "Invalid individual index " << i);
+                 return g.h.individualIds[i];
+             },
+             py::arg("individual_index"))
         .def("arrays",
              [](PyGRG& g) {
                  py::dict d;
@@ -442,9 +502,35 @@ PYBIND11_MODULE(_grgl_amd, m) {
     m.def("dot_product", &dotProduct, py::arg("grg"), py::arg("input"), py::arg("direction"),
           "DEPRECATED, use matmul instead.  1-D form of matmul.");
 
+    m.def(
+        "load_immutable_grg",
+        [](const std::string& filename, bool loadUpEdges) {
+            (void)loadUpEdges; // up edges are derived on demand
+            auto g = std::make_shared<PyGRG>();
+            {
+                py::gil_scoped_release release;
+                g->h = readGrgFile(filename);
+            }
+            return g;
+        },
+        py::arg("filename"), py::arg("load_up_edges") = true,
+        "Load a .grg file (format 5.x, src/serialize.cpp:815-890 of the reference).");
+    m.def(
+        "save_grg",
+        [](PyGRGPtr& g, const std::string& filename, bool allowSimplify) {
+            (void)allowSimplify; // written as is: nodes and edges are not simplified
+            py::gil_scoped_release release;
+            writeGrgFile(g->h, filename);
+        },
+        py::arg("grg"), py::arg("filename"), py::arg("allow_simplify") = true,
+        "Write the graph as a .grg file, format 5.3, without simplification.");
+
     m.def("grg_from_arrays", &fromArrays, py::arg("num_samples"), py::arg("ploidy"), py::arg("down_offsets"),
           py::arg("down_children"), py::arg("mutation_nodes"), py::arg("mutation_miss_nodes") = py::none(),
           py::arg("coal_counts") = py::none(), py::arg("positions") = py::none(),
+          py::arg("ref_alleles") = py::none(), py::arg("alt_alleles") = py::none(),
+          py::arg("populations") = py::none(), py::arg("population_ids") = py::none(),
+          py::arg("individual_ids") = py::none(),
           "Build an immutable, ordered GRG from a down-edge CSR and the mutation->node map.");
     m.def(
         "grg_from_coalescent",

@@ -103,6 +103,8 @@ def _load_ref():
         L.ref_matmul.restype = ci
         L.ref_matmul.argtypes = [vp, ci, vp, sz, sz, ci, vp, sz, ci, ci, vp, ci, vp]
         L.ref_last_error.restype = ctypes.c_char_p
+        L.ref_encode_edges.restype = sz
+        L.ref_encode_edges.argtypes = [u32, vp, vp, vp, sz, vp]
         _ref = L
     return _ref
 
@@ -200,6 +202,16 @@ class OracleGraph:
         if rc != 0:
             raise OracleMisuse(L.ref_last_error().decode())
         return out
+
+    def ref_edge_section(self):
+        """(edge bytes, u64 node offsets) of the .grg edge section, encoded by the reference's
+        own CSR storage."""
+        L = _load_ref()
+        n = L.ref_encode_edges(self.num_nodes, _ptr(self.off), _ptr(self.child), None, 0, None)
+        buf = np.zeros(n, np.uint8)
+        idx = np.zeros(self.num_nodes + 1, np.uint64)
+        L.ref_encode_edges(self.num_nodes, _ptr(self.off), _ptr(self.child), _ptr(buf), n, _ptr(idx))
+        return buf, idx
 
     def __del__(self):
         if self._ref_handle is not None and _ref is not None:

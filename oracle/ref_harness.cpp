@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -105,4 +106,30 @@ int ref_matmul(void* handle, int dtype, const void* in, size_t inCols, size_t ro
 }
 
 uint32_t ref_num_edges(void* handle) { return (uint32_t)((RefGraph*)handle)->grg->numEdges(); }
+
+/*
+ * The edge section of a .grg file as the reference's own CSR storage produces it
+ * (EagerCSREdges32::setData + flushBuckets/flushIndexes, csr_storage.h:440-553,
+ * src/serialize.cpp:450,575): encoded child lists, then numNodes + 1 u64 byte offsets.
+ * Returns the number of edge bytes; offsets go to outIndex.  outBytes may be NULL to
+ * query the size.
+ */
+size_t ref_encode_edges(uint32_t numNodes, const uint64_t* downOff, const uint32_t* downChild, uint8_t* outBytes,
+                        size_t capacity, uint64_t* outIndex) {
+    EagerCSREdges32 csr(numNodes);
+    for (uint32_t n = 0; n < numNodes; n++) {
+        if (downOff[n + 1] > downOff[n]) {
+            std::vector<NodeID> kids(downChild + downOff[n], downChild + downOff[n + 1]);
+            csr.setData(n, kids);
+        }
+    }
+    csr.finalizeNodes();
+    std::ostringstream bucket, index;
+    csr.flushBuckets(bucket, /*clearData=*/false);
+    csr.flushIndexes(index, /*clearData=*/false);
+    const std::string b = bucket.str(), i = index.str();
+    if (outBytes != nullptr && b.size() <= capacity) std::memcpy(outBytes, b.data(), b.size());
+    if (outIndex != nullptr) std::memcpy(outIndex, i.data(), i.size());
+    return b.size();
+}
 }

@@ -1,0 +1,135 @@
+"""The .grg file format (reference format 5.x, src/serialize.cpp:518-669 and :697-890)."""
+
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import grgl_amd
+import oracle
+from helpers import bits_equal, make_input, random_grg
+
+needs_ref = pytest.mark.skipif(not oracle.ref_available(), reason="reference library oracle/_ref not built here")
+
+
+def _arrays_equal(a, b):
+    for k in ("down_offsets", "down_children", "mutation_nodes"):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ("mutation_miss_nodes", "coal_counts"):
+        if a[k] is None:
+            assert b[k] is None, k
+        else:
+            assert np.array_equal(a[k], b[k]), k
+
+
+def _rich_graph():
+    a = random_grg(3, num_samples=40, num_internal=120, heavy=(50,), num_mutations=200, with_miss=True,
+                   with_coal=True)
+    rng = np.random.default_rng(4)
+    refs = ["A", "CT", "G" * 9, ""] * 50
+    alts = ["G", "C", "A", "TTTTTTTTTTT"] * 50
+    g = grgl_amd.grg_from_arrays(a["num_samples"], a["ploidy"], a["down_offsets"], a["down_children"],
+                                 a["mutation_nodes"], mutation_miss_nodes=a["mutation_miss_nodes"],
+                                 coal_counts=a["coal_counts"],
+                                 positions=np.sort(rng.integers(0, 10**7, 200)).astype(np.float64),
+                                 ref_alleles=refs, alt_alleles=alts, populations=["EUR", "AFR"],
+                                 population_ids=rng.integers(0, 2, 40).astype(np.uint16),
+                                 individual_ids=[f"ind{i}" for i in range(20)])
+    return a, g, refs, alts
+
+
+def test_round_trip_everything(tmp_path):
+    a, g, refs, alts = _rich_graph()
+    fn = str(tmp_path / "rich.grg")
+    grgl_amd.save_grg(g, fn)
+    h = grgl_amd.load_immutable_grg(fn)
+    _arrays_equal(g.arrays(), h.arrays())
+    for m in (0, 1, 2, 3, 199):
+        mu, mv = g.get_mutation_by_id(m), h.get_mutation_by_id(m)
+        assert (mu.position, mu.ref_allele, mu.allele) == (mv.position, mv.ref_allele, mv.allele)
+        assert (mv.ref_allele, mv.allele) == (refs[m], alts[m])
+    assert h.get_populations() == ["EUR", "AFR"]
+    assert h.has_individual_ids and h.get_individual_id(7) == "ind7"
+    assert h.has_missing_data and h.has_individual_coals
+
+
+def test_loaded_graph_multiplies_like_the_original(tmp_path):
+    g = grgl_amd.grg_from_coalescent(500, 4000, 3, seed=8)
+    fn = str(tmp_path / "tree.grg")
+    grgl_amd.save_grg(g, fn)
+    h = grgl_amd.load_immutable_grg(fn)
+    a, b = oracle.OracleGraph(g.arrays()), oracle.OracleGraph(h.arrays())
+    x = make_input(np.random.default_rng(1), np.float64, (3, 500))
+    assert bits_equal(a.matmul(x, oracle.UP), b.matmul(x, oracle.UP))
+    assert g.get_mutation_by_id(10).position == h.get_mutation_by_id(10).position
+
+
+def _sections(fn):
+    data = open(fn, "rb").read()
+    hdr = struct.unpack("<QHHHHQQQQQQQ", data[:72])
+    nodes = hdr[7]
+    ebytes = struct.unpack("<Q", data[128:136])[0]
+    edges = np.frombuffer(data[136:136 + ebytes], np.uint8)
+    idx = np.frombuffer(data[136 + ebytes:136 + ebytes + 8 * (nodes + 1)], np.uint64)
+    return data, hdr, edges, idx
+
+
+@needs_ref
+def test_edge_section_bytes_match_reference_encoder(tmp_path):
+    """The varint CSR edge section equals what the reference's EagerCSREdges32 writes."""
+    for a in (grgl_amd.grg_from_coalescent(800, 6000, 4, seed=2).arrays(),
+              random_grg(6, num_samples=100, num_internal=600, heavy=(300, 5000), num_mutations=100)):
+        g = grgl_amd.grg_from_arrays(a["num_samples"], a["ploidy"], a["down_offsets"], a["down_children"],
+                                     a["mutation_nodes"])
+        fn = str(tmp_path / "e.grg")
+        grgl_amd.save_grg(g, fn)
+        _, hdr, edges, idx = _sections(fn)
+        ref_edges, ref_idx = oracle.OracleGraph(a).ref_edge_section()
+        assert np.array_equal(edges, ref_edges)
+        assert np.array_equal(idx, ref_idx)
+        assert hdr[0] == 0xE9366C64DDC8C5B0 and (hdr[1], hdr[2]) == (5, 3)
+
+
+def test_reads_version_5_1_small_index(tmp_path):
+    """Files up to version 5.1 store u32 node offsets (serialize.cpp:85, :783-790)."""
+    g = grgl_amd.grg_from_coalescent(300, 2000, 2, seed=4)
+    fn = str(tmp_path / "v53.grg")
+    grgl_amd.save_grg(g, fn)
+    data, hdr, edges, idx = _sections(fn)
+    nodes = hdr[7]
+    start = 136 + len(edges)
+    old = bytearray(data[:start]) + idx.astype(np.uint32).tobytes() + data[start + 8 * (nodes + 1):]
+    struct.pack_into("<H", old, 10, 1)  # versionMinor = 1
+    fn51 = str(tmp_path / "v51.grg")
+    open(fn51, "wb").write(bytes(old))
+    _arrays_equal(g.arrays(), grgl_amd.load_immutable_grg(fn51).arrays())
+
+
+def test_corrupt_files_raise(tmp_path):
+    g = grgl_amd.grg_from_coalescent(100, 500, 1, seed=1)
+    fn = str(tmp_path / "ok.grg")
+    grgl_amd.save_grg(g, fn)
+    data = open(fn, "rb").read()
+    bad = str(tmp_path / "bad.grg")
+    open(bad, "wb").write(b"\0" * 8 + data[8:])
+    with pytest.raises(RuntimeError, match="Invalid file header"):
+        grgl_amd.load_immutable_grg(bad)
+    open(bad, "wb").write(data[: len(data) // 2])
+    with pytest.raises(RuntimeError, match="Malformed"):
+        grgl_amd.load_immutable_grg(bad)
+    with pytest.raises(RuntimeError):
+        grgl_amd.load_immutable_grg(str(tmp_path / "missing.grg"))
+
+
+@pytest.mark.gpu
+def test_gpu_matmul_on_loaded_file(gpu, tmp_path):
+    g = grgl_amd.grg_from_coalescent(2000, 20000, 4, seed=12)
+    fn = str(tmp_path / "g.grg")
+    grgl_amd.save_grg(g, fn)
+    h = grgl_amd.load_immutable_grg(fn)
+    o = oracle.OracleGraph(g.arrays())
+    x = make_input(np.random.default_rng(3), np.float32, (4, 2000))
+    y = make_input(np.random.default_rng(4), np.int32, (2, h.num_mutations))
+    assert bits_equal(grgl_amd.matmul(h, x, grgl_amd.UP, order_preserving=True), o.matmul(x, oracle.UP))
+    assert bits_equal(grgl_amd.matmul(h, y, grgl_amd.DOWN), o.matmul(y, oracle.DOWN))
