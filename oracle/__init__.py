@@ -105,12 +105,25 @@ def _load_ref():
         L.ref_last_error.restype = ctypes.c_char_p
         L.ref_encode_edges.restype = sz
         L.ref_encode_edges.argtypes = [u32, vp, vp, vp, sz, vp]
+        L.ref_mutation_record.restype = ci
+        L.ref_mutation_record.argtypes = [ctypes.c_uint64, ctypes.c_char_p, ctypes.c_char_p, u32, vp,
+                                          ctypes.c_char_p, sz]
         _ref = L
     return _ref
 
 
 def _ptr(a):
     return None if a is None else a.ctypes.data
+
+
+def ref_mutation_record(position, ref, alt, time=0x3FFFFFFF):
+    """(24 record bytes, is_short, allele storage string or None) of the reference's
+    Mutation(position, alt, ref, time) as a .grg file stores it."""
+    L = _load_ref()
+    rec = np.zeros(24, np.uint8)
+    buf = ctypes.create_string_buffer(len(ref) + len(alt) + 2)
+    short = L.ref_mutation_record(position, ref.encode(), alt.encode(), time, rec.ctypes.data, buf, len(buf))
+    return rec, bool(short), (None if short else buf.value.decode())
 
 
 class OracleGraph:

@@ -38,10 +38,13 @@ class OracleStdOperator:
         return (X_ind - self.mu[None, :]) * self.inv_sigma[None, :]
 
 
-def randomized_pca(op, k=10, oversample=10, power_iters=5, seed=20261203):
+def randomized_pca(op, k=10, oversample=10, power_iters=5, seed=20261203, omega=None):
+    """`omega` (M x (k + oversample)) is the random block; numpy's generator draws one from
+    `seed` when it is not given (pass the device's block to run the same algorithm)."""
     n, m = op.shape
     ell = k + oversample
-    omega = np.random.default_rng(seed).standard_normal((m, ell))
+    if omega is None:
+        omega = np.random.default_rng(seed).standard_normal((m, ell))
     Q, _ = np.linalg.qr(op.matmat(omega))
     for _ in range(power_iters):
         Z, _ = np.linalg.qr(op.rmatmat(Q))

@@ -114,6 +114,23 @@ uint32_t ref_num_edges(void* handle) { return (uint32_t)((RefGraph*)handle)->grg
  * Returns the number of edge bytes; offsets go to outIndex.  outBytes may be NULL to
  * query the size.
  */
+/*
+ * The 24 bytes the reference writes for one mutation record of a .grg file
+ * (writeScalar<Mutation>, src/serialize.cpp:618; layout include/grgl/mutation.h:248-261):
+ * the raw object of Mutation(position, alt, ref, time).  Returns 1 when the alleles are
+ * stored inline (isShort), 0 when the first 8 bytes hold a heap pointer with bit 0 set and
+ * the allele storage string follows the records (returned in outStorage, capacity bytes).
+ */
+int ref_mutation_record(uint64_t position, const char* ref, const char* alt, uint32_t time, uint8_t* out24,
+                        char* outStorage, size_t capacity) {
+    Mutation mut((BpPosition)position, std::string(alt), std::string(ref), time);
+    std::memcpy(out24, (const void*)&mut, sizeof(Mutation));
+    if (mut.isShort()) return 1;
+    const std::string st = mut.alleleStorageString();
+    if (outStorage != nullptr && st.size() < capacity) std::memcpy(outStorage, st.c_str(), st.size() + 1);
+    return 0;
+}
+
 size_t ref_encode_edges(uint32_t numNodes, const uint64_t* downOff, const uint32_t* downChild, uint8_t* outBytes,
                         size_t capacity, uint64_t* outIndex) {
     EagerCSREdges32 csr(numNodes);

@@ -133,3 +133,44 @@ def test_gpu_matmul_on_loaded_file(gpu, tmp_path):
     y = make_input(np.random.default_rng(4), np.int32, (2, h.num_mutations))
     assert bits_equal(grgl_amd.matmul(h, x, grgl_amd.UP, order_preserving=True), o.matmul(x, oracle.UP))
     assert bits_equal(grgl_amd.matmul(h, y, grgl_amd.DOWN), o.matmul(y, oracle.DOWN))
+
+
+@needs_ref
+def test_mutation_records_match_reference_layout(tmp_path):
+    """24-byte mutation records and deferred allele strings equal the raw bytes of the
+    reference's own Mutation objects (src/serialize.cpp:608-624); for long alleles the
+    first 8 bytes hold the writer's heap pointer, so only its stolen bit 0 is compared."""
+    a = grgl_amd.grg_from_coalescent(100, 12, 1, seed=3).arrays()
+    M = len(a["mutation_nodes"])
+    refs = ["A", "CT", "GGGGGGGGG", "", "ACG", "T", "AAAA", "", "C", "G" * 300, "A", "TT"][:M]
+    alts = ["G", "C", "A", "TTTTTTTTTTT", "TTTT", "TTTTTTT", "CCC", "", "T", "A", "ACGTACG", "A"][:M]
+    pos = np.arange(M, dtype=np.float64) * 1000 + 123456789
+    g = grgl_amd.grg_from_arrays(a["num_samples"], a["ploidy"], a["down_offsets"], a["down_children"],
+                                 a["mutation_nodes"], positions=pos, ref_alleles=refs, alt_alleles=alts)
+    fn = str(tmp_path / "m.grg")
+    grgl_amd.save_grg(g, fn)
+    data, hdr, edges, idx = _sections(fn)
+    start = 136 + len(edges) + 8 * len(idx) + 4 * M
+    recs = np.frombuffer(data[start:start + 24 * M], np.uint8).reshape(M, 24)
+    tail = data[start + 24 * M:]
+    deferred = []
+    for m in range(M):
+        want, short, storage = oracle.ref_mutation_record(int(pos[m]), refs[m], alts[m])
+        if short:
+            assert np.array_equal(recs[m], want), m
+        else:
+            assert recs[m][0] & 1 and np.array_equal(recs[m][8:], want[8:]), m
+            deferred.append(storage)
+    # Deferred strings: LEB128 length + bytes, in mutation order (serialize.cpp:620-622).
+    pos_t = 0
+    for st in deferred:
+        n, shift = 0, 0
+        while True:
+            b = tail[pos_t]
+            pos_t += 1
+            n |= (b & 0x7F) << shift
+            shift += 7
+            if not b & 0x80:
+                break
+        assert tail[pos_t:pos_t + n].decode() == st
+        pos_t += n
