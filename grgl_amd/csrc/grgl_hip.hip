@@ -251,7 +251,8 @@ void seedRows(TypeClass tc, size_t es, const void* in, uint64_t inCols, uint32_t
     a.accumulate = accumulate ? 1 : 0;
     a.tkShift = tileShift(K);
     a.normalizeBool = tc == kX8 ? 1 : 0;
-    const bool vr = rowVectorOk(ld, kOffset, K, es);
+    // The kernels' fixed access counts assume a tile line of at least one row-side vector.
+    const bool vr = rowVectorOk(ld, kOffset, K, es) && (1ULL << a.tkShift) * es >= 16;
     const bool vm = colMap == nullptr && per == 1 && (inCols * es) % 16 == 0 && aligned16(in);
     const uint32_t TK = 1U << a.tkShift, TC = tileElems(es) / TK;
     dim3 grid((D + TC - 1) / TC, (K + TK - 1) / TK);
@@ -274,7 +275,7 @@ void readoutRows(TypeClass tc, size_t es, const char* rows, uint64_t ld, uint32_
     a.outCols = outCols;
     a.accumulate = accumulate ? 1 : 0;
     a.tkShift = tileShift(K);
-    const bool vr = rowVectorOk(ld, kOffset, K, es);
+    const bool vr = rowVectorOk(ld, kOffset, K, es) && (1ULL << a.tkShift) * es >= 16;
     const bool vm = (outCols * es) % 16 == 0 && aligned16(out);
     const uint32_t TK = 1U << a.tkShift, TC = tileElems(es) / TK;
     dim3 grid((uint32_t)((C + TC - 1) / TC), (K + TK - 1) / TK);

@@ -222,6 +222,13 @@ __global__ __launch_bounds__(kBlock) void pull_level_kernel(PullArgs a) {
 // accesses never cross the last column.
 // ---------------------------------------------------------------------------------------
 template <typename T> constexpr int kTileElems() { return sizeof(T) == 8 ? 2048 : 4096; }
+constexpr uint32_t ilog2c(uint32_t v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
+// Tile index math uses shifts: TK, TC, VR and VM are powers of two, so every "vectors per
+// tile line" count is one as well (a runtime integer division costs dozens of instructions).
+template <typename T, int V> __device__ __forceinline__ uint32_t vecShift(uint32_t lineShift) {
+    constexpr uint32_t vs = ilog2c((uint32_t)V);
+    return lineShift > vs ? lineShift - vs : 0;
+}
 
 struct SeedArgs {
     const void* in;       // K x inCols, row-major
@@ -248,42 +255,55 @@ __global__ __launch_bounds__(kBlock) void seed_rows_kernel(SeedArgs a) {
     const uint64_t d0 = (uint64_t)blockIdx.x * TC;
     const uint32_t k0 = blockIdx.y * TK;
     const T* in = (const T*)a.in;
-    // Load: k-major, VM columns per access.
-    const uint32_t vecPerK = TC / VM;
-    for (uint32_t e = threadIdx.x; e < TK * vecPerK; e += kBlock) {
-        const uint32_t k = e / vecPerK, dv = (e % vecPerK) * VM;
+    const uint32_t tcShift = ilog2c((uint32_t)TE) - a.tkShift;
+    // Load: k-major, VM columns per access; TE / VM accesses per tile, all issued before
+    // the first LDS store so each thread keeps several global loads in flight.
+    const uint32_t sK = vecShift<T, VM>(tcShift), vecPerK = 1U << sK;
+    constexpr int NL = TE / VM / kBlock;
+    T v[NL][VM];
+#pragma unroll
+    for (int it = 0; it < NL; it++) {
+        const uint32_t e = threadIdx.x + it * kBlock;
+        const uint32_t k = e >> sK, dv = (e & (vecPerK - 1)) * VM;
         const uint32_t gk = k0 + k;
         const uint64_t gd = d0 + dv;
-        T v[VM];
 #pragma unroll
-        for (int i = 0; i < VM; i++) v[i] = (T)0;
+        for (int i = 0; i < VM; i++) v[it][i] = (T)0;
         if (gk < a.K) {
             if constexpr (VM > 1) {
                 if (gd + VM <= a.D) {
                     typename RawVec<16>::type raw = *reinterpret_cast<const typename RawVec<16>::type*>(
                         in + (uint64_t)gk * a.inCols + gd);
-                    __builtin_memcpy(v, &raw, 16);
+                    __builtin_memcpy(v[it], &raw, 16);
                 } else {
                     for (int i = 0; i < VM; i++)
-                        if (gd + i < a.D) v[i] = in[(uint64_t)gk * a.inCols + gd + i];
+                        if (gd + i < a.D) v[it][i] = in[(uint64_t)gk * a.inCols + gd + i];
                 }
             } else if (gd < a.D) {
                 const uint64_t c = a.colMap ? (uint64_t)a.colMap[gd] : gd / a.per;
-                v[0] = in[(uint64_t)gk * a.inCols + c];
+                v[it][0] = in[(uint64_t)gk * a.inCols + c];
             }
         }
+    }
+#pragma unroll
+    for (int it = 0; it < NL; it++) {
+        const uint32_t e = threadIdx.x + it * kBlock;
+        const uint32_t k = e >> sK, dv = (e & (vecPerK - 1)) * VM;
 #pragma unroll
         for (int i = 0; i < VM; i++) {
-            T x = v[i];
+            T x = v[it][i];
             if (a.normalizeBool) x = (T)(x != (T)0);
             tile[k * (TC + 1) + dv + i] = x;
         }
     }
     __syncthreads();
     // Store: row-major over destination rows, VR k-values per access.
-    const uint32_t vecPerD = (TK + VR - 1) / VR;
-    for (uint32_t e = threadIdx.x; e < TC * vecPerD; e += kBlock) {
-        const uint32_t d = e / vecPerD, k = (e % vecPerD) * VR;
+    // TE / VR accesses per tile (the host takes VR > 1 only when TK >= VR).
+    const uint32_t sD = vecShift<T, VR>(a.tkShift), vecPerD = 1U << sD;
+#pragma unroll
+    for (int it = 0; it < TE / VR / kBlock; it++) {
+        const uint32_t e = threadIdx.x + it * kBlock;
+        const uint32_t d = e >> sD, k = (e & (vecPerD - 1)) * VR;
         const uint64_t gd = d0 + d;
         const uint32_t gk = k0 + k;
         if (gd >= a.D || gk >= a.K) continue;
@@ -323,74 +343,89 @@ struct ReadoutArgs {
     uint32_t tkShift;
 };
 
+// The block's slice of the column map is staged in LDS once per summed row (`per`), so
+// the load phase reads one row index per column and the sum phase checks validity in LDS.
+// Unmapped columns contribute nothing: the written value is the initial value (0, or the
+// existing output when accumulating), bit for bit.
 template <typename T, bool XOR, int VR, int VM>
 __global__ __launch_bounds__(kBlock) void readout_kernel(ReadoutArgs a) {
     constexpr int TE = kTileElems<T>();
     constexpr int OWN = TE / kBlock / VM; // matrix-side vectors owned per thread
     __shared__ T tile[TE + 64];
+    __shared__ uint32_t srcRow[TE];
     const uint32_t TK = 1U << a.tkShift;
     const uint32_t TC = (uint32_t)TE >> a.tkShift;
     const uint64_t c0 = (uint64_t)blockIdx.x * TC;
     const uint32_t k0 = blockIdx.y * TK;
     T* out = (T*)a.out;
-    const uint32_t vecPerK = TC / VM;
+    const uint32_t tcShift = ilog2c((uint32_t)TE) - a.tkShift;
+    const uint32_t sK = vecShift<T, VM>(tcShift), vecPerK = 1U << sK;
+    const bool fullCols = c0 + TC <= a.C;
     T acc[OWN][VM];
-    bool any[OWN][VM];
 #pragma unroll
     for (int o = 0; o < OWN; o++) {
         const uint32_t e = threadIdx.x + o * kBlock;
-        const uint32_t k = e / vecPerK, c = (e % vecPerK) * VM;
+        const uint32_t k = e >> sK, c = (e & (vecPerK - 1)) * VM;
 #pragma unroll
-        for (int i = 0; i < VM; i++) {
-            acc[o][i] = (T)0;
-            any[o][i] = false;
-        }
+        for (int i = 0; i < VM; i++) acc[o][i] = (T)0;
         if (a.accumulate && k < TK && k0 + k < a.K) {
             const T* src = out + (uint64_t)(k0 + k) * a.outCols + c0 + c;
+            if constexpr (VM > 1) {
+                if (fullCols) {
+                    typename RawVec<16>::type raw = *reinterpret_cast<const typename RawVec<16>::type*>(src);
+                    __builtin_memcpy(acc[o], &raw, 16);
+                    continue;
+                }
+            }
             for (int i = 0; i < VM; i++)
                 if (c0 + c + i < a.C) acc[o][i] = src[i];
         }
     }
-    const uint32_t vecPerC = (TK + VR - 1) / VR;
+    const uint32_t sC = vecShift<T, VR>(a.tkShift), vecPerC = 1U << sC;
     for (uint32_t j = 0; j < a.per; j++) {
-        // Load: one row slice per column, VR k-values per access.
-        for (uint32_t e = threadIdx.x; e < TC * vecPerC; e += kBlock) {
-            const uint32_t c = e / vecPerC, k = (e % vecPerC) * VR;
-            const uint64_t gc = c0 + c;
-            const uint32_t gk = k0 + k;
-            T v[VR];
+        for (uint32_t c = threadIdx.x; c < TC; c += kBlock)
+            srcRow[c] = (c0 + c < a.C) ? a.map[(c0 + c) * a.per + j] : kInvalidRow;
+        __syncthreads();
+        // Load: one row slice per column, VR k-values per access; TE / VR accesses per tile
+        // (the host takes VR > 1 only when TK >= VR), issued before the first LDS store.
+        constexpr int NL = TE / VR / kBlock;
+        T v[NL][VR];
 #pragma unroll
-            for (int i = 0; i < VR; i++) v[i] = (T)0;
-            if (gc < a.C && gk < a.K) {
-                const uint32_t r = a.map[gc * a.per + j];
-                if (r != kInvalidRow) {
-                    const T* src = (const T*)(a.rows + (uint64_t)r * a.ld) + a.kOffset + gk;
-                    if constexpr (VR > 1) {
-                        typename RawVec<16>::type raw = *reinterpret_cast<const typename RawVec<16>::type*>(src);
-                        __builtin_memcpy(v, &raw, 16);
-                    } else {
-                        v[0] = *src;
-                    }
+        for (int it = 0; it < NL; it++) {
+            const uint32_t e = threadIdx.x + it * kBlock;
+            const uint32_t c = e >> sC, k = (e & (vecPerC - 1)) * VR;
+            const uint32_t gk = k0 + k;
+            const uint32_t r = srcRow[c];
+#pragma unroll
+            for (int i = 0; i < VR; i++) v[it][i] = (T)0;
+            if (r != kInvalidRow && gk < a.K) {
+                const T* src = (const T*)(a.rows + (uint64_t)r * a.ld) + a.kOffset + gk;
+                if constexpr (VR > 1) {
+                    typename RawVec<16>::type raw = *reinterpret_cast<const typename RawVec<16>::type*>(src);
+                    __builtin_memcpy(v[it], &raw, 16);
+                } else {
+                    v[it][0] = *src;
                 }
             }
+        }
+#pragma unroll
+        for (int it = 0; it < NL; it++) {
+            const uint32_t e = threadIdx.x + it * kBlock;
+            const uint32_t c = e >> sC, k = (e & (vecPerC - 1)) * VR;
 #pragma unroll
             for (int i = 0; i < VR; i++)
-                if (k + i < TK) tile[(k + i) * (TC + 1) + c] = v[i];
+                if (k + i < TK) tile[(k + i) * (TC + 1) + c] = v[it][i];
         }
         __syncthreads();
 #pragma unroll
         for (int o = 0; o < OWN; o++) {
             const uint32_t e = threadIdx.x + o * kBlock;
-            const uint32_t k = e / vecPerK, c = (e % vecPerK) * VM;
-            if (k < TK && k0 + k < a.K) {
+            const uint32_t k = e >> sK, c = (e & (vecPerK - 1)) * VM;
+            if (k < TK) {
 #pragma unroll
-                for (int i = 0; i < VM; i++) {
-                    const uint64_t gc = c0 + c + i;
-                    if (gc < a.C && a.map[gc * a.per + j] != kInvalidRow) {
+                for (int i = 0; i < VM; i++)
+                    if (srcRow[c + i] != kInvalidRow)
                         acc[o][i] = combine<T, XOR>(acc[o][i], tile[k * (TC + 1) + c + i]);
-                        any[o][i] = true;
-                    }
-                }
             }
         }
         __syncthreads();
@@ -398,14 +433,11 @@ __global__ __launch_bounds__(kBlock) void readout_kernel(ReadoutArgs a) {
 #pragma unroll
     for (int o = 0; o < OWN; o++) {
         const uint32_t e = threadIdx.x + o * kBlock;
-        const uint32_t k = e / vecPerK, c = (e % vecPerK) * VM;
+        const uint32_t k = e >> sK, c = (e & (vecPerK - 1)) * VM;
         if (k >= TK || k0 + k >= a.K) continue;
         T* dst = out + (uint64_t)(k0 + k) * a.outCols + c0 + c;
-        bool full = c0 + c + VM <= a.C;
-#pragma unroll
-        for (int i = 0; i < VM; i++) full = full && (any[o][i] || !a.accumulate);
         if constexpr (VM > 1) {
-            if (full) {
+            if (fullCols) {
                 typename RawVec<16>::type raw;
                 __builtin_memcpy(&raw, acc[o], 16);
                 *reinterpret_cast<typename RawVec<16>::type*>(dst) = raw;
@@ -413,7 +445,7 @@ __global__ __launch_bounds__(kBlock) void readout_kernel(ReadoutArgs a) {
             }
         }
         for (int i = 0; i < VM; i++)
-            if (c0 + c + i < a.C && (any[o][i] || !a.accumulate)) dst[i] = acc[o][i];
+            if (c0 + c + i < a.C) dst[i] = acc[o][i];
     }
 }
 
