@@ -67,7 +67,8 @@ struct DeviceDirPlan {
     uint32_t* missRow = nullptr;
     uint32_t* coalByRow = nullptr;
     uint32_t heavyFanin = 64;
-    std::vector<uint32_t> heavyCount; // per level, for heavyFanin
+    std::vector<uint32_t> heavyCount;   // per level, for heavyFanin
+    std::vector<uint64_t> levelEntries; // per level, gather entries
     void computeHeavy();
 };
 

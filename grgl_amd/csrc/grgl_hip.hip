@@ -89,8 +89,10 @@ RowGeometry rowGeometry(uint32_t K, size_t elemSize) {
 
 void DeviceDirPlan::computeHeavy() {
     heavyCount.assign(numLevels, 0);
+    levelEntries.assign(numLevels, 0);
     for (uint32_t l = firstPulledLevel; l < numLevels; l++) {
         uint32_t lo = levelStart[l], hi = levelStart[l + 1];
+        for (uint32_t r = lo; r < hi; r++) levelEntries[l] += fanIn[r];
         const uint32_t begin = lo;
         while (lo < hi) {
             const uint32_t mid = lo + (hi - lo) / 2;
@@ -152,6 +154,38 @@ template <typename T, bool XOR> void launchPullT(uint32_t vb, uint32_t tw, dim3 
         }
     }
 #undef GRGL_PULL
+}
+
+template <typename T, bool XOR> void launchTailT(uint32_t vb, uint32_t tw, const TailArgs& a, hipStream_t s) {
+#define GRGL_TAIL(VB, TW) hipLaunchKernelGGL((pull_tail_kernel<T, VB, TW, XOR>), dim3(1), dim3(kBlock), 0, s, a)
+    if (vb == 4) {
+        if constexpr (sizeof(T) <= 4) GRGL_TAIL(4, 1);
+    } else if (vb == 8) {
+        if constexpr (sizeof(T) <= 8) GRGL_TAIL(8, 1);
+    } else {
+        switch (tw) {
+        case 1: GRGL_TAIL(16, 1); break;
+        case 2: GRGL_TAIL(16, 2); break;
+        case 4: GRGL_TAIL(16, 4); break;
+        case 8: GRGL_TAIL(16, 8); break;
+        case 16: GRGL_TAIL(16, 16); break;
+        case 32: GRGL_TAIL(16, 32); break;
+        default: GRGL_TAIL(16, 64); break;
+        }
+    }
+#undef GRGL_TAIL
+}
+
+void launchTail(TypeClass tc, uint32_t vb, uint32_t tw, const TailArgs& a, hipStream_t s) {
+    switch (tc) {
+    case kF64: launchTailT<double, false>(vb, tw, a, s); break;
+    case kF32: launchTailT<float, false>(vb, tw, a, s); break;
+    case kW64: launchTailT<uint64_t, false>(vb, tw, a, s); break;
+    case kW32: launchTailT<uint32_t, false>(vb, tw, a, s); break;
+    case kW16: launchTailT<uint16_t, false>(vb, tw, a, s); break;
+    case kW8: launchTailT<uint8_t, false>(vb, tw, a, s); break;
+    case kX8: launchTailT<uint8_t, true>(vb, tw, a, s); break;
+    }
 }
 
 void launchPull(TypeClass tc, uint32_t vb, uint32_t tw, dim3 grid, const PullArgs& a, hipStream_t s) {
@@ -303,11 +337,30 @@ Tuning chooseTuning(grgl_graph* g, grgl_direction dir, TypeClass tc, uint32_t K,
 }
 
 // Launch every pulled level of one direction.
+// First level of the trailing run that one block handles (numLevels: none).  A level
+// qualifies when its light rows take at most two passes of the block's teams, it has at
+// most two heavy rows, and its rows and gathered rows come to at most 64 KB per column
+// slice set; the run needs at least two levels to save a launch.
+uint32_t tailStart(const DeviceDirPlan& plan, const RowGeometry& geo, uint32_t tw, bool orderPreserving) {
+    const uint32_t teams = kBlock / tw;
+    uint32_t l = plan.numLevels;
+    while (l > plan.firstPulledLevel && plan.numLevels - l < (uint32_t)kMaxTailLevels) {
+        const uint32_t k = l - 1;
+        const uint64_t nodes = plan.levelStart[k + 1] - plan.levelStart[k];
+        const uint64_t heavy = orderPreserving ? 0 : plan.heavyCount[k];
+        const uint64_t bytes = (plan.levelEntries[k] + nodes) * (uint64_t)geo.rowBytes;
+        if (nodes - heavy > 2ULL * teams || heavy > 2 || bytes > (64ULL << 10)) break;
+        l = k;
+    }
+    return plan.numLevels - l >= 2 ? l : plan.numLevels;
+}
+
 void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const RowGeometry& geo, uint32_t tw,
                    bool readSelf, bool orderPreserving, uint32_t K, size_t elemSize, char* rows, hipStream_t stream) {
     const uint32_t tiles = (geo.lanes + tw - 1) / tw;
     const uint32_t teamsPerBlock = kBlock / tw;
-    for (uint32_t l = plan.firstPulledLevel; l < plan.numLevels; l++) {
+    const uint32_t tailFrom = g->timing ? plan.numLevels : tailStart(plan, geo, tw, orderPreserving);
+    for (uint32_t l = plan.firstPulledLevel; l < tailFrom; l++) {
         const uint32_t begin = plan.levelStart[l], end = plan.levelStart[l + 1];
         if (end == begin) continue;
         const uint32_t heavy = orderPreserving ? 0 : plan.heavyCount[l];
@@ -341,6 +394,22 @@ void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const
             g->timingEvents.push_back({ev0, ev1});
             g->timingBytes.push_back(bytes);
         }
+    }
+    if (tailFrom < plan.numLevels) {
+        TailArgs t;
+        t.rows = rows;
+        t.ld = geo.ld;
+        t.off = plan.off;
+        t.src = plan.src;
+        t.rowBytes = geo.rowBytes;
+        t.readSelf = readSelf ? 1 : 0;
+        t.tiles = tiles;
+        t.numLevels = plan.numLevels - tailFrom;
+        for (uint32_t l = tailFrom; l <= plan.numLevels; l++) t.levelStart[l - tailFrom] = plan.levelStart[l];
+        for (uint32_t l = tailFrom; l < plan.numLevels; l++)
+            t.heavyCount[l - tailFrom] = orderPreserving ? 0 : plan.heavyCount[l];
+        launchTail(tc, geo.vb, tw, t, stream);
+        GRGL_HIP_CHECK(hipGetLastError());
     }
 }
 

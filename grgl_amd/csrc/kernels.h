@@ -105,29 +105,37 @@ struct PullArgs {
     uint32_t readSelf;    // 1: start from the row's current value (node initialisation)
 };
 
-// Light rows: one team per row, list summed front to back, four loads in flight.
-template <typename T, int VB, int TW, bool XOR>
-__device__ __forceinline__ void pull_light(const PullArgs& a, uint32_t row, uint64_t colByte, bool active) {
+// Light rows: one team per row, list summed front to back with UNR row loads in flight.
+// `idx(i)` returns gather entry i (from global memory or from an LDS copy).
+template <typename T, int VB, int TW, bool XOR, int UNR, typename Idx>
+__device__ __forceinline__ void pull_light(const PullArgs& a, uint32_t row, uint64_t colByte, bool active,
+                                           Idx idx) {
     using P = Pack<T, VB>;
     const uint32_t b = a.off[row], e = a.off[row + 1];
     P acc = zero_pack<T, VB>();
     if (!active) return;
     if (a.readSelf) acc = load_pack<T, VB>(a.rows + (uint64_t)row * a.ld + colByte);
+    // Batches of UNR entries: every load of a batch is issued before the first add, and
+    // the adds run in list order.  The last, partial batch loads clamped (in-list) entries
+    // unconditionally and adds only the real ones, so it is one round trip as well.
     uint32_t i = b;
-    for (; i + 4 <= e; i += 4) {
-        const uint32_t s0 = a.src[i], s1 = a.src[i + 1], s2 = a.src[i + 2], s3 = a.src[i + 3];
-        const P x0 = load_pack<T, VB>(a.rows + (uint64_t)s0 * a.ld + colByte);
-        const P x1 = load_pack<T, VB>(a.rows + (uint64_t)s1 * a.ld + colByte);
-        const P x2 = load_pack<T, VB>(a.rows + (uint64_t)s2 * a.ld + colByte);
-        const P x3 = load_pack<T, VB>(a.rows + (uint64_t)s3 * a.ld + colByte);
-        acc_pack<T, VB, XOR>(acc, x0);
-        acc_pack<T, VB, XOR>(acc, x1);
-        acc_pack<T, VB, XOR>(acc, x2);
-        acc_pack<T, VB, XOR>(acc, x3);
+    for (; i + UNR <= e; i += UNR) {
+        P x[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) x[u] = load_pack<T, VB>(a.rows + (uint64_t)idx(i + u) * a.ld + colByte);
+#pragma unroll
+        for (int u = 0; u < UNR; u++) acc_pack<T, VB, XOR>(acc, x[u]);
     }
-    for (; i < e; i++) {
-        const P x = load_pack<T, VB>(a.rows + (uint64_t)a.src[i] * a.ld + colByte);
-        acc_pack<T, VB, XOR>(acc, x);
+    if (i < e) {
+        P x[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const uint32_t j = i + u < e ? i + u : e - 1;
+            x[u] = load_pack<T, VB>(a.rows + (uint64_t)idx(j) * a.ld + colByte);
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; u++)
+            if (i + u < e) acc_pack<T, VB, XOR>(acc, x[u]);
     }
     store_pack<T, VB>(a.rows + (uint64_t)row * a.ld + colByte, acc);
 }
@@ -151,20 +159,39 @@ __device__ __forceinline__ void pull_heavy(const PullArgs& a, uint32_t row, uint
         // Entry b + i belongs to partial (i mod 64); team t owns partials t + q*A.
         // Walking i = t, t+A, t+2A, ... visits partial q = (step mod PPT), each partial's
         // entries in increasing order.
+        // Each iteration covers STEPS steps (STEPS * PPT entries per team): all its loads
+        // are issued before the adds, which run step by step, so each partial's order is
+        // unchanged.  The last, partial iteration loads clamped (in-list) entries and adds
+        // only the real ones.
+        constexpr int STEPS = PPT >= 4 ? 1 : 4 / PPT;
+        constexpr uint32_t STRIDE = (uint32_t)(STEPS * PPT) * A;
         uint32_t i = b + team;
-        for (; i + (uint32_t)(PPT - 1) * A < e; i += (uint32_t)PPT * A) {
-            P x[PPT];
+        for (; i + STRIDE - A < e; i += STRIDE) {
+            P x[STEPS][PPT];
 #pragma unroll
-            for (int q = 0; q < PPT; q++) x[q] = load_pack<T, VB>(a.rows + (uint64_t)a.src[i + q * A] * a.ld + colByte);
+            for (int st = 0; st < STEPS; st++)
 #pragma unroll
-            for (int q = 0; q < PPT; q++) acc_pack<T, VB, XOR>(part[q], x[q]);
+                for (int q = 0; q < PPT; q++)
+                    x[st][q] = load_pack<T, VB>(a.rows + (uint64_t)a.src[i + (st * PPT + q) * A] * a.ld + colByte);
+#pragma unroll
+            for (int st = 0; st < STEPS; st++)
+#pragma unroll
+                for (int q = 0; q < PPT; q++) acc_pack<T, VB, XOR>(part[q], x[st][q]);
         }
+        if (i < e) {
+            P x[STEPS][PPT];
 #pragma unroll
-        for (int q = 0; q < PPT; q++) {
-            if (i + (uint32_t)q * A < e) {
-                const P x = load_pack<T, VB>(a.rows + (uint64_t)a.src[i + q * A] * a.ld + colByte);
-                acc_pack<T, VB, XOR>(part[q], x);
-            }
+            for (int st = 0; st < STEPS; st++)
+#pragma unroll
+                for (int q = 0; q < PPT; q++) {
+                    const uint32_t j = i + (uint32_t)(st * PPT + q) * A;
+                    x[st][q] = load_pack<T, VB>(a.rows + (uint64_t)a.src[j < e ? j : e - 1] * a.ld + colByte);
+                }
+#pragma unroll
+            for (int st = 0; st < STEPS; st++)
+#pragma unroll
+                for (int q = 0; q < PPT; q++)
+                    if (i + (uint32_t)(st * PPT + q) * A < e) acc_pack<T, VB, XOR>(part[q], x[st][q]);
         }
         // In-register tree steps: strides 32 .. A pair partials held by the same team
         // (partial t + q*A with partial t + (q + s/A)*A).
@@ -197,17 +224,95 @@ __device__ __forceinline__ void pull_heavy(const PullArgs& a, uint32_t row, uint
     }
 }
 
+// Narrow teams (TW <= 4: many rows per block, short row slices) are latency bound: the
+// block's rows are consecutive, so their gather lists form one contiguous range of `src`,
+// which is copied to LDS with coalesced loads (when it fits) before the teams walk their
+// lists with 8 (16 for single-lane teams) row loads in flight.  The summation order is the
+// same in every variant.
+constexpr uint32_t kStageEntries = 4096;
+
+struct GlobalIdx {
+    const uint32_t* src;
+    __device__ uint32_t operator()(uint32_t i) const { return src[i]; }
+};
+struct StagedIdx {
+    const uint32_t* lds;
+    uint32_t base;
+    __device__ uint32_t operator()(uint32_t i) const { return lds[i - base]; }
+};
+
 template <typename T, int VB, int TW, bool XOR>
 __global__ __launch_bounds__(kBlock) void pull_level_kernel(PullArgs a) {
     const uint32_t lane = threadIdx.x % TW;
     const uint32_t team = threadIdx.x / TW;
     const uint64_t colByte = ((uint64_t)blockIdx.y * TW + lane) * VB;
     const bool active = colByte < a.rowBytes;
+    constexpr uint32_t NT = kBlock / TW;
     if (blockIdx.x < a.heavyCount) {
         pull_heavy<T, VB, TW, XOR>(a, a.levelBegin + blockIdx.x, colByte, active, team, lane);
+    } else if constexpr (TW <= 4) {
+        constexpr int UNR = TW == 1 ? 16 : 8;
+        __shared__ uint32_t staged[kStageEntries];
+        const uint32_t r0 = a.levelBegin + a.heavyCount + (blockIdx.x - a.heavyCount) * NT;
+        const uint32_t r1 = r0 + NT < a.levelEnd ? r0 + NT : a.levelEnd;
+        const uint32_t e0 = a.off[r0], n = a.off[r1] - e0;
+        const uint32_t row = r0 + team;
+        if (n <= kStageEntries) { // block-uniform
+            for (uint32_t t = threadIdx.x; t < n; t += kBlock) staged[t] = a.src[e0 + t];
+            __syncthreads();
+            if (row < r1) pull_light<T, VB, TW, XOR, UNR>(a, row, colByte, active, StagedIdx{staged, e0});
+        } else if (row < r1) {
+            pull_light<T, VB, TW, XOR, UNR>(a, row, colByte, active, GlobalIdx{a.src});
+        }
     } else {
-        const uint32_t row = a.levelBegin + a.heavyCount + (blockIdx.x - a.heavyCount) * (kBlock / TW) + team;
-        if (row < a.levelEnd) pull_light<T, VB, TW, XOR>(a, row, colByte, active);
+        const uint32_t row = a.levelBegin + a.heavyCount + (blockIdx.x - a.heavyCount) * NT + team;
+        if (row < a.levelEnd) pull_light<T, VB, TW, XOR, 4>(a, row, colByte, active, GlobalIdx{a.src});
+    }
+}
+
+// The trailing levels of a traversal that are too small to fill the chip (at most two
+// passes of teams, a few KB of rows) run in ONE block, level after level, with a block
+// barrier between levels instead of a kernel boundary.  Every row goes through the same
+// pull_heavy / pull_light code as in pull_level_kernel, so results are bit-identical.
+constexpr int kMaxTailLevels = 64;
+struct TailArgs {
+    char* rows;
+    uint64_t ld;
+    const uint32_t* off;
+    const uint32_t* src;
+    uint32_t rowBytes;
+    uint32_t readSelf;
+    uint32_t tiles;       // column slices of TW * VB bytes
+    uint32_t numLevels;
+    uint32_t levelStart[kMaxTailLevels + 1];
+    uint32_t heavyCount[kMaxTailLevels];
+};
+
+template <typename T, int VB, int TW, bool XOR>
+__global__ __launch_bounds__(kBlock) void pull_tail_kernel(TailArgs t) {
+    const uint32_t lane = threadIdx.x % TW;
+    const uint32_t team = threadIdx.x / TW;
+    constexpr uint32_t NT = kBlock / TW;
+    for (uint32_t l = 0; l < t.numLevels; l++) {
+        PullArgs a;
+        a.rows = t.rows;
+        a.ld = t.ld;
+        a.off = t.off;
+        a.src = t.src;
+        a.levelBegin = t.levelStart[l];
+        a.heavyCount = t.heavyCount[l];
+        a.levelEnd = t.levelStart[l + 1];
+        a.rowBytes = t.rowBytes;
+        a.readSelf = t.readSelf;
+        for (uint32_t y = 0; y < t.tiles; y++) {
+            const uint64_t colByte = ((uint64_t)y * TW + lane) * VB;
+            const bool active = colByte < a.rowBytes;
+            for (uint32_t h = 0; h < a.heavyCount; h++) // block-uniform; pull_heavy synchronises
+                pull_heavy<T, VB, TW, XOR>(a, a.levelBegin + h, colByte, active, team, lane);
+            for (uint32_t row = a.levelBegin + a.heavyCount + team; row < a.levelEnd; row += NT)
+                pull_light<T, VB, TW, XOR, 4>(a, row, colByte, active, GlobalIdx{a.src});
+        }
+        __syncthreads(); // the next level reads rows written by other waves of this block
     }
 }
 
