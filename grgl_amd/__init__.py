@@ -24,6 +24,9 @@ try:
         GRG,
         INVALID_NODE,
         NO_UP_EDGES,
+        MutableGRG,
+        Mutation,
+        MutationMappingStats,
         TraversalDirection,
         device_count,
         dot_product,
@@ -31,6 +34,7 @@ try:
         grg_from_arrays,
         grg_from_coalescent,
         load_immutable_grg,
+        map_mutations,
         matmul,
         save_grg,
         set_device,
@@ -48,6 +52,9 @@ __all__ = [
     "GRG",
     "INVALID_NODE",
     "NO_UP_EDGES",
+    "MutableGRG",
+    "Mutation",
+    "MutationMappingStats",
     "TraversalDirection",
     "UP",
     "DOWN",
@@ -57,6 +64,7 @@ __all__ = [
     "grg_from_arrays",
     "grg_from_coalescent",
     "load_immutable_grg",
+    "map_mutations",
     "matmul",
     "save_grg",
     "set_device",

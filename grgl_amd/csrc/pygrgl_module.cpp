@@ -18,6 +18,7 @@
 #include <stdexcept>
 
 #include "grgl_hip.h"
+#include "construct.h"
 #include "host_graph.h"
 #include "levelize.h"
 #include "stacking.h"
@@ -50,17 +51,26 @@ enum class Direction : int { DOWN = 1, UP = 2 };
 struct PyGRG {
     HostGrg h;
     std::mutex mu;
+    std::mutex syncMu;
     std::map<int, grgl_graph*> dev;
     bool upBuilt = false;
     std::vector<uint64_t> upOff;
     std::vector<uint32_t> upPar;
     std::vector<std::pair<uint32_t, uint32_t>> nodeMut; // (node, mutId) sorted, invalid first
 
-    ~PyGRG() {
+    virtual ~PyGRG() {
         for (auto& kv : dev) grgl_hip_graph_destroy(kv.second);
     }
 
+    // Brings `h` (and the derived caches) up to date; mutable graphs override it.
+    virtual void sync() {}
+    HostGrg& H() {
+        sync();
+        return h;
+    }
+
     grgl_graph* deviceGraph() {
+        sync();
         std::lock_guard<std::mutex> lk(mu);
         auto it = dev.find(g_device);
         if (it != dev.end()) return it->second;
@@ -81,6 +91,7 @@ struct PyGRG {
     }
 
     void buildUp() {
+        sync();
         std::lock_guard<std::mutex> lk(mu);
         if (upBuilt) return;
         const uint32_t V = h.numNodes();
@@ -102,6 +113,27 @@ struct PyGRG {
     }
 };
 using PyGRGPtr = std::shared_ptr<PyGRG>;
+
+// MutableGRG (src/python/_grgl.cpp:759-925 of the reference): the growing graph lives in a
+// MutableGraph; every read through the GRG interface first refreezes it into `h` when it
+// changed, dropping device copies of the old version.
+struct PyMutableGRG : PyGRG {
+    MutableGraph mg;
+    uint64_t syncedVersion = ~0ULL;
+    PyMutableGRG(uint32_t numSamples, uint32_t ploidy, bool phased) : mg(numSamples, ploidy, phased) {}
+    void sync() override {
+        std::lock_guard<std::mutex> sl(syncMu);
+        if (mg.version() == syncedVersion) return;
+        HostGrg fresh = mg.freeze();
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto& kv : dev) grgl_hip_graph_destroy(kv.second);
+        dev.clear();
+        upBuilt = false;
+        h = std::move(fresh);
+        syncedVersion = mg.version();
+    }
+};
+using PyMutableGRGPtr = std::shared_ptr<PyMutableGRG>;
 
 // A mutation's public fields (include/grgl/mutation.h): position, ALT, REF, time.
 struct PyMutation {
@@ -178,7 +210,7 @@ py::array dispatchMult(PyGRG& grg, grgl_dtype dt, const py::array& arr, size_t r
             initMode = GRGL_INIT_VECTOR;
         }
         if (initArr.ndim() == 2) {
-            API_CHECK((size_t)initArr.shape(0) == rows && (size_t)initArr.shape(1) == grg.h.numNodes(),
+            API_CHECK((size_t)initArr.shape(0) == rows && (size_t)initArr.shape(1) == grg.H().numNodes(),
                       "If init is a matrix, it must match the dimensions ROW x NODES");
             initMode = GRGL_INIT_MATRIX;
         }
@@ -227,9 +259,9 @@ py::array matMul(PyGRGPtr& grg, py::handle input, Direction direction, bool emit
     API_CHECK(arr.ndim() == 2, "matmul() only support two dimensional numpy arrays as input.");
     const size_t rows = arr.shape(0), cols = arr.shape(1);
     API_CHECK(rows != 0 && cols != 0, "matmul() requires non-zero dimensions.");
-    const size_t numSamples = byIndividual ? grg->h.numSamples / grg->h.ploidy : grg->h.numSamples;
+    const size_t numSamples = byIndividual ? grg->H().numSamples / grg->H().ploidy : grg->H().numSamples;
     const size_t outCols =
-        emitAllNodes ? grg->h.numNodes() : (direction == Direction::DOWN ? numSamples : grg->h.numMutations());
+        emitAllNodes ? grg->H().numNodes() : (direction == Direction::DOWN ? numSamples : grg->H().numMutations());
     bool ok = false;
     const grgl_dtype dt = dtypeOf(input, ok);
     API_CHECK(ok, "Unsupported numpy dtype: " << std::string(py::str(arr.dtype())));
@@ -245,7 +277,7 @@ py::array dotProduct(PyGRGPtr& grg, py::handle input, Direction direction) {
     API_CHECK(arr.ndim() == 1, "dot_product() only support one-dimensional numpy arrays as input.");
     const size_t cols = arr.shape(0);
     API_CHECK(cols != 0, "dot_product() requires non-empty array input.");
-    const size_t outCols = direction == Direction::DOWN ? grg->h.numSamples : grg->h.numMutations();
+    const size_t outCols = direction == Direction::DOWN ? grg->H().numSamples : grg->H().numMutations();
     bool ok = false;
     const grgl_dtype dt = dtypeOf(input, ok);
     API_CHECK(ok && (dt == GRGL_F64 || dt == GRGL_F32 || dt == GRGL_I64 || dt == GRGL_I32),
@@ -283,33 +315,33 @@ PyGRGPtr fromArrays(uint32_t numSamples, uint32_t ploidy, py::handle downOffsets
                     py::handle refAlleles, py::handle altAlleles, py::handle populations, py::handle populationIds,
                     py::handle individualIds) {
     auto g = std::make_shared<PyGRG>();
-    g->h.numSamples = numSamples;
-    g->h.ploidy = ploidy;
-    g->h.downOff = toVec<uint64_t>(downOffsets, "down_offsets");
-    g->h.downChild = toVec<uint32_t>(downChildren, "down_children");
-    g->h.mutNode = toVec<uint32_t>(mutationNodes, "mutation_nodes");
-    if (!mutationMiss.is_none()) g->h.mutMiss = toVec<uint32_t>(mutationMiss, "mutation_miss_nodes");
-    if (!coalCounts.is_none()) g->h.coal = toVec<uint32_t>(coalCounts, "coal_counts");
-    if (!positions.is_none()) g->h.mutPos = toVec<double>(positions, "positions");
-    API_CHECK(!g->h.downOff.empty(), "down_offsets must have num_nodes + 1 entries");
+    g->H().numSamples = numSamples;
+    g->H().ploidy = ploidy;
+    g->H().downOff = toVec<uint64_t>(downOffsets, "down_offsets");
+    g->H().downChild = toVec<uint32_t>(downChildren, "down_children");
+    g->H().mutNode = toVec<uint32_t>(mutationNodes, "mutation_nodes");
+    if (!mutationMiss.is_none()) g->H().mutMiss = toVec<uint32_t>(mutationMiss, "mutation_miss_nodes");
+    if (!coalCounts.is_none()) g->H().coal = toVec<uint32_t>(coalCounts, "coal_counts");
+    if (!positions.is_none()) g->H().mutPos = toVec<double>(positions, "positions");
+    API_CHECK(!g->H().downOff.empty(), "down_offsets must have num_nodes + 1 entries");
     if (!refAlleles.is_none() || !altAlleles.is_none()) {
         const auto refs = refAlleles.cast<std::vector<std::string>>();
         const auto alts = altAlleles.cast<std::vector<std::string>>();
-        API_CHECK(refs.size() == g->h.numMutations() && alts.size() == g->h.numMutations(),
+        API_CHECK(refs.size() == g->H().numMutations() && alts.size() == g->H().numMutations(),
                   "ref_alleles and alt_alleles need one entry per mutation");
         for (size_t i = 0; i < refs.size(); i++) {
             API_CHECK(refs[i].size() <= 0xffff, "REF allele length exceeded maximum of 65535");
-            g->h.mutAlleles.push_back(refs[i] + alts[i]);
-            g->h.mutAltIndex.push_back((uint16_t)refs[i].size());
+            g->H().mutAlleles.push_back(refs[i] + alts[i]);
+            g->H().mutAltIndex.push_back((uint16_t)refs[i].size());
         }
     }
-    if (!populations.is_none()) g->h.populations = populations.cast<std::vector<std::string>>();
-    if (!populationIds.is_none()) g->h.popIds = toVec<uint16_t>(populationIds, "population_ids");
+    if (!populations.is_none()) g->H().populations = populations.cast<std::vector<std::string>>();
+    if (!populationIds.is_none()) g->H().popIds = toVec<uint16_t>(populationIds, "population_ids");
     if (!individualIds.is_none()) {
-        g->h.individualIds = individualIds.cast<std::vector<std::string>>();
-        API_CHECK(g->h.individualIds.size() == numSamples / ploidy, "individual_ids needs one entry per individual");
+        g->H().individualIds = individualIds.cast<std::vector<std::string>>();
+        API_CHECK(g->H().individualIds.size() == numSamples / ploidy, "individual_ids needs one entry per individual");
     }
-    validateHostGrg(g->h);
+    validateHostGrg(g->H());
     return g;
 }
 
@@ -324,6 +356,16 @@ PYBIND11_MODULE(_grgl_amd, m) {
         .export_values();
 
     py::class_<PyMutation>(m, "Mutation")
+        .def(py::init([](double position, const std::string& allele, const std::string& refAllele, double time) {
+                 PyMutation x;
+                 x.position = (uint64_t)position;
+                 x.allele = allele;
+                 x.ref_allele = refAllele;
+                 x.time = time < 0 ? 0x3FFFFFFFU : (uint32_t)time; // MUT_TIME_NOT_SET, mutation.h:44
+                 return x;
+             }),
+             py::arg("position"), py::arg("allele"), py::arg("ref_allele") = "", py::arg("time") = -1.0,
+             "A mutation to add to a MutableGRG (src/python/_grgl.cpp:304-312 of the reference).")
         .def_readonly("position", &PyMutation::position)
         .def_readonly("allele", &PyMutation::allele)
         .def_readonly("ref_allele", &PyMutation::ref_allele)
@@ -333,57 +375,57 @@ PYBIND11_MODULE(_grgl_amd, m) {
         });
 
     py::class_<PyGRG, PyGRGPtr>(m, "GRG")
-        .def_property_readonly("num_samples", [](PyGRG& g) { return g.h.numSamples; })
-        .def_property_readonly("num_individuals", [](PyGRG& g) { return g.h.numSamples / g.h.ploidy; })
-        .def_property_readonly("ploidy", [](PyGRG& g) { return g.h.ploidy; })
+        .def_property_readonly("num_samples", [](PyGRG& g) { return g.H().numSamples; })
+        .def_property_readonly("num_individuals", [](PyGRG& g) { return g.H().numSamples / g.H().ploidy; })
+        .def_property_readonly("ploidy", [](PyGRG& g) { return g.H().ploidy; })
         .def_property_readonly("shape",
-                               [](PyGRG& g) { return std::make_pair(g.h.numSamples / g.h.ploidy, g.h.numMutations()); })
-        .def_property_readonly("num_nodes", [](PyGRG& g) { return g.h.numNodes(); })
-        .def_property_readonly("num_edges", [](PyGRG& g) { return g.h.numEdges(); })
-        .def_property_readonly("num_mutations", [](PyGRG& g) { return g.h.numMutations(); })
-        .def_property_readonly("has_missing_data", [](PyGRG& g) { return !g.h.mutMiss.empty(); })
-        .def_property_readonly("has_individual_coals", [](PyGRG& g) { return g.h.hasCoals(); })
+                               [](PyGRG& g) { return std::make_pair(g.H().numSamples / g.H().ploidy, g.H().numMutations()); })
+        .def_property_readonly("num_nodes", [](PyGRG& g) { return g.H().numNodes(); })
+        .def_property_readonly("num_edges", [](PyGRG& g) { return g.H().numEdges(); })
+        .def_property_readonly("num_mutations", [](PyGRG& g) { return g.H().numMutations(); })
+        .def_property_readonly("has_missing_data", [](PyGRG& g) { return !g.H().mutMiss.empty(); })
+        .def_property_readonly("has_individual_coals", [](PyGRG& g) { return g.H().hasCoals(); })
         .def_property_readonly("nodes_are_ordered", [](PyGRG&) { return true; })
         .def_property_readonly("is_mutable", [](PyGRG&) { return false; })
-        .def("is_sample", [](PyGRG& g, uint32_t n) { return n < g.h.numSamples; }, py::arg("node_id"))
+        .def("is_sample", [](PyGRG& g, uint32_t n) { return n < g.H().numSamples; }, py::arg("node_id"))
         .def("num_down_edges",
              [](PyGRG& g, uint32_t n) {
-                 API_CHECK(n < g.h.numNodes(), "Index out of bounds (CSR data)");
-                 return (size_t)(g.h.downOff[n + 1] - g.h.downOff[n]);
+                 API_CHECK(n < g.H().numNodes(), "Index out of bounds (CSR data)");
+                 return (size_t)(g.H().downOff[n + 1] - g.H().downOff[n]);
              },
              py::arg("node_id"))
         .def("get_down_edges",
              [](PyGRG& g, uint32_t n) {
-                 API_CHECK(n < g.h.numNodes(), "Index out of bounds (CSR data)");
-                 return std::vector<uint32_t>(g.h.downChild.begin() + g.h.downOff[n],
-                                              g.h.downChild.begin() + g.h.downOff[n + 1]);
+                 API_CHECK(n < g.H().numNodes(), "Index out of bounds (CSR data)");
+                 return std::vector<uint32_t>(g.H().downChild.begin() + g.H().downOff[n],
+                                              g.H().downChild.begin() + g.H().downOff[n + 1]);
              },
              py::arg("node_id"))
         .def("num_up_edges",
              [](PyGRG& g, uint32_t n) {
                  g.buildUp();
-                 API_CHECK(n < g.h.numNodes(), "Index out of bounds (CSR data)");
+                 API_CHECK(n < g.H().numNodes(), "Index out of bounds (CSR data)");
                  return (size_t)(g.upOff[n + 1] - g.upOff[n]);
              },
              py::arg("node_id"))
         .def("get_up_edges",
              [](PyGRG& g, uint32_t n) {
                  g.buildUp();
-                 API_CHECK(n < g.h.numNodes(), "Index out of bounds (CSR data)");
+                 API_CHECK(n < g.H().numNodes(), "Index out of bounds (CSR data)");
                  return std::vector<uint32_t>(g.upPar.begin() + g.upOff[n], g.upPar.begin() + g.upOff[n + 1]);
              },
              py::arg("node_id"))
         .def("get_mutation_node_pairs",
              [](PyGRG& g) {
                  std::vector<std::pair<uint32_t, uint32_t>> r;
-                 for (uint32_t m = 0; m < g.h.numMutations(); m++) r.push_back({m, g.h.mutNode[m]});
+                 for (uint32_t m = 0; m < g.H().numMutations(); m++) r.push_back({m, g.H().mutNode[m]});
                  return r;
              })
         .def("get_mutation_node_miss",
              [](PyGRG& g) {
                  std::vector<std::tuple<uint32_t, uint32_t, uint32_t>> r;
-                 for (uint32_t m = 0; m < g.h.numMutations(); m++)
-                     r.push_back({m, g.h.mutNode[m], g.h.mutMiss.empty() ? kInvalidNode : g.h.mutMiss[m]});
+                 for (uint32_t m = 0; m < g.H().numMutations(); m++)
+                     r.push_back({m, g.H().mutNode[m], g.H().mutMiss.empty() ? kInvalidNode : g.H().mutMiss[m]});
                  return r;
              })
         .def("get_mutations_for_node",
@@ -402,58 +444,58 @@ PYBIND11_MODULE(_grgl_amd, m) {
              py::arg("node_id"))
         .def("get_num_individual_coals",
              [](PyGRG& g, uint32_t n) {
-                 if (n < g.h.numSamples) return 0U;
-                 if (g.h.coal.empty() || n >= g.h.numNodes()) return kCoalNotSet;
-                 return g.h.coal[n];
+                 if (n < g.H().numSamples) return 0U;
+                 if (g.H().coal.empty() || n >= g.H().numNodes()) return kCoalNotSet;
+                 return g.H().coal[n];
              },
              py::arg("node_id"))
         .def("get_mutation_position", [](PyGRG& g, uint32_t m) {
-            API_CHECK(m < g.h.numMutations(), "Invalid MutationID: " << m);
-            return g.h.mutPos.empty() ? (double)m : g.h.mutPos[m];
+            API_CHECK(m < g.H().numMutations(), "Invalid MutationID: " << m);
+            return g.H().mutPos.empty() ? (double)m : g.H().mutPos[m];
         })
         .def("get_mutation_by_id",
              [](PyGRG& g, uint32_t m) {
-                 API_CHECK(m < g.h.numMutations(), "Invalid MutationID: " << m);
+                 API_CHECK(m < g.H().numMutations(), "Invalid MutationID: " << m);
                  PyMutation r;
-                 r.position = g.h.mutPos.empty() ? (uint64_t)m : (uint64_t)g.h.mutPos[m];
-                 const std::string s = g.h.mutAlleles.empty() ? std::string("AG") : g.h.mutAlleles[m];
-                 const size_t alt = g.h.mutAltIndex.empty() ? 1 : std::min<size_t>(g.h.mutAltIndex[m], s.size());
+                 r.position = g.H().mutPos.empty() ? (uint64_t)m : (uint64_t)g.H().mutPos[m];
+                 const std::string s = g.H().mutAlleles.empty() ? std::string("AG") : g.H().mutAlleles[m];
+                 const size_t alt = g.H().mutAltIndex.empty() ? 1 : std::min<size_t>(g.H().mutAltIndex[m], s.size());
                  r.ref_allele = s.substr(0, alt);
                  r.allele = s.substr(alt);
-                 r.time = g.h.mutTime.empty() ? 0x3FFFFFFFU : g.h.mutTime[m];
+                 r.time = g.H().mutTime.empty() ? 0x3FFFFFFFU : g.H().mutTime[m];
                  return r;
              },
              py::arg("mut_id"))
         .def_property_readonly("specified_bp_range",
-                               [](PyGRG& g) { return std::make_pair(g.h.rangeStart, g.h.rangeEnd); })
-        .def_property_readonly("num_populations", [](PyGRG& g) { return g.h.populations.size(); })
-        .def("get_populations", [](PyGRG& g) { return g.h.populations; })
-        .def_property_readonly("has_individual_ids", [](PyGRG& g) { return !g.h.individualIds.empty(); })
+                               [](PyGRG& g) { return std::make_pair(g.H().rangeStart, g.H().rangeEnd); })
+        .def_property_readonly("num_populations", [](PyGRG& g) { return g.H().populations.size(); })
+        .def("get_populations", [](PyGRG& g) { return g.H().populations; })
+        .def_property_readonly("has_individual_ids", [](PyGRG& g) { return !g.H().individualIds.empty(); })
         .def("get_individual_id",
              [](PyGRG& g, uint32_t i) {
-                 API_CHECK(i < g.h.individualIds.size(), "Invalid individual index " << i);
-                 return g.h.individualIds[i];
+                 API_CHECK(i < g.H().individualIds.size(), "Invalid individual index " << i);
+                 return g.H().individualIds[i];
              },
              py::arg("individual_index"))
         .def("arrays",
              [](PyGRG& g) {
                  py::dict d;
-                 d["num_samples"] = g.h.numSamples;
-                 d["ploidy"] = g.h.ploidy;
-                 d["down_offsets"] = toArray(g.h.downOff);
-                 d["down_children"] = toArray(g.h.downChild);
-                 d["mutation_nodes"] = toArray(g.h.mutNode);
-                 d["mutation_miss_nodes"] = g.h.mutMiss.empty() ? py::object(py::none()) : py::object(toArray(g.h.mutMiss));
-                 d["coal_counts"] = g.h.coal.empty() ? py::object(py::none()) : py::object(toArray(g.h.coal));
-                 d["positions"] = toArray(g.h.mutPos);
+                 d["num_samples"] = g.H().numSamples;
+                 d["ploidy"] = g.H().ploidy;
+                 d["down_offsets"] = toArray(g.H().downOff);
+                 d["down_children"] = toArray(g.H().downChild);
+                 d["mutation_nodes"] = toArray(g.H().mutNode);
+                 d["mutation_miss_nodes"] = g.H().mutMiss.empty() ? py::object(py::none()) : py::object(toArray(g.H().mutMiss));
+                 d["coal_counts"] = g.H().coal.empty() ? py::object(py::none()) : py::object(toArray(g.H().coal));
+                 d["positions"] = toArray(g.H().mutPos);
                  return d;
              },
              "The graph as plain arrays (down-edge CSR, mutation map, coalescence counts).")
         .def("graph_stats",
              [](PyGRG& g) {
-                 const GraphStats st = computeGraphStats(g.h);
+                 const GraphStats st = computeGraphStats(g.H());
                  py::dict d;
-                 d["num_nodes"] = g.h.numNodes();
+                 d["num_nodes"] = g.H().numNodes();
                  d["num_edges"] = st.numEdges;
                  d["up_levels"] = st.upLevels;
                  d["max_fan_in"] = st.maxFanIn;
@@ -482,16 +524,98 @@ PYBIND11_MODULE(_grgl_amd, m) {
         .def("__str__",
              [](PyGRG& g) {
                  std::ostringstream ss;
-                 ss << "<[grgl_amd.GRG] mutations=" << g.h.numMutations() << ", individuals=" << g.h.numSamples / g.h.ploidy
-                    << ", ploidy=" << g.h.ploidy << ", mutable=No>";
+                 ss << "<[grgl_amd.GRG] mutations=" << g.H().numMutations() << ", individuals=" << g.H().numSamples / g.H().ploidy
+                    << ", ploidy=" << g.H().ploidy << ", mutable=No>";
                  return ss.str();
              })
         .def("__repr__", [](PyGRG& g) {
             std::ostringstream ss;
-            ss << "<[grgl_amd.GRG] mutations=" << g.h.numMutations() << ", individuals=" << g.h.numSamples / g.h.ploidy
-               << ", ploidy=" << g.h.ploidy << ", mutable=No>";
+            ss << "<[grgl_amd.GRG] mutations=" << g.H().numMutations() << ", individuals=" << g.H().numSamples / g.H().ploidy
+               << ", ploidy=" << g.H().ploidy << ", mutable=No>";
             return ss.str();
         });
+
+    py::class_<MappingStats>(m, "MutationMappingStats")
+        .def_readonly("total_mutations", &MappingStats::totalMutations)
+        .def_readonly("empty_mutations", &MappingStats::emptyMutations)
+        .def_readonly("mutations_with_one_sample", &MappingStats::mutationsWithOneSample)
+        .def_readonly("mutations_with_no_candidates", &MappingStats::mutationsWithNoCandidates)
+        .def_readonly("reused_nodes", &MappingStats::reusedNodes)
+        .def_readonly("reused_node_coverage", &MappingStats::reusedNodeCoverage)
+        .def_readonly("reused_exactly", &MappingStats::reusedExactly)
+        .def_readonly("singleton_sample_edges", &MappingStats::singletonSampleEdges)
+        .def_readonly("new_tree_nodes", &MappingStats::newTreeNodes)
+        .def_readonly("samples_processed", &MappingStats::samplesProcessed)
+        .def_readonly("num_candidates", &MappingStats::numCandidates)
+        .def_readonly("reuse_size_bigger_than_hist_max", &MappingStats::reuseSizeBiggerThanHistMax)
+        .def_readonly("num_with_singletons", &MappingStats::numWithSingletons)
+        .def_readonly("max_singletons", &MappingStats::maxSingletons)
+        .def_readonly("reused_mut_nodes", &MappingStats::reusedMutNodes)
+        .def_readonly("reuse_size_hist", &MappingStats::reuseSizeHist);
+
+    auto toRecord = [](const PyMutation& x) {
+        MutationRecordIn r;
+        r.position = (double)x.position;
+        r.allele = x.allele;
+        r.refAllele = x.ref_allele;
+        return r;
+    };
+    py::class_<PyMutableGRG, PyGRG, PyMutableGRGPtr>(m, "MutableGRG")
+        .def(py::init([](uint32_t numSamples, uint32_t ploidy, bool phased, size_t initialNodeCapacity) {
+                 (void)initialNodeCapacity;
+                 return std::make_shared<PyMutableGRG>(numSamples, ploidy, phased);
+             }),
+             py::arg("num_samples"), py::arg("ploidy"), py::arg("phased") = true,
+             py::arg("initial_node_capacity") = 1024,
+             "A GRG that can grow: nodes 0..num_samples-1 are the samples (reference: "
+             "src/python/_grgl.cpp:759-775).  New nodes come after every existing node, and edges must go "
+             "from a node to one with a smaller ID, so the graph stays ordered.")
+        .def(
+            "make_node",
+            [](PyMutableGRG& g, uint32_t count, bool negative) {
+                API_CHECK(!negative, "negative nodes are not supported: new nodes always come last");
+                return g.mg.makeNode(count);
+            },
+            py::arg("count") = 1, py::arg("negative") = false)
+        .def(
+            "connect", [](PyMutableGRG& g, uint32_t source, uint32_t target) { g.mg.connect(source, target); },
+            py::arg("source"), py::arg("target"))
+        .def(
+            "add_mutation",
+            [toRecord](PyMutableGRG& g, const PyMutation& mut, uint32_t node, uint32_t miss) {
+                return g.mg.addMutation(toRecord(mut), node, miss);
+            },
+            py::arg("mutation"), py::arg("node_id"), py::arg("miss_node_id") = kInvalidNode)
+        .def(
+            "set_num_individual_coals",
+            [](PyMutableGRG& g, uint32_t node, uint32_t coals) { g.mg.setNumIndividualCoals(node, coals); },
+            py::arg("node_id"), py::arg("num_coals"))
+        .def("__repr__", [](PyMutableGRG& g) {
+            std::ostringstream ss;
+            ss << "<[grgl_amd.GRG] mutations=" << g.mg.numMutations() << ", individuals="
+               << g.mg.numSamples() / g.mg.ploidy() << ", ploidy=" << g.mg.ploidy() << ", mutable=Yes>";
+            return ss.str();
+        });
+
+    m.def(
+        "map_mutations",
+        [toRecord](PyMutableGRG& g, const std::vector<PyMutation>& mutations,
+                   const std::vector<std::vector<uint32_t>>& samples, bool verbose, size_t batchSize,
+                   bool sizeOrder) {
+            (void)verbose;
+            std::vector<MutationRecordIn> recs;
+            recs.reserve(mutations.size());
+            for (const auto& x : mutations) recs.push_back(toRecord(x));
+            py::gil_scoped_release release;
+            return g.mg.mapMutations(recs, samples, batchSize, sizeOrder);
+        },
+        py::arg("grg"), py::arg("mutations"), py::arg("samples"), py::arg("verbose") = false,
+        py::arg("mutation_batch_size") = 0, py::kw_only(), py::arg("size_order") = false,
+        "Map mutations into a MutableGRG so that the samples below each mutation's node are exactly its "
+        "sample list (the contract of the reference's map_mutations, src/python/_grgl.cpp:1029-1062).  "
+        "Nodes already in the graph when a batch starts are reused where they fit.  size_order=True (not in "
+        "the reference) maps the sets smallest first; with mutation_batch_size=1 this builds a compact graph "
+        "from an empty one.  Mutation IDs follow the input order.");
 
     m.def("matmul", &matMul, py::arg("grg"), py::arg("input"), py::arg("direction"), py::arg("emit_all_nodes") = false,
           py::arg("by_individual") = false, py::arg("init") = py::none(), py::arg("miss") = py::none(),
@@ -520,7 +644,7 @@ PYBIND11_MODULE(_grgl_amd, m) {
         [](PyGRGPtr& g, const std::string& filename, bool allowSimplify) {
             (void)allowSimplify; // written as is: nodes and edges are not simplified
             py::gil_scoped_release release;
-            writeGrgFile(g->h, filename);
+            writeGrgFile(g->H(), filename);
         },
         py::arg("grg"), py::arg("filename"), py::arg("allow_simplify") = true,
         "Write the graph as a .grg file, format 5.3, without simplification.");
@@ -557,7 +681,7 @@ PYBIND11_MODULE(_grgl_amd, m) {
     m.def(
         "_level_plan",
         [](PyGRGPtr& g, Direction dir) {
-            const DirPlan p = dir == Direction::UP ? buildUpPlan(g->h) : buildDownPlan(g->h);
+            const DirPlan p = dir == Direction::UP ? buildUpPlan(g->H()) : buildDownPlan(g->H());
             py::dict d;
             d["num_levels"] = p.numLevels;
             d["first_pulled_level"] = p.firstPulledLevel;
