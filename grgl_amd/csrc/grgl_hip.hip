@@ -101,6 +101,10 @@ void DeviceDirPlan::computeHeavy() {
         }
         heavyCount[l] = lo - begin;
     }
+    if (numLevels > 0) {
+        if (!dHeavyCount) GRGL_HIP_CHECK(hipMalloc(&dHeavyCount, numLevels * sizeof(uint32_t)));
+        GRGL_HIP_CHECK(hipMemcpy(dHeavyCount, heavyCount.data(), numLevels * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
 }
 
 size_t rowsBufferBytes(const grgl_graph* g, grgl_direction dir, uint32_t Ktotal, size_t elemSize) {
@@ -404,10 +408,10 @@ void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const
         t.rowBytes = geo.rowBytes;
         t.readSelf = readSelf ? 1 : 0;
         t.tiles = tiles;
+        t.levelStart = plan.dLevelStart + tailFrom;
+        t.heavyCount = plan.dHeavyCount + tailFrom;
         t.numLevels = plan.numLevels - tailFrom;
-        for (uint32_t l = tailFrom; l <= plan.numLevels; l++) t.levelStart[l - tailFrom] = plan.levelStart[l];
-        for (uint32_t l = tailFrom; l < plan.numLevels; l++)
-            t.heavyCount[l - tailFrom] = orderPreserving ? 0 : plan.heavyCount[l];
+        t.noHeavy = orderPreserving ? 1 : 0;
         launchTail(tc, geo.vb, tw, t, stream);
         GRGL_HIP_CHECK(hipGetLastError());
     }
@@ -661,7 +665,9 @@ void uploadPlan(DeviceDirPlan& d, const DirPlan& p, uint64_t& bytes) {
     d.outRow = uploadVec(p.outRow, bytes);
     d.missRow = uploadVec(p.missRow, bytes);
     d.coalByRow = uploadVec(p.coalByRow, bytes);
+    d.dLevelStart = uploadVec(p.levelStart, bytes);
     d.computeHeavy();
+    bytes += d.numLevels * sizeof(uint32_t);
 }
 
 void freePlan(DeviceDirPlan& d) {
@@ -672,6 +678,8 @@ void freePlan(DeviceDirPlan& d) {
     freeDev(d.outRow);
     freeDev(d.missRow);
     freeDev(d.coalByRow);
+    freeDev(d.dLevelStart);
+    freeDev(d.dHeavyCount);
     d = DeviceDirPlan();
 }
 

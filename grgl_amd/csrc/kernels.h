@@ -280,12 +280,13 @@ struct TailArgs {
     uint64_t ld;
     const uint32_t* off;
     const uint32_t* src;
+    const uint32_t* levelStart; // numLevels + 1 row boundaries (device)
+    const uint32_t* heavyCount; // numLevels heavy-row counts (device)
     uint32_t rowBytes;
     uint32_t readSelf;
     uint32_t tiles;       // column slices of TW * VB bytes
     uint32_t numLevels;
-    uint32_t levelStart[kMaxTailLevels + 1];
-    uint32_t heavyCount[kMaxTailLevels];
+    uint32_t noHeavy;     // 1: every row is light (order-preserving traversal)
 };
 
 template <typename T, int VB, int TW, bool XOR>
@@ -300,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void pull_tail_kernel(TailArgs t) {
         a.off = t.off;
         a.src = t.src;
         a.levelBegin = t.levelStart[l];
-        a.heavyCount = t.heavyCount[l];
+        a.heavyCount = t.noHeavy ? 0 : t.heavyCount[l];
         a.levelEnd = t.levelStart[l + 1];
         a.rowBytes = t.rowBytes;
         a.readSelf = t.readSelf;
