@@ -251,6 +251,27 @@ def serve_benchmark(num_requests=2000, load=0.6):
         srv.wait(rid)
     served_poisson_wall = time.perf_counter() - t0
     st = srv.stats(reset=True)
+    # Burst of latency-bound requests only (K in {1, 8}): where batching pays most.
+    small = []
+    for _ in range(num_requests):
+        gi = int(rng.integers(0, len(graphs)))
+        K = int(rng.choice([1, 8]))
+        d = capi.UP if rng.random() < 0.5 else capi.DOWN
+        cdt = capi.F32 if rng.random() < 0.7 else capi.I32
+        small.append((gi, K, d, cdt))
+    small_edge_rhs = sum(edges[gi] * K for gi, K, _, _ in small)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for r in small:
+        plain_one(r)
+    torch.cuda.synchronize()
+    plain_small = time.perf_counter() - t
+    t = time.perf_counter()
+    ids = [served_submit(r) for r in small]
+    for rid in ids:
+        srv.wait(rid)
+    served_small = time.perf_counter() - t
+    st_small = srv.stats(reset=True)
     srv.close()
     for cg in graphs:
         cg.close()
@@ -278,6 +299,15 @@ def serve_benchmark(num_requests=2000, load=0.6):
             "plain_wall_s": round(plain_poisson_wall, 3),
             "served_wall_s": round(served_poisson_wall, 3),
             "served_batches": st["batches"] - batches1,
+        },
+        "burst_small_k": {
+            "workload": "same graphs and mix of direction / dtype, K in {1, 8} p .5/.5",
+            "plain_req_per_s": round(num_requests / plain_small, 1),
+            "served_req_per_s": round(num_requests / served_small, 1),
+            "plain_edge_rhs_per_s": small_edge_rhs / plain_small,
+            "served_edge_rhs_per_s": small_edge_rhs / served_small,
+            "speedup": round(plain_small / served_small, 3),
+            "served_batches": st_small["batches"] - st["batches"],
         },
     }
 

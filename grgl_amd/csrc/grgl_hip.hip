@@ -223,6 +223,32 @@ void launchSeed(TypeClass tc, dim3 grid, const SeedArgs& a, bool vr, bool vm, hi
     }
 }
 
+template <typename T, bool XOR> void launchStackedT(bool seed, dim3 grid, const StackArgs& a, bool vr, bool vm,
+                                                    hipStream_t s) {
+    constexpr int V = 16 / (int)sizeof(T);
+#define GRGL_STACK(R, M)                                                                                            \
+    do {                                                                                                            \
+        if (seed) hipLaunchKernelGGL((seed_stacked_kernel<T, XOR, R, M>), grid, dim3(kBlock), 0, s, a);            \
+        else hipLaunchKernelGGL((readout_stacked_kernel<T, XOR, R, M>), grid, dim3(kBlock), 0, s, a);              \
+    } while (0)
+    if (vr && vm) GRGL_STACK(V, V);
+    else if (vr) GRGL_STACK(V, 1);
+    else if (vm) GRGL_STACK(1, V);
+    else GRGL_STACK(1, 1);
+#undef GRGL_STACK
+}
+void launchStacked(TypeClass tc, bool seed, dim3 grid, const StackArgs& a, bool vr, bool vm, hipStream_t s) {
+    switch (tc) {
+    case kF64: launchStackedT<double, false>(seed, grid, a, vr, vm, s); break;
+    case kF32: launchStackedT<float, false>(seed, grid, a, vr, vm, s); break;
+    case kW64: launchStackedT<uint64_t, false>(seed, grid, a, vr, vm, s); break;
+    case kW32: launchStackedT<uint32_t, false>(seed, grid, a, vr, vm, s); break;
+    case kW16: launchStackedT<uint16_t, false>(seed, grid, a, vr, vm, s); break;
+    case kW8: launchStackedT<uint8_t, false>(seed, grid, a, vr, vm, s); break;
+    case kX8: launchStackedT<uint8_t, true>(seed, grid, a, vr, vm, s); break;
+    }
+}
+
 template <typename T, bool XOR> void launchReadoutT(dim3 grid, const ReadoutArgs& a, bool vr, bool vm, hipStream_t s) {
     constexpr int V = 16 / (int)sizeof(T);
     if (vr && vm) hipLaunchKernelGGL((readout_kernel<T, XOR, V, V>), grid, dim3(kBlock), 0, s, a);
@@ -318,6 +344,53 @@ void readoutRows(TypeClass tc, size_t es, const char* rows, uint64_t ld, uint32_
     const uint32_t TK = 1U << a.tkShift, TC = tileElems(es) / TK;
     dim3 grid((uint32_t)((C + TC - 1) / TC), (K + TK - 1) / TK);
     launchReadout(tc, grid, a, vr, vm, s);
+}
+
+// Stacked transposes (one launch for all requests of a traversal), see kernels.h.  `ptrs`
+// are the request matrices (K x cols each), `seed` selects the direction of the transpose.
+// Requests must be sorted by kOffset and fit kMaxStack.
+void stackedTranspose(TypeClass tc, size_t es, bool seed, const std::vector<TravRequest>& reqs, bool useOut,
+                      const RowGeometry& geo, char* rows, uint32_t rowBase, uint64_t C, uint64_t cols, uint32_t per,
+                      const uint32_t* map, bool accumulate, hipStream_t s) {
+    if (C == 0) return;
+    StackArgs a{};
+    a.rows = rows;
+    a.ld = geo.ld;
+    a.rowBase = rowBase;
+    a.kCols = (uint32_t)(geo.rowBytes / es);
+    a.C = C;
+    a.cols = cols;
+    a.per = per;
+    a.map = map;
+    a.tkShift = tileShift(a.kCols);
+    a.accumulate = accumulate ? 1 : 0;
+    a.normalizeBool = (seed && tc == kX8) ? 1 : 0;
+    a.numReq = (uint32_t)reqs.size();
+    bool vm = per == 1 || !seed;
+    for (size_t i = 0; i < reqs.size(); i++) {
+        const void* p = useOut ? reqs[i].out : reqs[i].in;
+        a.req[i].ptr = p;
+        a.req[i].kOffset = reqs[i].kOffset;
+        a.req[i].K = reqs[i].K;
+        vm = vm && aligned16(p);
+    }
+    vm = vm && (cols * es) % 16 == 0;
+    if (seed) vm = vm && per == 1;
+    const bool vr = geo.ld % 16 == 0 && geo.rowBytes % 16 == 0 && (1ULL << a.tkShift) * es >= 16;
+    const uint32_t TK = 1U << a.tkShift, TC = tileElems(es) / TK;
+    dim3 grid((uint32_t)((C + TC - 1) / TC), (a.kCols + TK - 1) / TK);
+    launchStacked(tc, seed, grid, a, vr, vm, s);
+}
+
+// Requests sorted by kOffset, or empty when they cannot be stacked.
+std::vector<TravRequest> stackable(const std::vector<TravRequest>& reqs) {
+    if (reqs.size() < 2 || reqs.size() > (size_t)kMaxStack) return {};
+    std::vector<TravRequest> sorted(reqs);
+    std::sort(sorted.begin(), sorted.end(),
+              [](const TravRequest& x, const TravRequest& y) { return x.kOffset < y.kOffset; });
+    for (size_t i = 1; i < sorted.size(); i++)
+        if (sorted[i].kOffset < sorted[i - 1].kOffset + sorted[i - 1].K) return {}; // overlapping columns
+    return sorted;
 }
 
 Tuning chooseTuning(grgl_graph* g, grgl_direction dir, TypeClass tc, uint32_t K, const RowGeometry& geo) {
@@ -510,8 +583,32 @@ void enqueueTraversal(grgl_graph* g, grgl_dtype dt, grgl_direction dir, const st
         a.kind = kInitZero;
         launchInit(tc, a, stream);
     }
-    // 2. Seeds.
+    // 2. Seeds.  Several requests without node initialisation: one stacked launch (and one
+    // zero fill of the missingness seed rows for all columns).
+    const std::vector<TravRequest> stacked = stackable(reqs);
+    const bool stackSeeds = !stacked.empty() && !hasInit && (dir == GRGL_DIRECTION_UP || !opt.miss);
+    if (stackSeeds) {
+        if (dir == GRGL_DIRECTION_UP) {
+            const uint32_t per = opt.byIndividual ? g->P : 1;
+            const uint64_t inCols = opt.byIndividual ? g->N / g->P : g->N;
+            stackedTranspose(tc, es, true, stacked, false, geo, rows, 0, g->N, inCols, per, nullptr, false, stream);
+        } else {
+            stackedTranspose(tc, es, true, stacked, false, geo, rows, g->V, g->M, g->M, 1, nullptr, false, stream);
+            if (g->hasMiss) {
+                InitArgs a{};
+                a.rows = rows;
+                a.ld = geo.ld;
+                a.rowBegin = g->V + g->M;
+                a.rowEnd = g->V + 2 * g->M;
+                a.kOffset = 0;
+                a.K = (uint32_t)(geo.rowBytes / es);
+                a.kind = kInitZero;
+                launchInit(tc, a, stream);
+            }
+        }
+    }
     for (const auto& r : reqs) {
+        if (stackSeeds) break;
         if (dir == GRGL_DIRECTION_UP) {
             // Samples are rows 0..N-1 (grg.h:1456-1469).
             const uint32_t per = opt.byIndividual ? g->P : 1;
@@ -544,8 +641,24 @@ void enqueueTraversal(grgl_graph* g, grgl_dtype dt, grgl_direction dir, const st
     // ORDER_PRESERVING: every list is summed by one team front to back, i.e. in the
     // reference's order.
     runLevels(g, dir, tc, geo, tuning, hasInit, opt.orderPreserving, Ktotal, es, rows, stream);
-    // 4. Read-out.
+    // 4. Read-out.  Several requests: one stacked launch for the main outputs.
+    const bool stackOut = !stacked.empty() && !(dir == GRGL_DIRECTION_UP && opt.miss && plan.missRow && !opt.emitAll);
+    if (stackOut) {
+        if (opt.emitAll) {
+            stackedTranspose(tc, es, false, stacked, true, geo, rows, 0, g->V, g->V, 1, plan.rank, opt.accumulate,
+                             stream);
+        } else if (dir == GRGL_DIRECTION_UP) {
+            stackedTranspose(tc, es, false, stacked, true, geo, rows, 0, g->M, g->M, 1, plan.outRow, opt.accumulate,
+                             stream);
+        } else {
+            const uint32_t per = opt.byIndividual ? g->P : 1;
+            const uint64_t C = g->N / per;
+            stackedTranspose(tc, es, false, stacked, true, geo, rows, 0, C, C, per, plan.outRow, opt.accumulate,
+                             stream);
+        }
+    }
     for (const auto& r : reqs) {
+        if (stackOut) break;
         if (opt.emitAll) {
             readoutRows(tc, es, rows, geo.ld, r.kOffset, r.K, plan.rank, 1, g->V, r.out, g->V, opt.accumulate, stream);
         } else if (dir == GRGL_DIRECTION_UP) {

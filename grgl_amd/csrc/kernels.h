@@ -555,6 +555,235 @@ __global__ __launch_bounds__(kBlock) void readout_kernel(ReadoutArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Stacked transposes: the seeds (or read-outs) of every request of a batch in ONE launch.
+// Column k of a stacked row belongs to the request whose [kOffset, kOffset + K) holds it
+// (requests sorted by kOffset, in the kernel arguments); other columns are padding.  The
+// seed writes whole stacked rows (padding as zero) with 16-byte vectors; the read-out reads
+// whole row slices.  Per column the arithmetic is that of seed_rows_kernel /
+// readout_kernel, so results are bit-identical to per-request launches.
+// ---------------------------------------------------------------------------------------
+constexpr int kMaxStack = 128;
+struct StackReq {
+    const void* ptr;  // request input (seed) or output (read-out), row-major K x cols
+    uint32_t kOffset;
+    uint32_t K;
+};
+struct StackArgs {
+    char* rows;
+    uint64_t ld;
+    uint32_t rowBase;     // seed: first destination row
+    uint32_t kCols;       // stacked columns to cover (row bytes / element size)
+    uint64_t C;           // seed: destination rows; read-out: output columns
+    uint64_t cols;        // columns of every request matrix
+    uint32_t per;         // seed: row d reads column d / per; read-out: rows summed per column
+    const uint32_t* map;  // read-out: C * per row indices
+    uint32_t tkShift;
+    uint32_t accumulate;  // read-out: add into the existing output
+    uint32_t normalizeBool;
+    uint32_t numReq;
+    StackReq req[kMaxStack];
+};
+
+// Fills kReq[t] / kLoc[t] for the tile's stacked columns k0 + t (kReq = -1: padding).
+__device__ __forceinline__ void stack_resolve(const StackArgs& a, uint32_t k0, uint32_t TK, int* kReq, uint32_t* kLoc) {
+    for (uint32_t t = threadIdx.x; t < TK; t += kBlock) {
+        const uint32_t k = k0 + t;
+        int lo = 0, hi = (int)a.numReq - 1, r = -1;
+        while (lo <= hi) { // last request with kOffset <= k
+            const int mid = (lo + hi) / 2;
+            if (a.req[mid].kOffset <= k) {
+                r = mid;
+                lo = mid + 1;
+            } else {
+                hi = mid - 1;
+            }
+        }
+        if (r >= 0 && k - a.req[r].kOffset < a.req[r].K) {
+            kReq[t] = r;
+            kLoc[t] = k - a.req[r].kOffset;
+        } else {
+            kReq[t] = -1;
+            kLoc[t] = 0;
+        }
+    }
+}
+
+template <typename T, bool XOR, int VR, int VM>
+__global__ __launch_bounds__(kBlock) void seed_stacked_kernel(StackArgs a) {
+    constexpr int TE = kTileElems<T>();
+    __shared__ T tile[TE + 64];
+    __shared__ int kReq[64];
+    __shared__ uint32_t kLoc[64];
+    const uint32_t TK = 1U << a.tkShift;
+    const uint32_t TC = (uint32_t)TE >> a.tkShift;
+    const uint64_t d0 = (uint64_t)blockIdx.x * TC;
+    const uint32_t k0 = blockIdx.y * TK;
+    stack_resolve(a, k0, TK, kReq, kLoc);
+    __syncthreads();
+    const uint32_t tcShift = ilog2c((uint32_t)TE) - a.tkShift;
+    const uint32_t sK = vecShift<T, VM>(tcShift), vecPerK = 1U << sK;
+    constexpr int NL = TE / VM / kBlock;
+    T v[NL][VM];
+#pragma unroll
+    for (int it = 0; it < NL; it++) {
+        const uint32_t e = threadIdx.x + it * kBlock;
+        const uint32_t k = e >> sK, dv = (e & (vecPerK - 1)) * VM;
+        const uint64_t gd = d0 + dv;
+        const int r = kReq[k];
+#pragma unroll
+        for (int i = 0; i < VM; i++) v[it][i] = (T)0;
+        if (r >= 0) {
+            const T* in = (const T*)a.req[r].ptr + (uint64_t)kLoc[k] * a.cols;
+            if constexpr (VM > 1) {
+                if (gd + VM <= a.C) {
+                    typename RawVec<16>::type raw = *reinterpret_cast<const typename RawVec<16>::type*>(in + gd);
+                    __builtin_memcpy(v[it], &raw, 16);
+                } else {
+                    for (int i = 0; i < VM; i++)
+                        if (gd + i < a.C) v[it][i] = in[gd + i];
+                }
+            } else if (gd < a.C) {
+                v[it][0] = in[gd / a.per];
+            }
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < NL; it++) {
+        const uint32_t e = threadIdx.x + it * kBlock;
+        const uint32_t k = e >> sK, dv = (e & (vecPerK - 1)) * VM;
+#pragma unroll
+        for (int i = 0; i < VM; i++) {
+            T x = v[it][i];
+            if (a.normalizeBool) x = (T)(x != (T)0);
+            tile[k * (TC + 1) + dv + i] = x;
+        }
+    }
+    __syncthreads();
+    const uint32_t sD = vecShift<T, VR>(a.tkShift), vecPerD = 1U << sD;
+#pragma unroll
+    for (int it = 0; it < TE / VR / kBlock; it++) {
+        const uint32_t e = threadIdx.x + it * kBlock;
+        const uint32_t d = e >> sD, k = (e & (vecPerD - 1)) * VR;
+        const uint64_t gd = d0 + d;
+        const uint32_t gk = k0 + k;
+        if (gd >= a.C || gk >= a.kCols) continue;
+        T* dst = (T*)(a.rows + (uint64_t)(a.rowBase + gd) * a.ld) + gk;
+        if constexpr (VR > 1) {
+            T w[VR];
+#pragma unroll
+            for (int i = 0; i < VR; i++) w[i] = tile[(k + i) * (TC + 1) + d];
+            typename RawVec<16>::type raw;
+            __builtin_memcpy(&raw, w, 16);
+            *reinterpret_cast<typename RawVec<16>::type*>(dst) = raw;
+        } else {
+            *dst = tile[k * (TC + 1) + d];
+        }
+    }
+}
+
+template <typename T, bool XOR, int VR, int VM>
+__global__ __launch_bounds__(kBlock) void readout_stacked_kernel(StackArgs a) {
+    constexpr int TE = kTileElems<T>();
+    constexpr int OWN = TE / kBlock / VM;
+    __shared__ T tile[TE + 64];
+    __shared__ uint32_t srcRow[TE];
+    __shared__ int kReq[64];
+    __shared__ uint32_t kLoc[64];
+    const uint32_t TK = 1U << a.tkShift;
+    const uint32_t TC = (uint32_t)TE >> a.tkShift;
+    const uint64_t c0 = (uint64_t)blockIdx.x * TC;
+    const uint32_t k0 = blockIdx.y * TK;
+    stack_resolve(a, k0, TK, kReq, kLoc);
+    __syncthreads();
+    const uint32_t tcShift = ilog2c((uint32_t)TE) - a.tkShift;
+    const uint32_t sK = vecShift<T, VM>(tcShift), vecPerK = 1U << sK;
+    const bool fullCols = c0 + TC <= a.C;
+    T acc[OWN][VM];
+#pragma unroll
+    for (int o = 0; o < OWN; o++) {
+        const uint32_t e = threadIdx.x + o * kBlock;
+        const uint32_t k = e >> sK, c = (e & (vecPerK - 1)) * VM;
+        const int r = kReq[k];
+#pragma unroll
+        for (int i = 0; i < VM; i++) acc[o][i] = (T)0;
+        if (a.accumulate && r >= 0) {
+            const T* src = (const T*)a.req[r].ptr + (uint64_t)kLoc[k] * a.cols + c0 + c;
+            if constexpr (VM > 1) {
+                if (fullCols) {
+                    typename RawVec<16>::type raw = *reinterpret_cast<const typename RawVec<16>::type*>(src);
+                    __builtin_memcpy(acc[o], &raw, 16);
+                    continue;
+                }
+            }
+            for (int i = 0; i < VM; i++)
+                if (c0 + c + i < a.C) acc[o][i] = src[i];
+        }
+    }
+    const uint32_t sC = vecShift<T, VR>(a.tkShift), vecPerC = 1U << sC;
+    for (uint32_t j = 0; j < a.per; j++) {
+        for (uint32_t c = threadIdx.x; c < TC; c += kBlock)
+            srcRow[c] = (c0 + c < a.C) ? a.map[(c0 + c) * a.per + j] : kInvalidRow;
+        __syncthreads();
+        constexpr int NL = TE / VR / kBlock;
+        T v[NL][VR];
+#pragma unroll
+        for (int it = 0; it < NL; it++) {
+            const uint32_t e = threadIdx.x + it * kBlock;
+            const uint32_t c = e >> sC, k = (e & (vecPerC - 1)) * VR;
+            const uint32_t gk = k0 + k;
+            const uint32_t r = srcRow[c];
+#pragma unroll
+            for (int i = 0; i < VR; i++) v[it][i] = (T)0;
+            if (r != kInvalidRow && gk < a.kCols) {
+                const T* src = (const T*)(a.rows + (uint64_t)r * a.ld) + gk;
+                if constexpr (VR > 1) {
+                    typename RawVec<16>::type raw = *reinterpret_cast<const typename RawVec<16>::type*>(src);
+                    __builtin_memcpy(v[it], &raw, 16);
+                } else {
+                    v[it][0] = *src;
+                }
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < NL; it++) {
+            const uint32_t e = threadIdx.x + it * kBlock;
+            const uint32_t c = e >> sC, k = (e & (vecPerC - 1)) * VR;
+#pragma unroll
+            for (int i = 0; i < VR; i++)
+                if (k + i < TK) tile[(k + i) * (TC + 1) + c] = v[it][i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int o = 0; o < OWN; o++) {
+            const uint32_t e = threadIdx.x + o * kBlock;
+            const uint32_t k = e >> sK, c = (e & (vecPerK - 1)) * VM;
+#pragma unroll
+            for (int i = 0; i < VM; i++)
+                if (srcRow[c + i] != kInvalidRow) acc[o][i] = combine<T, XOR>(acc[o][i], tile[k * (TC + 1) + c + i]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int o = 0; o < OWN; o++) {
+        const uint32_t e = threadIdx.x + o * kBlock;
+        const uint32_t k = e >> sK, c = (e & (vecPerK - 1)) * VM;
+        const int r = kReq[k];
+        if (r < 0) continue;
+        T* dst = (T*)a.req[r].ptr + (uint64_t)kLoc[k] * a.cols + c0 + c;
+        if constexpr (VM > 1) {
+            if (fullCols) {
+                typename RawVec<16>::type raw;
+                __builtin_memcpy(&raw, acc[o], 16);
+                *reinterpret_cast<typename RawVec<16>::type*>(dst) = raw;
+                continue;
+            }
+        }
+        for (int i = 0; i < VM; i++)
+            if (c0 + c + i < a.C) dst[i] = acc[o][i];
+    }
+}
+
 enum InitKind : uint32_t { kInitZero = 0, kInitVector = 1, kInitXtx = 3 };
 
 struct InitArgs {

@@ -180,31 +180,51 @@ def _torch_dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
 
 
-@pytest.mark.parametrize("dtype", [np.float32, np.int32, np.float64])
-def test_batch_alone_equals_mixed(gpu, dtype):
-    """Requests stacked into one traversal give the same bits as each request alone."""
+_MISS_GRAPH = {}
+
+
+def _miss_graph():
+    if not _MISS_GRAPH:
+        a = random_grg(21, num_samples=120, num_internal=700, heavy=(90, 300), num_mutations=900, with_miss=True)
+        _MISS_GRAPH["g"] = (a, oracle.OracleGraph(a))
+    return _MISS_GRAPH["g"]
+
+
+@pytest.mark.parametrize("graph", ["random_heavy", "with_miss"])
+@pytest.mark.parametrize("accumulate", [False, True])
+@pytest.mark.parametrize("dtype", [np.float32, np.int32, np.float64, np.int8, np.uint16, np.bool_])
+def test_batch_alone_equals_mixed(gpu, dtype, accumulate, graph):
+    """Requests stacked into one traversal (one stacked seed and read-out launch) give the
+    same bits as each request alone, with and without accumulation into the output."""
     import torch
 
-    arrays, _, og = _graph("random_heavy")
+    if graph == "random_heavy":
+        arrays, _, og = _graph("random_heavy")
+    else:
+        arrays, og = _miss_graph()
     cg = capi.Graph(arrays)
     dt = capi.NUMPY_TO_DTYPE[np.dtype(dtype)]
+    flags = capi.FLAG_ACCUMULATE if accumulate else 0
     rng = np.random.default_rng(8)
     for direction in (capi.UP, capi.DOWN):
         cin = cg.num_samples if direction == capi.UP else cg.num_mutations
         cout = cg.num_mutations if direction == capi.UP else cg.num_samples
-        Ks = [1, 8, 32, 3, 128]
+        Ks = [1, 8, 32, 3, 128, 1, 5]
         xs = [_torch_dev(make_input(rng, dtype, (k, cin))) for k in Ks]
+        base = [_torch_dev(make_input(rng, dtype, (k, cout))) if accumulate else
+                torch.zeros((k, cout), dtype=x.dtype, device="cuda") for k, x in zip(Ks, xs)]
         solo = []
-        for k, x in zip(Ks, xs):
-            out = torch.zeros((k, cout), dtype=x.dtype, device="cuda")
-            cg.matmul_ptr(dt, x.data_ptr(), cin, k, direction, out.data_ptr(), out.numel())
+        for k, x, b0 in zip(Ks, xs, base):
+            out = b0.clone()
+            cg.matmul_ptr(dt, x.data_ptr(), cin, k, direction, out.data_ptr(), out.numel(), flags=flags)
             solo.append(out)
-        outs = [torch.zeros((k, cout), dtype=x.dtype, device="cuda") for k, x in zip(Ks, xs)]
-        cg.matmul_batch_ptr(dt, direction, [(k, x.data_ptr(), o.data_ptr()) for k, x, o in zip(Ks, xs, outs)])
+        outs = [b0.clone() for b0 in base]
+        cg.matmul_batch_ptr(dt, direction, [(k, x.data_ptr(), o.data_ptr()) for k, x, o in zip(Ks, xs, outs)],
+                            flags=flags)
         torch.cuda.synchronize()
         for a, b in zip(solo, outs):
             assert bits_equal(a.cpu().numpy(), b.cpu().numpy())
-        if dtype == np.int32:
+        if np.dtype(dtype).kind in "iub" and not accumulate:
             for k, x, o in zip(Ks, xs, outs):
                 assert bits_equal(o.cpu().numpy(), og.matmul(x.cpu().numpy(), _odir(UP if direction == capi.UP
                                                                                       else DOWN)))
