@@ -111,7 +111,8 @@ class Request(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("team_lanes", ctypes.c_uint32), ("heavy_fanin", ctypes.c_uint32), ("use_graph", ctypes.c_uint32)]
+    _fields_ = [("team_lanes", ctypes.c_uint32), ("heavy_fanin", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
+                ("xcd_map", ctypes.c_uint32)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -304,15 +305,16 @@ class Graph:
             arr[i].output = op
         check(lib().grgl_hip_matmul_batch(self.handle, dtype, direction, arr, len(reqs), flags, stream))
 
-    def set_tuning(self, direction, dtype, rows, team_lanes=0, heavy_fanin=0, use_graph=0):
-        t = Tuning(team_lanes, heavy_fanin, use_graph)
+    def set_tuning(self, direction, dtype, rows, team_lanes=0, heavy_fanin=0, use_graph=0, xcd_map=0):
+        t = Tuning(team_lanes, heavy_fanin, use_graph, xcd_map)
         check(lib().grgl_hip_set_tuning(self.handle, direction, dtype, rows, ctypes.byref(t)))
 
     def autotune(self, direction, dtype, rows):
         t = Tuning()
         ms = ctypes.c_float()
         check(lib().grgl_hip_autotune(self.handle, direction, dtype, rows, ctypes.byref(t), ctypes.byref(ms)))
-        return {"team_lanes": t.team_lanes, "heavy_fanin": t.heavy_fanin, "use_graph": t.use_graph}, ms.value
+        return {"team_lanes": t.team_lanes, "heavy_fanin": t.heavy_fanin, "use_graph": t.use_graph,
+                "xcd_map": t.xcd_map}, ms.value
 
     def kernel_timing(self, enable):
         check(lib().grgl_hip_kernel_timing(self.handle, int(enable)))

@@ -86,6 +86,7 @@ struct Workspace {
 struct Tuning {
     uint32_t teamLanes = 0; // 0 = default
     uint32_t useGraph = 0;
+    uint32_t xcdMap = 0;    // 0 = default (off), 1 = XCD-aware block order, 2 = off
 };
 
 struct CapturedLevels {

@@ -433,7 +433,7 @@ uint32_t tailStart(const DeviceDirPlan& plan, const RowGeometry& geo, uint32_t t
 }
 
 void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const RowGeometry& geo, uint32_t tw,
-                   bool readSelf, bool orderPreserving, uint32_t K, size_t elemSize, char* rows, hipStream_t stream) {
+                   bool xcdMap, bool readSelf, bool orderPreserving, uint32_t K, size_t elemSize, char* rows, hipStream_t stream) {
     const uint32_t tiles = (geo.lanes + tw - 1) / tw;
     const uint32_t teamsPerBlock = kBlock / tw;
     const uint32_t tailFrom = g->timing ? plan.numLevels : tailStart(plan, geo, tw, orderPreserving);
@@ -452,7 +452,9 @@ void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const
         a.levelEnd = end;
         a.rowBytes = geo.rowBytes;
         a.readSelf = readSelf ? 1 : 0;
-        dim3 grid(heavy + (light + teamsPerBlock - 1) / teamsPerBlock, tiles);
+        const uint32_t bxCount = heavy + (light + teamsPerBlock - 1) / teamsPerBlock;
+        a.xcdTiles = (xcdMap && tiles % 8 == 0) ? tiles : 0;
+        dim3 grid = a.xcdTiles ? dim3(bxCount * tiles, 1) : dim3(bxCount, tiles);
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
         if (g->timing) {
             GRGL_HIP_CHECK(hipEventCreate(&ev0));
@@ -497,11 +499,12 @@ void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometr
                bool readSelf, bool orderPreserving, uint32_t K, size_t es, char* rows, hipStream_t stream) {
     DeviceDirPlan& plan = dir == GRGL_DIRECTION_UP ? g->up : g->down;
     if (!t.useGraph || stream == nullptr || g->timing) {
-        enqueueLevels(g, plan, tc, geo, t.teamLanes, readSelf, orderPreserving, K, es, rows, stream);
+        enqueueLevels(g, plan, tc, geo, t.teamLanes, t.xcdMap == 1, readSelf, orderPreserving, K, es, rows, stream);
         return;
     }
     const auto key = std::make_tuple((int)dir, (int)tc, geo.vb, t.teamLanes, geo.ld, geo.rowBytes,
-                                     (int)readSelf * 2 + (int)orderPreserving + 4 * (int)plan.heavyFanin,
+                                     (int)readSelf * 2 + (int)orderPreserving + 4 * (int)(t.xcdMap == 1) +
+                                         8 * (int)plan.heavyFanin,
                                      (uintptr_t)rows);
     hipGraphExec_t exec = nullptr;
     {
@@ -513,7 +516,7 @@ void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometr
         CapturedLevels cap;
         GRGL_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         try {
-            enqueueLevels(g, plan, tc, geo, t.teamLanes, readSelf, orderPreserving, K, es, rows, stream);
+            enqueueLevels(g, plan, tc, geo, t.teamLanes, t.xcdMap == 1, readSelf, orderPreserving, K, es, rows, stream);
         } catch (...) {
             hipGraph_t broken = nullptr;
             (void)hipStreamEndCapture(stream, &broken);
@@ -1091,6 +1094,8 @@ GRGL_EXPORT grgl_status grgl_hip_set_tuning(grgl_graph* g, grgl_direction direct
     Tuning t;
     t.teamLanes = tuning->team_lanes;
     t.useGraph = tuning->use_graph;
+    if (tuning->xcd_map > 2) throw MisuseError("xcd_map must be 0, 1 or 2");
+    t.xcdMap = tuning->xcd_map;
     g->tuning[std::make_tuple((int)direction, (int)tc, rows)] = t;
     if (tuning->heavy_fanin) {
         DeviceDirPlan& plan = direction == GRGL_DIRECTION_UP ? g->up : g->down;
@@ -1165,11 +1170,15 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
         GRGL_HIP_CHECK(hipEventCreate(&e0));
         GRGL_HIP_CHECK(hipEventCreate(&e1));
         GRGL_HIP_CHECK(hipMemsetAsync(ws->ptr, 0, rowsBufferBytes(g, direction, rows, es), s));
-        for (uint32_t useGraph = 0; useGraph <= 1; useGraph++) {
-            for (uint32_t tw : tws) {
+        for (uint32_t trialIdx = 0; trialIdx < 2 * tws.size() * 2; trialIdx++) {
+            const uint32_t useGraph = trialIdx % 2, xcd = 1 + (trialIdx / 2) % 2;
+            const uint32_t tw = tws[trialIdx / 4];
+            if (xcd == 2 && ((geo.lanes + tw - 1) / tw) % 8 != 0) continue; // same launch as xcd = 1
+            {
                 Tuning t;
                 t.teamLanes = tw;
                 t.useGraph = useGraph;
+                t.xcdMap = xcd;
                 runLevels(g, direction, tc, geo, t, false, false, rows, es, ws->ptr, s); // warm-up / capture
                 float trial[3];
                 for (float& ms : trial) {
@@ -1207,6 +1216,7 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
         chosen->team_lanes = best.teamLanes;
         chosen->heavy_fanin = (direction == GRGL_DIRECTION_UP ? g->up : g->down).heavyFanin;
         chosen->use_graph = best.useGraph;
+        chosen->xcd_map = best.xcdMap;
     }
     if (best_ms) *best_ms = bestMs;
     GRGL_API_END

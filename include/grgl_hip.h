@@ -150,6 +150,11 @@ typedef struct grgl_tuning {
     uint32_t team_lanes;     /* lanes that share one node's row slice: 1..64, power of two */
     uint32_t heavy_fanin;    /* gather lists longer than this are split over a whole block */
     uint32_t use_graph;      /* 1 = replay the level sequence as a captured hipGraph */
+    uint32_t xcd_map;        /* block order of the level kernels when the row splits into a multiple
+                                of 8 column slices: 0 = default (plain 2-D grid), 1 = XCD-aware (XCD i
+                                takes slices i, i+8, ...), 2 = plain 2-D grid.  The plain grid runs the
+                                column slices one after another, so one slice of every row (V x 256 B)
+                                stays in the memory-side cache shared by all XCDs; measured faster. */
 } grgl_tuning;
 
 grgl_status grgl_hip_set_tuning(grgl_graph* graph, grgl_direction direction, grgl_dtype dtype, uint32_t rows,
