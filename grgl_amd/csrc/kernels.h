@@ -555,7 +555,7 @@ __global__ __launch_bounds__(kBlock) void readout_kernel(ReadoutArgs a) {
                         acc[o][i] = combine<T, XOR>(acc[o][i], tile[k * (TC + 1) + c + i]);
             }
         }
-        __syncthreads();
+        if (j + 1 < a.per) __syncthreads(); // the next row of the sum reuses srcRow and the tile
     }
 #pragma unroll
     for (int o = 0; o < OWN; o++) {
@@ -606,26 +606,26 @@ struct StackArgs {
     StackReq req[kMaxStack];
 };
 
-// Fills kReq[t] / kLoc[t] for the tile's stacked columns k0 + t (kReq = -1: padding).
-__device__ __forceinline__ void stack_resolve(const StackArgs& a, uint32_t k0, uint32_t TK, int* kReq, uint32_t* kLoc) {
-    for (uint32_t t = threadIdx.x; t < TK; t += kBlock) {
-        const uint32_t k = k0 + t;
-        int lo = 0, hi = (int)a.numReq - 1, r = -1;
-        while (lo <= hi) { // last request with kOffset <= k
-            const int mid = (lo + hi) / 2;
-            if (a.req[mid].kOffset <= k) {
-                r = mid;
-                lo = mid + 1;
-            } else {
-                hi = mid - 1;
-            }
-        }
-        if (r >= 0 && k - a.req[r].kOffset < a.req[r].K) {
-            kReq[t] = r;
-            kLoc[t] = k - a.req[r].kOffset;
-        } else {
-            kReq[t] = -1;
-            kLoc[t] = 0;
+// kPtr[t] = first element of the request matrix row that holds stacked column k0 + t, or
+// null for padding.  Resolved once per block into LDS.  The request table is read only with
+// block-uniform indices (a uniform search for the first request that reaches the tile, then
+// a walk over the requests overlapping it), so those reads are scalar loads through the
+// scalar cache; per-lane indexing of the kernel-argument table would be vector loads from
+// kernel-argument memory in every block.
+__device__ __forceinline__ void stack_resolve(const StackArgs& a, uint32_t k0, uint32_t TK, char** kPtr, size_t es) {
+    for (uint32_t t = threadIdx.x; t < TK; t += kBlock) kPtr[t] = nullptr;
+    uint32_t lo = 0, hi = a.numReq; // first request with kOffset + K > k0
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (a.req[mid].kOffset + a.req[mid].K <= k0) lo = mid + 1;
+        else hi = mid;
+    }
+    for (uint32_t r = lo; r < a.numReq && a.req[r].kOffset < k0 + TK; r++) {
+        const uint32_t ko = a.req[r].kOffset, K = a.req[r].K;
+        char* base = (char*)a.req[r].ptr;
+        for (uint32_t t = threadIdx.x; t < TK; t += kBlock) {
+            const uint32_t k = k0 + t;
+            if (k >= ko && k < ko + K) kPtr[t] = base + (uint64_t)(k - ko) * a.cols * es;
         }
     }
 }
@@ -634,13 +634,12 @@ template <typename T, bool XOR, int VR, int VM>
 __global__ __launch_bounds__(kBlock) void seed_stacked_kernel(StackArgs a) {
     constexpr int TE = kTileElems<T>();
     __shared__ T tile[TE + 64];
-    __shared__ int kReq[64];
-    __shared__ uint32_t kLoc[64];
+    __shared__ char* kPtr[64];
     const uint32_t TK = 1U << a.tkShift;
     const uint32_t TC = (uint32_t)TE >> a.tkShift;
     const uint64_t d0 = (uint64_t)blockIdx.x * TC;
     const uint32_t k0 = blockIdx.y * TK;
-    stack_resolve(a, k0, TK, kReq, kLoc);
+    stack_resolve(a, k0, TK, kPtr, sizeof(T));
     __syncthreads();
     const uint32_t tcShift = ilog2c((uint32_t)TE) - a.tkShift;
     const uint32_t sK = vecShift<T, VM>(tcShift), vecPerK = 1U << sK;
@@ -651,11 +650,10 @@ __global__ __launch_bounds__(kBlock) void seed_stacked_kernel(StackArgs a) {
         const uint32_t e = threadIdx.x + it * kBlock;
         const uint32_t k = e >> sK, dv = (e & (vecPerK - 1)) * VM;
         const uint64_t gd = d0 + dv;
-        const int r = kReq[k];
+        const T* in = (const T*)kPtr[k];
 #pragma unroll
         for (int i = 0; i < VM; i++) v[it][i] = (T)0;
-        if (r >= 0) {
-            const T* in = (const T*)a.req[r].ptr + (uint64_t)kLoc[k] * a.cols;
+        if (in != nullptr) {
             if constexpr (VM > 1) {
                 if (gd + VM <= a.C) {
                     typename RawVec<16>::type raw = *reinterpret_cast<const typename RawVec<16>::type*>(in + gd);
@@ -709,14 +707,17 @@ __global__ __launch_bounds__(kBlock) void readout_stacked_kernel(StackArgs a) {
     constexpr int OWN = TE / kBlock / VM;
     __shared__ T tile[TE + 64];
     __shared__ uint32_t srcRow[TE];
-    __shared__ int kReq[64];
-    __shared__ uint32_t kLoc[64];
+    __shared__ char* kPtr[64];
     const uint32_t TK = 1U << a.tkShift;
     const uint32_t TC = (uint32_t)TE >> a.tkShift;
     const uint64_t c0 = (uint64_t)blockIdx.x * TC;
     const uint32_t k0 = blockIdx.y * TK;
-    stack_resolve(a, k0, TK, kReq, kLoc);
-    __syncthreads();
+    // The row pointers are needed by the accumulate preload and the final store only; without
+    // accumulation they are resolved under the first map-slice barrier below.
+    if (a.accumulate) {
+        stack_resolve(a, k0, TK, kPtr, sizeof(T));
+        __syncthreads();
+    }
     const uint32_t tcShift = ilog2c((uint32_t)TE) - a.tkShift;
     const uint32_t sK = vecShift<T, VM>(tcShift), vecPerK = 1U << sK;
     const bool fullCols = c0 + TC <= a.C;
@@ -725,11 +726,11 @@ __global__ __launch_bounds__(kBlock) void readout_stacked_kernel(StackArgs a) {
     for (int o = 0; o < OWN; o++) {
         const uint32_t e = threadIdx.x + o * kBlock;
         const uint32_t k = e >> sK, c = (e & (vecPerK - 1)) * VM;
-        const int r = kReq[k];
+        const T* row = (const T*)kPtr[k];
 #pragma unroll
         for (int i = 0; i < VM; i++) acc[o][i] = (T)0;
-        if (a.accumulate && r >= 0) {
-            const T* src = (const T*)a.req[r].ptr + (uint64_t)kLoc[k] * a.cols + c0 + c;
+        if (a.accumulate && row != nullptr) {
+            const T* src = row + c0 + c;
             if constexpr (VM > 1) {
                 if (fullCols) {
                     typename RawVec<16>::type raw = *reinterpret_cast<const typename RawVec<16>::type*>(src);
@@ -745,6 +746,7 @@ __global__ __launch_bounds__(kBlock) void readout_stacked_kernel(StackArgs a) {
     for (uint32_t j = 0; j < a.per; j++) {
         for (uint32_t c = threadIdx.x; c < TC; c += kBlock)
             srcRow[c] = (c0 + c < a.C) ? a.map[(c0 + c) * a.per + j] : kInvalidRow;
+        if (j == 0 && !a.accumulate) stack_resolve(a, k0, TK, kPtr, sizeof(T));
         __syncthreads();
         constexpr int NL = TE / VR / kBlock;
         T v[NL][VR];
@@ -783,15 +785,15 @@ __global__ __launch_bounds__(kBlock) void readout_stacked_kernel(StackArgs a) {
             for (int i = 0; i < VM; i++)
                 if (srcRow[c + i] != kInvalidRow) acc[o][i] = combine<T, XOR>(acc[o][i], tile[k * (TC + 1) + c + i]);
         }
-        __syncthreads();
+        if (j + 1 < a.per) __syncthreads(); // the next row of the sum reuses srcRow and the tile
     }
 #pragma unroll
     for (int o = 0; o < OWN; o++) {
         const uint32_t e = threadIdx.x + o * kBlock;
         const uint32_t k = e >> sK, c = (e & (vecPerK - 1)) * VM;
-        const int r = kReq[k];
-        if (r < 0) continue;
-        T* dst = (T*)a.req[r].ptr + (uint64_t)kLoc[k] * a.cols + c0 + c;
+        T* row = (T*)kPtr[k];
+        if (row == nullptr) continue;
+        T* dst = row + c0 + c;
         if constexpr (VM > 1) {
             if (fullCols) {
                 typename RawVec<16>::type raw;
