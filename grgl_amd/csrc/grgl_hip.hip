@@ -226,10 +226,13 @@ void launchSeed(TypeClass tc, dim3 grid, const SeedArgs& a, bool vr, bool vm, hi
 template <typename T, bool XOR> void launchStackedT(bool seed, dim3 grid, const StackArgs& a, bool vr, bool vm,
                                                     hipStream_t s) {
     constexpr int V = 16 / (int)sizeof(T);
+    const int mode = a.per != 1 ? 0 : (a.accumulate ? 2 : 1);
 #define GRGL_STACK(R, M)                                                                                            \
     do {                                                                                                            \
         if (seed) hipLaunchKernelGGL((seed_stacked_kernel<T, XOR, R, M>), grid, dim3(kBlock), 0, s, a);            \
-        else hipLaunchKernelGGL((readout_stacked_kernel<T, XOR, R, M>), grid, dim3(kBlock), 0, s, a);              \
+        else if (mode == 1) hipLaunchKernelGGL((readout_stacked_kernel<T, XOR, R, M, 1>), grid, dim3(kBlock), 0, s, a); \
+        else if (mode == 2) hipLaunchKernelGGL((readout_stacked_kernel<T, XOR, R, M, 2>), grid, dim3(kBlock), 0, s, a); \
+        else hipLaunchKernelGGL((readout_stacked_kernel<T, XOR, R, M, 0>), grid, dim3(kBlock), 0, s, a);           \
     } while (0)
     if (vr && vm) GRGL_STACK(V, V);
     else if (vr) GRGL_STACK(V, 1);
@@ -249,12 +252,18 @@ void launchStacked(TypeClass tc, bool seed, dim3 grid, const StackArgs& a, bool 
     }
 }
 
-template <typename T, bool XOR> void launchReadoutT(dim3 grid, const ReadoutArgs& a, bool vr, bool vm, hipStream_t s) {
+template <typename T, bool XOR, int MODE> void launchReadoutM(dim3 grid, const ReadoutArgs& a, bool vr, bool vm,
+                                                             hipStream_t s) {
     constexpr int V = 16 / (int)sizeof(T);
-    if (vr && vm) hipLaunchKernelGGL((readout_kernel<T, XOR, V, V>), grid, dim3(kBlock), 0, s, a);
-    else if (vr) hipLaunchKernelGGL((readout_kernel<T, XOR, V, 1>), grid, dim3(kBlock), 0, s, a);
-    else if (vm) hipLaunchKernelGGL((readout_kernel<T, XOR, 1, V>), grid, dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL((readout_kernel<T, XOR, 1, 1>), grid, dim3(kBlock), 0, s, a);
+    if (vr && vm) hipLaunchKernelGGL((readout_kernel<T, XOR, V, V, MODE>), grid, dim3(kBlock), 0, s, a);
+    else if (vr) hipLaunchKernelGGL((readout_kernel<T, XOR, V, 1, MODE>), grid, dim3(kBlock), 0, s, a);
+    else if (vm) hipLaunchKernelGGL((readout_kernel<T, XOR, 1, V, MODE>), grid, dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL((readout_kernel<T, XOR, 1, 1, MODE>), grid, dim3(kBlock), 0, s, a);
+}
+template <typename T, bool XOR> void launchReadoutT(dim3 grid, const ReadoutArgs& a, bool vr, bool vm, hipStream_t s) {
+    if (a.per != 1) launchReadoutM<T, XOR, 0>(grid, a, vr, vm, s);
+    else if (a.accumulate) launchReadoutM<T, XOR, 2>(grid, a, vr, vm, s);
+    else launchReadoutM<T, XOR, 1>(grid, a, vr, vm, s);
 }
 void launchReadout(TypeClass tc, dim3 grid, const ReadoutArgs& a, bool vr, bool vm, hipStream_t s) {
     switch (tc) {

@@ -470,11 +470,94 @@ struct ReadoutArgs {
     uint32_t tkShift;
 };
 
+// Read-out of one row per output column (per == 1): batched row loads into the LDS tile,
+// then each output vector is 0 (or the existing output) combined with the tile value and
+// stored directly.  Unmapped columns hold 0 in the tile: without accumulation the result is
+// 0 + 0 = 0; with accumulation they keep the existing output (nothing is added).
+// `outRow(k)` points at output row k of the tile (column c0).
+template <typename T, bool XOR, int VR, int VM, bool ACC, typename OutRow>
+__device__ __forceinline__ void readout_single(const char* rows, uint64_t ld, uint32_t kOffset, uint32_t K,
+                                               const uint32_t* map, uint64_t C, uint64_t c0,
+                                               uint32_t k0, uint32_t TK, uint32_t TC, uint32_t sK, uint32_t vecPerK,
+                                               uint32_t sC, uint32_t vecPerC, bool fullCols, OutRow outRow, T* tile,
+                                               uint32_t* srcRow) {
+    constexpr int TE = kTileElems<T>();
+    for (uint32_t c = threadIdx.x; c < TC; c += kBlock) srcRow[c] = (c0 + c < C) ? map[c0 + c] : kInvalidRow;
+    __syncthreads();
+    constexpr int NL = TE / VR / kBlock;
+    T v[NL][VR];
+#pragma unroll
+    for (int it = 0; it < NL; it++) {
+        const uint32_t e = threadIdx.x + it * kBlock;
+        const uint32_t c = e >> sC, k = (e & (vecPerC - 1)) * VR;
+        const uint32_t gk = k0 + k;
+        const uint32_t r = srcRow[c];
+#pragma unroll
+        for (int i = 0; i < VR; i++) v[it][i] = (T)0;
+        if (r != kInvalidRow && gk < K) {
+            const T* src = (const T*)(rows + (uint64_t)r * ld) + kOffset + gk;
+            if constexpr (VR > 1) {
+                typename RawVec<16>::type raw = *reinterpret_cast<const typename RawVec<16>::type*>(src);
+                __builtin_memcpy(v[it], &raw, 16);
+            } else {
+                v[it][0] = *src;
+            }
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < NL; it++) {
+        const uint32_t e = threadIdx.x + it * kBlock;
+        const uint32_t c = e >> sC, k = (e & (vecPerC - 1)) * VR;
+#pragma unroll
+        for (int i = 0; i < VR; i++)
+            if (k + i < TK) tile[(k + i) * (TC + 1) + c] = v[it][i];
+    }
+    __syncthreads();
+    constexpr int OWN = TE / kBlock / VM;
+#pragma unroll
+    for (int o = 0; o < OWN; o++) {
+        const uint32_t e = threadIdx.x + o * kBlock;
+        const uint32_t k = e >> sK, c = (e & (vecPerK - 1)) * VM;
+        if (k >= TK || k0 + k >= K) continue;
+        T* dst = outRow(k);
+        if (dst == nullptr) continue;
+        dst += c;
+        T w[VM];
+        if constexpr (ACC) {
+            if (VM > 1 && fullCols) {
+                typename RawVec<16>::type raw = *reinterpret_cast<const typename RawVec<16>::type*>(dst);
+                __builtin_memcpy(w, &raw, 16);
+            } else {
+                for (int i = 0; i < VM; i++) w[i] = (c0 + c + i < C) ? dst[i] : (T)0;
+            }
+#pragma unroll
+            for (int i = 0; i < VM; i++)
+                if (srcRow[c + i] != kInvalidRow) w[i] = combine<T, XOR>(w[i], tile[k * (TC + 1) + c + i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < VM; i++) w[i] = combine<T, XOR>((T)0, tile[k * (TC + 1) + c + i]);
+        }
+        if constexpr (VM > 1) {
+            if (fullCols) {
+                typename RawVec<16>::type raw;
+                __builtin_memcpy(&raw, w, 16);
+                *reinterpret_cast<typename RawVec<16>::type*>(dst) = raw;
+                continue;
+            }
+        }
+        for (int i = 0; i < VM; i++)
+            if (c0 + c + i < C) dst[i] = w[i];
+    }
+}
+
 // The block's slice of the column map is staged in LDS once per summed row (`per`), so
 // the load phase reads one row index per column and the sum phase checks validity in LDS.
 // Unmapped columns contribute nothing: the written value is the initial value (0, or the
 // existing output when accumulating), bit for bit.
-template <typename T, bool XOR, int VR, int VM>
+// MODE: 0 = any `per` (rows summed per column), 1 = one row per column, 2 = one row per
+// column added into the existing output.  The host picks the mode, so each instantiation
+// carries only its own path (registers and branches of the others cost ~20% here).
+template <typename T, bool XOR, int VR, int VM, int MODE>
 __global__ __launch_bounds__(kBlock) void readout_kernel(ReadoutArgs a) {
     constexpr int TE = kTileElems<T>();
     constexpr int OWN = TE / kBlock / VM; // matrix-side vectors owned per thread
@@ -488,6 +571,16 @@ __global__ __launch_bounds__(kBlock) void readout_kernel(ReadoutArgs a) {
     const uint32_t tcShift = ilog2c((uint32_t)TE) - a.tkShift;
     const uint32_t sK = vecShift<T, VM>(tcShift), vecPerK = 1U << sK;
     const bool fullCols = c0 + TC <= a.C;
+    const uint32_t sC = vecShift<T, VR>(a.tkShift), vecPerC = 1U << sC;
+    if constexpr (MODE != 0) {
+        // One row per column: the sum is 0 (or the existing output) plus the row value,
+        // written straight from the tile.
+        readout_single<T, XOR, VR, VM, MODE == 2>(a.rows, a.ld, a.kOffset, a.K, a.map, a.C, c0, k0, TK, TC, sK,
+                                                  vecPerK, sC, vecPerC, fullCols,
+                                                  [&](uint32_t k) { return out + (uint64_t)(k0 + k) * a.outCols + c0; },
+                                                  tile, srcRow);
+        return;
+    }
     T acc[OWN][VM];
 #pragma unroll
     for (int o = 0; o < OWN; o++) {
@@ -508,7 +601,6 @@ __global__ __launch_bounds__(kBlock) void readout_kernel(ReadoutArgs a) {
                 if (c0 + c + i < a.C) acc[o][i] = src[i];
         }
     }
-    const uint32_t sC = vecShift<T, VR>(a.tkShift), vecPerC = 1U << sC;
     for (uint32_t j = 0; j < a.per; j++) {
         for (uint32_t c = threadIdx.x; c < TC; c += kBlock)
             srcRow[c] = (c0 + c < a.C) ? a.map[(c0 + c) * a.per + j] : kInvalidRow;
@@ -701,7 +793,7 @@ __global__ __launch_bounds__(kBlock) void seed_stacked_kernel(StackArgs a) {
     }
 }
 
-template <typename T, bool XOR, int VR, int VM>
+template <typename T, bool XOR, int VR, int VM, int MODE>
 __global__ __launch_bounds__(kBlock) void readout_stacked_kernel(StackArgs a) {
     constexpr int TE = kTileElems<T>();
     constexpr int OWN = TE / kBlock / VM;
@@ -712,6 +804,16 @@ __global__ __launch_bounds__(kBlock) void readout_stacked_kernel(StackArgs a) {
     const uint32_t TC = (uint32_t)TE >> a.tkShift;
     const uint64_t c0 = (uint64_t)blockIdx.x * TC;
     const uint32_t k0 = blockIdx.y * TK;
+    if constexpr (MODE != 0) {
+        const uint32_t tcShift1 = ilog2c((uint32_t)TE) - a.tkShift;
+        const uint32_t sK1 = vecShift<T, VM>(tcShift1), sC1 = vecShift<T, VR>(a.tkShift);
+        stack_resolve(a, k0, TK, kPtr, sizeof(T)); // published by the helper's first barrier
+        readout_single<T, XOR, VR, VM, MODE == 2>(a.rows, a.ld, 0, a.kCols, a.map, a.C, c0, k0, TK, TC, sK1,
+                                                  1U << sK1, sC1, 1U << sC1, c0 + TC <= a.C,
+                                                  [&](uint32_t k) { return kPtr[k] ? (T*)kPtr[k] + c0 : (T*)nullptr; },
+                                                  tile, srcRow);
+        return;
+    }
     // The row pointers are needed by the accumulate preload and the final store only; without
     // accumulation they are resolved under the first map-slice barrier below.
     if (a.accumulate) {
