@@ -93,6 +93,19 @@ __device__ __forceinline__ void acc_pack(Pack<T, VB>& a, const Pack<T, VB>& b) {
     }
 }
 
+// Lane shuffle of a whole pack (32-bit words), from the lane `delta` above within the
+// 64-lane wavefront.
+template <typename T, int VB> __device__ __forceinline__ Pack<T, VB> shfl_down_pack(const Pack<T, VB>& v, unsigned delta) {
+    static_assert(VB % 4 == 0, "packs are whole 32-bit words");
+    int w[VB / 4];
+    __builtin_memcpy(w, &v, VB);
+#pragma unroll
+    for (int i = 0; i < VB / 4; i++) w[i] = __shfl_down(w[i], delta, 64);
+    Pack<T, VB> out;
+    __builtin_memcpy(&out, w, VB);
+    return out;
+}
+
 struct PullArgs {
     char* rows;
     uint64_t ld;          // row stride, bytes
@@ -182,7 +195,7 @@ __device__ __forceinline__ void pull_heavy(const PullArgs& a, uint32_t row, uint
         // are issued before the adds, which run step by step, so each partial's order is
         // unchanged.  The last, partial iteration loads clamped (in-list) entries and adds
         // only the real ones.
-        constexpr int STEPS = PPT >= 4 ? 1 : 4 / PPT;
+        constexpr int STEPS = PPT >= 4 ? 1 : (VB * PPT <= 4 ? 32 : (VB * PPT <= 8 ? 16 : 4 / PPT));
         constexpr uint32_t STRIDE = (uint32_t)(STEPS * PPT) * A;
         uint32_t i = b + team;
         for (; i + STRIDE - A < e; i += STRIDE) {
@@ -219,8 +232,29 @@ __device__ __forceinline__ void pull_heavy(const PullArgs& a, uint32_t row, uint
 #pragma unroll
             for (int q = 0; q < s / A; q++) acc_pack<T, VB, XOR>(part[q], part[q + s / A]);
         }
-        store_pack<T, VB>(lds + ((uint64_t)team * TW + lane) * VB, part[0]);
+        if constexpr (TW == 1) {
+            // Single-lane teams: the 64 partials are the 64 lanes of wavefront 0, so the
+            // tree (strides 32 .. 1, partial t += partial t + s) runs on lane shuffles with
+            // no LDS and no block barrier.  Every lane shuffles; lanes t < s combine.
+            P mine = part[0];
+#pragma unroll
+            for (int s = kHeavyPartials / 2; s >= 1; s /= 2) {
+                const P other = shfl_down_pack<T, VB>(mine, (unsigned)s);
+                if (team < (uint32_t)s) acc_pack<T, VB, XOR>(mine, other);
+            }
+            if (team == 0) {
+                if (a.readSelf) {
+                    P self = load_pack<T, VB>(a.rows + (uint64_t)row * a.ld + colByte);
+                    acc_pack<T, VB, XOR>(self, mine);
+                    mine = self;
+                }
+                store_pack<T, VB>(a.rows + (uint64_t)row * a.ld + colByte, mine);
+            }
+        } else {
+            store_pack<T, VB>(lds + ((uint64_t)team * TW + lane) * VB, part[0]);
+        }
     }
+    if constexpr (TW == 1) return;
     __syncthreads();
     // Remaining tree steps across teams through LDS: strides A/2 .. 1.
     for (int s = A / 2; s >= 1; s /= 2) {
@@ -246,8 +280,8 @@ __device__ __forceinline__ void pull_heavy(const PullArgs& a, uint32_t row, uint
 // Narrow teams (TW <= 4: many rows per block, short row slices) are latency bound: the
 // block's rows are consecutive, so their gather lists form one contiguous range of `src`,
 // which is copied to LDS with coalesced loads (when it fits) before the teams walk their
-// lists with 8 (16 for single-lane teams) row loads in flight.  The summation order is the
-// same in every variant.
+// lists with 8 row loads in flight (16, or 32 for 4- and 8-byte slices, for single-lane
+// teams).  The summation order is the same in every variant.
 constexpr uint32_t kStageEntries = 4096;
 
 struct GlobalIdx {
@@ -272,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void pull_level_kernel(PullArgs a) {
     if (bx < a.heavyCount) {
         pull_heavy<T, VB, TW, XOR>(a, a.levelBegin + bx, colByte, active, team, lane);
     } else if constexpr (TW <= 4) {
-        constexpr int UNR = TW == 1 ? 16 : 8;
+        constexpr int UNR = TW == 1 ? (VB <= 8 ? 32 : 16) : 8;
         __shared__ uint32_t staged[kStageEntries];
         const uint32_t r0 = a.levelBegin + a.heavyCount + (bx - a.heavyCount) * NT;
         const uint32_t r1 = r0 + NT < a.levelEnd ? r0 + NT : a.levelEnd;
