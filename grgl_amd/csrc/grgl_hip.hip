@@ -948,7 +948,6 @@ GRGL_EXPORT grgl_status grgl_hip_matmul(grgl_graph* g, grgl_dtype dtype, const v
         s = acquireInternalStream(g);
         ownStream = true;
     }
-    const size_t outCols = output_size / input_rows;
     const size_t rowsBytes = rowsBufferBytes(g, direction, K, es);
     auto align = [](size_t x) { return (x + 255) / 256 * 256; };
     size_t total = align(rowsBytes);
@@ -997,7 +996,6 @@ GRGL_EXPORT grgl_status grgl_hip_matmul(grgl_graph* g, grgl_dtype dtype, const v
         opt.accumulate = accumulate;
         opt.orderPreserving = (flags & GRGL_FLAG_ORDER_PRESERVING) != 0;
         std::vector<TravRequest> reqs{{K, dIn, dOut, 0}};
-        (void)outCols;
         enqueueTraversal(g, dtype, direction, reqs, K, opt, base, s);
         if (host) {
             GRGL_HIP_CHECK(hipMemcpyAsync(output, dOut, outBytes, hipMemcpyDeviceToHost, s));

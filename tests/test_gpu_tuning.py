@@ -1,0 +1,92 @@
+"""Tuning never changes results: every team width, hipGraph replay on/off and both level-kernel
+block orders give bit-identical outputs (the per-column arithmetic depends only on the graph
+and the heavy threshold).  Captured level sequences are replayed several times, with eager
+work in between, and every replay is checked."""
+
+import numpy as np
+import pytest
+
+import oracle
+from grgl_amd import capi
+from helpers import bits_equal, make_input, random_grg
+
+pytestmark = pytest.mark.gpu
+
+_G = {}
+
+
+def _graphs():
+    if not _G:
+        import grgl_amd
+
+        _G["heavy"] = random_grg(31, num_samples=300, num_internal=2500, max_children=9, heavy=(100, 900, 2500),
+                                 num_mutations=2000)
+        _G["tree"] = grgl_amd.grg_from_coalescent(3000, 30000, 3, seed=5).arrays()
+    return _G
+
+
+def _run(cg, dt, x, K, direction, cin, cout, stream):
+    import torch
+
+    y = torch.zeros((K, cout), dtype=x.dtype, device="cuda")
+    cg.matmul_ptr(dt, x.data_ptr(), cin, K, direction, y.data_ptr(), y.numel(), stream=stream)
+    return y
+
+
+@pytest.mark.parametrize("name", ["heavy", "tree"])
+@pytest.mark.parametrize("dtype", [np.float32, np.int32, np.float64])
+def test_every_tuning_gives_the_same_bits(gpu, name, dtype):
+    import torch
+
+    arrays = _graphs()[name]
+    cg = capi.Graph(arrays)
+    og = oracle.OracleGraph(arrays)
+    dt = capi.NUMPY_TO_DTYPE[np.dtype(dtype)]
+    es = np.dtype(dtype).itemsize
+    rng = np.random.default_rng(11)
+    stream = torch.cuda.current_stream().cuda_stream
+    for direction in (capi.UP, capi.DOWN):
+        cin = cg.num_samples if direction == capi.UP else cg.num_mutations
+        cout = cg.num_mutations if direction == capi.UP else cg.num_samples
+        for K in (1, 5, 64, 200, 511):
+            x = torch.from_numpy(make_input(rng, dtype, (K, cin))).cuda()
+            cg.set_tuning(direction, dt, K)  # library default
+            ref = _run(cg, dt, x, K, direction, cin, cout, stream).cpu().numpy()
+            if np.dtype(dtype).kind == "i":
+                assert bits_equal(ref, og.matmul(x.cpu().numpy(), oracle.UP if direction == capi.UP else oracle.DOWN))
+            lanes = max(1, -(-K * es // 16))
+            tws = [1] if K * es <= 8 else [t for t in (1, 2, 4, 8, 16, 32, 64) if t <= max(1, 1 << (lanes - 1).bit_length())]
+            for tw in tws:
+                for use_graph in (0, 1):
+                    for xcd in (1, 2):
+                        cg.set_tuning(direction, dt, K, team_lanes=tw, use_graph=use_graph, xcd_map=xcd)
+                        for rep in range(3 if use_graph else 1):
+                            got = _run(cg, dt, x, K, direction, cin, cout, stream)
+                            torch.cuda.synchronize()
+                            # eager work between replays
+                            _ = (x.float() * 2).sum().item()
+                            assert bits_equal(got.cpu().numpy(), ref), (name, direction, K, tw, use_graph, xcd, rep)
+            cg.set_tuning(direction, dt, K)
+    cg.close()
+
+
+def test_autotune_choice_is_used_and_exact(gpu):
+    import torch
+
+    arrays = _graphs()["tree"]
+    cg = capi.Graph(arrays)
+    og = oracle.OracleGraph(arrays)
+    stream = torch.cuda.current_stream().cuda_stream
+    rng = np.random.default_rng(2)
+    for direction in (capi.UP, capi.DOWN):
+        cin = cg.num_samples if direction == capi.UP else cg.num_mutations
+        cout = cg.num_mutations if direction == capi.UP else cg.num_samples
+        chosen, ms = cg.autotune(direction, capi.I32, 300)
+        assert chosen["team_lanes"] in (1, 2, 4, 8, 16, 32, 64) and ms > 0
+        assert chosen["xcd_map"] in (1, 2) and chosen["use_graph"] in (0, 1)
+        x = torch.from_numpy(make_input(rng, np.int32, (300, cin))).cuda()
+        got = _run(cg, capi.I32, x, 300, direction, cin, cout, stream).cpu().numpy()
+        assert bits_equal(got, og.matmul(x.cpu().numpy(), oracle.UP if direction == capi.UP else oracle.DOWN))
+    with pytest.raises(RuntimeError):
+        cg.set_tuning(capi.UP, capi.I32, 8, xcd_map=7)
+    cg.close()
