@@ -312,6 +312,76 @@ def serve_benchmark(num_requests=2000, load=0.6):
     }
 
 
+def options_benchmark(cg, N, M, V, E, P=2, K=16):
+    """The remaining matmul options on the benchmark graph, f32, K = 16, one request per call:
+    by_individual (UP reads K x N/P, DOWN writes K x N/P) and emit_all_nodes (K x V out)."""
+    import torch
+
+    from grgl_amd import capi
+
+    sp = torch.cuda.current_stream().cuda_stream
+    cases = [("by_individual_UP", capi.UP, N // P, M, dict(by_individual=True)),
+             ("by_individual_DOWN", capi.DOWN, M, N // P, dict(by_individual=True)),
+             ("emit_all_nodes_UP", capi.UP, N, V, dict(emit_all=True)),
+             ("emit_all_nodes_DOWN", capi.DOWN, M, V, dict(emit_all=True))]
+    out = {}
+    for name, d, cin, cout, kw in cases:
+        x = torch.randn((K, cin), device="cuda")
+        y = torch.zeros((K, cout), device="cuda")
+        for _ in range(2):
+            cg.matmul_ptr(capi.F32, x.data_ptr(), cin, K, d, y.data_ptr(), y.numel(), stream=sp, **kw)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(5):
+            cg.matmul_ptr(capi.F32, x.data_ptr(), cin, K, d, y.data_ptr(), y.numel(), stream=sp, **kw)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) / 5 * 1e3
+        out[name] = {"ms": round(ms, 4), "edge_rhs_per_s": E * K / (ms / 1e3)}
+    return out
+
+
+def grg_io_benchmark(g):
+    """Write and read the benchmark graph as a .grg file (format 5.3) in a temporary directory."""
+    import tempfile
+
+    import grgl_amd
+
+    with tempfile.TemporaryDirectory() as tmp:
+        fn = os.path.join(tmp, "bench.grg")
+        t = time.perf_counter()
+        grgl_amd.save_grg(g, fn)
+        w = time.perf_counter() - t
+        size = os.path.getsize(fn)
+        t = time.perf_counter()
+        h = grgl_amd.load_immutable_grg(fn)
+        r = time.perf_counter() - t
+        same = (h.num_nodes, h.num_edges, h.num_mutations) == (g.num_nodes, g.num_edges, g.num_mutations)
+    return {"file_bytes": size, "write_s": round(w, 3), "read_s": round(r, 3),
+            "read_MBps": round(size / r / 1e6, 1), "round_trip_equal": bool(same)}
+
+
+def construction_benchmark():
+    """map_mutations (smallest carrier sets first, one mutation per batch) on the carrier lists
+    of the reference's test VCF (400 haplotypes x 10,893 sites), on one host core."""
+    import numpy as np
+
+    import grgl_amd
+
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "test200.npz"))
+    off, car = fx["carrier_off"], fx["carriers"]
+    sets = [car[off[m]:off[m + 1]].astype(np.uint32).tolist() for m in range(len(off) - 1)]
+    muts = [grgl_amd.Mutation(float(p), "A", "G") for p in fx["positions"]]
+    g = grgl_amd.MutableGRG(int(fx["num_samples"]), int(fx["ploidy"]))
+    t = time.perf_counter()
+    grgl_amd.map_mutations(g, muts, sets, mutation_batch_size=1, size_order=True)
+    dt = time.perf_counter() - t
+    entries = int(sum(len(x) for x in sets))
+    return {"workload": "test-200-samples VCF carriers: 400 haplotypes x 10,893 sites",
+            "seconds": round(dt, 3), "mutations_per_s": round(len(sets) / dt, 1),
+            "nodes": g.num_nodes, "edges": g.num_edges, "carrier_entries": entries,
+            "edges_per_carrier": round(g.num_edges / entries, 4)}
+
+
 def plain_at_scale():
     """One request per traversal on 100,000 haplotypes x 500,000 sites (20 trees):
     K = 1 and 16, FP64 and FP32, UP and DOWN (ms per traversal, edge*RHS/s, algorithmic GB/s)."""
@@ -403,6 +473,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving (mixed request stream) scenario")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--no-pca", action="store_true", help="skip the standardized PCA scenario")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the option, .grg file and construction measurements")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -557,6 +629,12 @@ def main():
         log("PCA scenario ...")
         pca = pca_benchmark()
         log(f"pca: {json.dumps(pca)}")
+    extras = None
+    if not args.no_extras:
+        log("options, .grg file and construction ...")
+        extras = {"options_K16_f32": options_benchmark(cg, N, M, V, E), "grg_io": grg_io_benchmark(g),
+                  "construction": construction_benchmark()}
+        log(f"extras: {json.dumps(extras)}")
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU baseline ...")
@@ -594,6 +672,7 @@ def main():
         "plain_100k_x_500k": at_scale,
         "serving": serving,
         "pca": pca,
+        "extras": extras,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)
