@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -107,8 +108,8 @@ struct grgl_graph {
     std::vector<std::unique_ptr<grgl_amd::Workspace>> workspaces; // stable addresses
     std::vector<hipStream_t> freeStreams;
     std::map<std::tuple<int, int, uint32_t>, grgl_amd::Tuning> tuning; // (dir, typeClass, K)
-    // Kernel timing (level kernels).
-    bool timing = false;
+    // Kernel timing (level kernels); read by traversals on any thread.
+    std::atomic<bool> timing{false};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> timingEvents;
     std::vector<double> timingBytes;
     double timedMs = 0.0;

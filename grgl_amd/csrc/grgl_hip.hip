@@ -422,7 +422,6 @@ Tuning chooseTuning(grgl_graph* g, grgl_direction dir, TypeClass tc, uint32_t K,
     return t;
 }
 
-// Launch every pulled level of one direction.
 // First level of the trailing run that one block handles (numLevels: none).  A level
 // qualifies when its light rows take at most two passes of the block's teams, it has at
 // most two heavy rows, and its rows and gathered rows come to at most 64 KB per column
@@ -441,6 +440,7 @@ uint32_t tailStart(const DeviceDirPlan& plan, const RowGeometry& geo, uint32_t t
     return plan.numLevels - l >= 2 ? l : plan.numLevels;
 }
 
+// Launch every pulled level of one direction (the trailing small levels as one tail launch).
 void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const RowGeometry& geo, uint32_t tw,
                    bool xcdMap, bool readSelf, bool orderPreserving, uint32_t K, size_t elemSize, char* rows, hipStream_t stream) {
     const uint32_t tiles = (geo.lanes + tw - 1) / tw;
@@ -479,6 +479,7 @@ void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const
             for (uint32_t r = begin; r < end; r++) ent += plan.fanIn[r];
             const double rowB = (double)K * (double)elemSize;
             const double bytes = 4.0 * (double)(ent + nodes + 1) + rowB * (double)(ent + nodes + (readSelf ? nodes : 0));
+            std::lock_guard<std::mutex> lk(g->mu);
             g->timingEvents.push_back({ev0, ev1});
             g->timingBytes.push_back(bytes);
         }
@@ -1124,19 +1125,30 @@ GRGL_EXPORT grgl_status grgl_hip_kernel_timing_read(grgl_graph* g, double* total
     GRGL_API_BEGIN
     if (!g) throw MisuseError("null graph");
     GRGL_HIP_CHECK(hipSetDevice(g->device));
-    for (size_t i = 0; i < g->timingEvents.size(); i++) {
-        auto& ev = g->timingEvents[i];
-        GRGL_HIP_CHECK(hipEventSynchronize(ev.second));
-        float ms = 0.0f;
-        GRGL_HIP_CHECK(hipEventElapsedTime(&ms, ev.first, ev.second));
-        g->timedMs += ms;
-        g->timedLaunches++;
-        g->timedBytes += g->timingBytes[i];
-        (void)hipEventDestroy(ev.first);
-        (void)hipEventDestroy(ev.second);
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    std::vector<double> bytes;
+    {
+        std::lock_guard<std::mutex> lk(g->mu);
+        events.swap(g->timingEvents);
+        bytes.swap(g->timingBytes);
     }
-    g->timingEvents.clear();
-    g->timingBytes.clear();
+    double ms = 0.0, moved = 0.0;
+    hipError_t firstError = hipSuccess;
+    for (size_t i = 0; i < events.size(); i++) {
+        const hipError_t e = hipEventSynchronize(events[i].second);
+        float one = 0.0f;
+        if (e == hipSuccess) (void)hipEventElapsedTime(&one, events[i].first, events[i].second);
+        else if (firstError == hipSuccess) firstError = e;
+        ms += one;
+        moved += bytes[i];
+        (void)hipEventDestroy(events[i].first);
+        (void)hipEventDestroy(events[i].second);
+    }
+    GRGL_HIP_CHECK(firstError);
+    std::lock_guard<std::mutex> lk(g->mu);
+    g->timedMs += ms;
+    g->timedLaunches += events.size();
+    g->timedBytes += moved;
     if (total_ms) *total_ms = g->timedMs;
     if (launches) *launches = g->timedLaunches;
     if (algorithmic_bytes) *algorithmic_bytes = g->timedBytes;
@@ -1158,8 +1170,7 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
     const size_t es = dtypeSize(dtype);
     const RowGeometry geo = rowGeometry(rows, es);
     GRGL_HIP_CHECK(hipSetDevice(g->device));
-    const bool savedTiming = g->timing;
-    g->timing = false;
+    const bool savedTiming = g->timing.exchange(false);
     hipStream_t s = acquireInternalStream(g);
     Workspace* ws = acquireWorkspace(g, rowsBufferBytes(g, direction, rows, es), s);
     std::vector<uint32_t> tws;

@@ -145,7 +145,10 @@ grgl_status grgl_hip_matmul_batch(grgl_graph* graph, grgl_dtype dtype, grgl_dire
                                   const grgl_request* requests, uint32_t num_requests, uint32_t flags,
                                   void* stream);
 
-/* Tuning of the level kernels for (graph, direction, dtype, rows).  0 = library default. */
+/* Tuning of the level kernels for (graph, direction, dtype, rows).  0 = library default.
+ * Team width, hipGraph replay and block order only change speed.  heavy_fanin changes which
+ * gather lists are split 64 ways (float rounding of those rows) for every later call in that
+ * direction; set it before traversals start, not while others run on the graph. */
 typedef struct grgl_tuning {
     uint32_t team_lanes;     /* lanes that share one node's row slice: 1..64, power of two */
     uint32_t heavy_fanin;    /* gather lists longer than this are split over a whole block */
