@@ -81,7 +81,16 @@ struct Server {
     std::vector<double> latencies;
     bool stop = false;
     std::thread worker;
+    // Batches completed on the device, counted by a stream host function so the scheduler
+    // wakes as soon as a batch ends instead of polling.  The host function takes no lock and
+    // makes no HIP call (the scheduler may hold `mu` inside a HIP call that waits for it).
+    std::atomic<uint64_t> completions{0};
 
+    static void onBatchDone(void* self) {
+        Server* srv = static_cast<Server*>(self);
+        srv->completions.fetch_add(1);
+        srv->cvWork.notify_all();
+    }
     void run();
     bool pollSlots();
     bool launchBatch(Slot& s, std::unique_lock<std::mutex>& lk);
@@ -262,6 +271,7 @@ bool Server::launchBatch(Slot& s, std::unique_lock<std::mutex>& lk) {
             GRGL_HIP_CHECK(hipMemcpyAsync(s.pinned + alignUp(K * inCols * es), devOut, (size_t)K * outCols * es,
                                           hipMemcpyDeviceToHost, s.stream));
         GRGL_HIP_CHECK(hipEventRecord(s.done, s.stream));
+        GRGL_HIP_CHECK(hipLaunchHostFunc(s.stream, &Server::onBatchDone, this));
     } catch (const std::exception& e) {
         // Wait for whatever was enqueued, then fail the batch.
         (void)hipStreamSynchronize(s.stream);
@@ -282,6 +292,7 @@ void Server::run() {
     (void)hipSetDevice(cfg.device);
     std::unique_lock<std::mutex> lk(mu);
     while (true) {
+        const uint64_t seenCompletions = completions.load();
         bool progressed = pollSlots();
         if (stop) {
             bool busy = false;
@@ -307,8 +318,12 @@ void Server::run() {
             if (windowOpen) {
                 cvWork.wait_until(lk, windowEnd);
             } else if (busy) {
-                // Batches in flight: check again shortly (event completion is polled).
-                cvWork.wait_for(lk, std::chrono::microseconds(50));
+                // Batches in flight: sleep until one completes (onBatchDone) or new work
+                // arrives; the timeout only bounds a wake-up lost between the check and the wait.
+                const size_t seenQueue = queue.size();
+                cvWork.wait_for(lk, std::chrono::microseconds(500), [&] {
+                    return stop || completions.load() != seenCompletions || queue.size() != seenQueue;
+                });
             } else {
                 cvWork.wait(lk, [&] { return stop || !queue.empty(); });
             }
