@@ -119,12 +119,22 @@ def cpu_baseline(arrays, E, budget_s=8.0):
         counts = list(ex.map(worker, [t0 + budget_s] * threads))
     dtn = time.perf_counter() - t0
     done = sum(counts)
+    # Single-thread latency per call at K = 1, 16 and 256 (UP and DOWN), one call each after
+    # a warm-up for K = 256 (about a second per call), a few for the smaller K.
+    latency = {}
+    for KL, reps in ((1, 5), (16, 2), (256, 1)):
+        vl = rng.standard_normal((KL, og.num_samples)).astype(np.float32)
+        wl = rng.standard_normal((KL, og.num_mutations)).astype(np.float32)
+        latency[f"K{KL}"] = {"UP_ms": round(_cpu_ms(og, kind, vl, oracle.UP, reps), 2),
+                             "DOWN_ms": round(_cpu_ms(og, kind, wl, oracle.DOWN, reps), 2)}
     return {
         "value": E * K * done / dtn,
         "unit": "edge\u00b7RHS/s",
         "cores": threads,
         "kind": kind,
         "single_thread_value": single,
+        "single_thread_latency": latency,
+        "host_cpus": os.cpu_count(),
         "cpu_model": _cpu_model(),
         "sample": f"f32 K=16 traversals (UP and DOWN alternating) of the benchmark graph: {done} calls on "
                   f"{threads} threads in {dtn:.1f} s (independent calls, shared graph); single thread: "
@@ -382,16 +392,39 @@ def construction_benchmark():
             "edges_per_carrier": round(g.num_edges / entries, 4)}
 
 
-def plain_at_scale():
+def _cpu_ms(og, kind, x, direction, reps):
+    """Single-thread milliseconds per call of the CPU matmul (the reference's own, oracle/_ref,
+    or the oracle's faithful port)."""
+    run = og.ref_matmul if kind == "reference" else og.faithful_matmul
+    if reps > 1:
+        run(x, direction)  # warm-up (page-in); a single long call is timed as it comes
+    t = time.perf_counter()
+    for _ in range(reps):
+        run(x, direction)
+    return (time.perf_counter() - t) / reps * 1e3
+
+
+def plain_at_scale(with_cpu=True):
     """One request per traversal on 100,000 haplotypes x 500,000 sites (20 trees):
-    K = 1 and 16, FP64 and FP32, UP and DOWN (ms per traversal, edge*RHS/s, algorithmic GB/s)."""
+    K = 1 and 16, FP64 and FP32, UP and DOWN (ms per traversal, edge*RHS/s, algorithmic GB/s),
+    with the CPU matmul's single-thread milliseconds per call on the same input beside it."""
+    import numpy as np
     import torch
 
     import grgl_amd
     from grgl_amd import capi
 
     g = grgl_amd.grg_from_coalescent(100_000, 500_000, 20, seed=SEED)
-    cg = capi.Graph(g.arrays())
+    arrays = g.arrays()
+    cg = capi.Graph(arrays)
+    og = kind = None
+    if with_cpu:
+        import oracle
+
+        og = oracle.OracleGraph(arrays)
+        kind = "reference" if oracle.ref_available() else "port"
+        if kind == "reference":
+            og.ref_handle()
     info = cg.info()
     V, E, N, M = info["num_nodes"], info["num_edges"], info["num_samples"], info["num_mutations"]
     sp = torch.cuda.current_stream().cuda_stream
@@ -411,8 +444,15 @@ def plain_at_scale():
                     cg.matmul_ptr(cdt, x.data_ptr(), cin, K, d, y.data_ptr(), y.numel(), stream=sp)
                 torch.cuda.synchronize()
                 ms = (time.perf_counter() - t) / reps * 1e3
-                out[f"{name}_{dn}_K{K}"] = {"ms": round(ms, 4), "edge_rhs_per_s": E * K / (ms / 1e3),
-                                            "alg_GBps": round(algorithmic_bytes(V, E, K, es, cin, cout) / (ms / 1e3) / 1e9, 1)}
+                rec = {"ms": round(ms, 4), "edge_rhs_per_s": E * K / (ms / 1e3),
+                       "alg_GBps": round(algorithmic_bytes(V, E, K, es, cin, cout) / (ms / 1e3) / 1e9, 1)}
+                if og is not None:
+                    xh = np.ascontiguousarray(x.cpu().numpy())
+                    od = oracle.UP if d == capi.UP else oracle.DOWN
+                    rec["cpu_1thread_ms"] = round(_cpu_ms(og, kind, xh, od, 2), 2)
+                out[f"{name}_{dn}_K{K}"] = rec
+    if og is not None:
+        out["cpu_kind"] = kind
     cg.close()
     return out
 
@@ -623,7 +663,7 @@ def main():
     at_scale = None
     if not args.no_plain:
         log("plain path at scale (100k x 500k) ...")
-        at_scale = plain_at_scale()
+        at_scale = plain_at_scale(with_cpu=not args.no_cpu_baseline)
     pca = None
     if not args.no_pca:
         log("PCA scenario ...")
