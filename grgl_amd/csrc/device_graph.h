@@ -54,6 +54,24 @@ struct RowGeometry {
     uint32_t lanes;     // rowBytes / vb
 };
 RowGeometry rowGeometry(uint32_t K, size_t elemSize);
+// Team widths (lanes per row) of the level kernels: powers of two up to the number of
+// 16-byte slices of a row rounded up, or up to 8 for rows of at most eight slices.
+uint32_t maxTeamLanes(const RowGeometry& geo);
+// Bytes each lane of a team of `tw` lanes accesses in the level kernels.  A team of up to 8
+// lanes takes the narrowest access (4, 8 or 16 bytes, never less than one element) with
+// which it covers the whole row at once, so that the row is one column slice; lanes left
+// over idle, which spreads a level's rows over more wavefronts and compute units.  Rows
+// too wide for that, and teams of 16 lanes or more, take 16-byte accesses.
+inline uint32_t levelVb(const RowGeometry& geo, uint32_t tw, size_t elemSize) {
+    if (tw > 8) return geo.vb;
+    for (uint32_t v = elemSize > 4 ? (uint32_t)elemSize : 4; v < geo.vb; v *= 2)
+        if (geo.rowBytes / v <= tw) return v;
+    return geo.vb;
+}
+// Column slices (grid y) a row is cut into for a team of `tw` lanes.
+inline uint32_t levelTiles(const RowGeometry& geo, uint32_t tw, size_t elemSize) {
+    return (geo.rowBytes / levelVb(geo, tw, elemSize) + tw - 1) / tw;
+}
 
 struct DeviceDirPlan {
     uint32_t numNodes = 0, numLevels = 0, firstPulledLevel = 0, maxFanIn = 0;

@@ -81,10 +81,16 @@ RowGeometry rowGeometry(uint32_t K, size_t elemSize) {
     if (rb > 0xffffffffULL) throw MisuseError("too many rows (K) for one traversal");
     r.rowBytes = (uint32_t)rb;
     // Row starts aligned so that the 256-byte slices a team reads never straddle an extra
-    // cache line: 256-byte alignment for wide rows, 64 bytes for medium ones.
-    r.ld = rb >= 256 ? (rb + 255) / 256 * 256 : (rb >= 64 ? (rb + 63) / 64 * 64 : rb);
+    // cache line: 256-byte alignment for wide rows, 64 bytes for medium ones; rows under
+    // 64 bytes (4, 8, 16, 32 or 48 bytes) get a power-of-two stride, so none straddles a
+    // 64-byte boundary (48-byte rows at a 48-byte stride did, and ran 1.5x slower).
+    r.ld = rb >= 256 ? (rb + 255) / 256 * 256 : (rb >= 64 ? (rb + 63) / 64 * 64 : pow2ceil((uint32_t)rb));
     r.lanes = r.rowBytes / r.vb;
     return r;
+}
+
+uint32_t maxTeamLanes(const RowGeometry& geo) {
+    return std::min<uint32_t>(64, std::max<uint32_t>(8, pow2ceil(geo.lanes)));
 }
 
 void DeviceDirPlan::computeHeavy() {
@@ -140,12 +146,23 @@ void validateShapes(const grgl_graph* g, grgl_direction dir, size_t inCols, size
 // ---------------------------------------------------------------------------------------
 namespace {
 
+// Accesses narrower than 16 bytes come with teams of at most 8 lanes (levelVb).
 template <typename T, bool XOR> void launchPullT(uint32_t vb, uint32_t tw, dim3 grid, const PullArgs& a, hipStream_t s) {
 #define GRGL_PULL(VB, TW) hipLaunchKernelGGL((pull_level_kernel<T, VB, TW, XOR>), grid, dim3(kBlock), 0, s, a)
     if (vb == 4) {
-        if constexpr (sizeof(T) <= 4) GRGL_PULL(4, 1);
+        if constexpr (sizeof(T) <= 4) {
+            if (tw == 1) GRGL_PULL(4, 1);
+            else if (tw == 2) GRGL_PULL(4, 2);
+            else if (tw == 4) GRGL_PULL(4, 4);
+            else GRGL_PULL(4, 8);
+        }
     } else if (vb == 8) {
-        if constexpr (sizeof(T) <= 8) GRGL_PULL(8, 1);
+        if constexpr (sizeof(T) <= 8) {
+            if (tw == 1) GRGL_PULL(8, 1);
+            else if (tw == 2) GRGL_PULL(8, 2);
+            else if (tw == 4) GRGL_PULL(8, 4);
+            else GRGL_PULL(8, 8);
+        }
     } else {
         switch (tw) {
         case 1: GRGL_PULL(16, 1); break;
@@ -163,9 +180,19 @@ template <typename T, bool XOR> void launchPullT(uint32_t vb, uint32_t tw, dim3 
 template <typename T, bool XOR> void launchTailT(uint32_t vb, uint32_t tw, const TailArgs& a, hipStream_t s) {
 #define GRGL_TAIL(VB, TW) hipLaunchKernelGGL((pull_tail_kernel<T, VB, TW, XOR>), dim3(1), dim3(kBlock), 0, s, a)
     if (vb == 4) {
-        if constexpr (sizeof(T) <= 4) GRGL_TAIL(4, 1);
+        if constexpr (sizeof(T) <= 4) {
+            if (tw == 1) GRGL_TAIL(4, 1);
+            else if (tw == 2) GRGL_TAIL(4, 2);
+            else if (tw == 4) GRGL_TAIL(4, 4);
+            else GRGL_TAIL(4, 8);
+        }
     } else if (vb == 8) {
-        if constexpr (sizeof(T) <= 8) GRGL_TAIL(8, 1);
+        if constexpr (sizeof(T) <= 8) {
+            if (tw == 1) GRGL_TAIL(8, 1);
+            else if (tw == 2) GRGL_TAIL(8, 2);
+            else if (tw == 4) GRGL_TAIL(8, 4);
+            else GRGL_TAIL(8, 8);
+        }
     } else {
         switch (tw) {
         case 1: GRGL_TAIL(16, 1); break;
@@ -410,15 +437,14 @@ Tuning chooseTuning(grgl_graph* g, grgl_direction dir, TypeClass tc, uint32_t K,
         if (it != g->tuning.end()) t = it->second;
     }
     uint32_t tw = t.teamLanes;
-    if (geo.vb != 16) {
-        tw = 1;
-    } else {
-        // Default: 16 lanes (256-byte row slices) at most.  Blocks are dispatched K-tile
-        // by K-tile, so narrower slices keep each tile's gathered rows cache resident.
-        if (tw == 0) tw = std::min<uint32_t>(16, pow2ceil(geo.lanes));
-        tw = std::min<uint32_t>(64, pow2ceil(tw));
-    }
-    t.teamLanes = tw;
+    // Default: 16 lanes (256-byte row slices) at most.  Blocks are dispatched K-tile by
+    // K-tile, so narrower slices keep each tile's gathered rows cache resident.  Rows of
+    // one or two slices take 8 lanes (4 on rows of at most 8 bytes): narrower accesses,
+    // then idle lanes, put fewer rows in each wavefront, so a level's rows reach more
+    // compute units and more of them are in flight at once (latency-bound small K: on the
+    // 200k x 1M graph FP32 DOWN K = 1 goes from 0.35 to 0.22 ms, K = 8 from 0.36 to 0.25 ms).
+    if (tw == 0) tw = geo.lanes >= 3 ? std::min<uint32_t>(16, pow2ceil(geo.lanes)) : (geo.rowBytes <= 8 ? 4 : 8);
+    t.teamLanes = std::min<uint32_t>(maxTeamLanes(geo), pow2ceil(tw));
     return t;
 }
 
@@ -443,7 +469,8 @@ uint32_t tailStart(const DeviceDirPlan& plan, const RowGeometry& geo, uint32_t t
 // Launch every pulled level of one direction (the trailing small levels as one tail launch).
 void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const RowGeometry& geo, uint32_t tw,
                    bool xcdMap, bool readSelf, bool orderPreserving, uint32_t K, size_t elemSize, char* rows, hipStream_t stream) {
-    const uint32_t tiles = (geo.lanes + tw - 1) / tw;
+    const uint32_t tiles = levelTiles(geo, tw, elemSize);
+    const uint32_t vb = levelVb(geo, tw, elemSize);
     const uint32_t teamsPerBlock = kBlock / tw;
     const uint32_t tailFrom = g->timing ? plan.numLevels : tailStart(plan, geo, tw, orderPreserving);
     for (uint32_t l = plan.firstPulledLevel; l < tailFrom; l++) {
@@ -458,10 +485,13 @@ void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const
         a.src = plan.src;
         a.levelBegin = begin;
         a.heavyCount = heavy;
+        // Rows of one access (tiles == 1): a wavefront per heavy row, 4 per block.
+        a.heavyPerWave = geo.rowBytes <= 16 ? 1 : 0;
+        a.heavyBlocks = a.heavyPerWave ? (heavy + kBlock / 64 - 1) / (kBlock / 64) : heavy;
         a.levelEnd = end;
         a.rowBytes = geo.rowBytes;
         a.readSelf = readSelf ? 1 : 0;
-        const uint32_t bxCount = heavy + (light + teamsPerBlock - 1) / teamsPerBlock;
+        const uint32_t bxCount = a.heavyBlocks + (light + teamsPerBlock - 1) / teamsPerBlock;
         a.xcdTiles = (xcdMap && tiles % 8 == 0) ? tiles : 0;
         dim3 grid = a.xcdTiles ? dim3(bxCount * tiles, 1) : dim3(bxCount, tiles);
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -470,7 +500,7 @@ void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const
             GRGL_HIP_CHECK(hipEventCreate(&ev1));
             GRGL_HIP_CHECK(hipEventRecord(ev0, stream));
         }
-        launchPull(tc, geo.vb, tw, grid, a, stream);
+        launchPull(tc, vb, tw, grid, a, stream);
         GRGL_HIP_CHECK(hipGetLastError());
         if (g->timing) {
             GRGL_HIP_CHECK(hipEventRecord(ev1, stream));
@@ -497,7 +527,7 @@ void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const
         t.heavyCount = plan.dHeavyCount + tailFrom;
         t.numLevels = plan.numLevels - tailFrom;
         t.noHeavy = orderPreserving ? 1 : 0;
-        launchTail(tc, geo.vb, tw, t, stream);
+        launchTail(tc, vb, tw, t, stream);
         GRGL_HIP_CHECK(hipGetLastError());
     }
 }
@@ -1174,12 +1204,8 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
     hipStream_t s = acquireInternalStream(g);
     Workspace* ws = acquireWorkspace(g, rowsBufferBytes(g, direction, rows, es), s);
     std::vector<uint32_t> tws;
-    if (geo.vb != 16) {
-        tws.push_back(1);
-    } else {
-        for (uint32_t tw = 1; tw <= std::min<uint32_t>(64, pow2ceil(geo.lanes)); tw *= 2) {
-            if ((geo.lanes + tw - 1) / tw <= 65535) tws.push_back(tw);
-        }
+    for (uint32_t tw = 1; tw <= maxTeamLanes(geo); tw *= 2) {
+        if (levelTiles(geo, tw, es) <= 65535) tws.push_back(tw);
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     Tuning best;
@@ -1191,7 +1217,7 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
         for (uint32_t trialIdx = 0; trialIdx < 2 * tws.size() * 2; trialIdx++) {
             const uint32_t useGraph = trialIdx % 2, xcd = 1 + (trialIdx / 2) % 2;
             const uint32_t tw = tws[trialIdx / 4];
-            if (xcd == 2 && ((geo.lanes + tw - 1) / tw) % 8 != 0) continue; // same launch as xcd = 1
+            if (xcd == 2 && levelTiles(geo, tw, es) % 8 != 0) continue; // same launch as xcd = 1
             {
                 Tuning t;
                 t.teamLanes = tw;

@@ -112,7 +112,11 @@ struct PullArgs {
     const uint32_t* off;
     const uint32_t* src;
     uint32_t levelBegin;  // first row of the level
-    uint32_t heavyCount;  // rows at the front of the level handled one per block
+    uint32_t heavyCount;  // heavy rows at the front of the level
+    uint32_t heavyBlocks; // blocks on heavy rows: heavyCount, or heavyCount / 4 rounded up
+                          // when each wavefront takes a heavy row of its own (heavyPerWave)
+    uint32_t heavyPerWave; // 1: rows of one access (at most 16 bytes, one column slice);
+                           // a wavefront's 64 lanes are a heavy row's 64 partials
     uint32_t levelEnd;
     uint32_t rowBytes;    // bytes of each row holding values (multiple of VB)
     uint32_t readSelf;    // 1: start from the row's current value (node initialisation)
@@ -277,11 +281,13 @@ __device__ __forceinline__ void pull_heavy(const PullArgs& a, uint32_t row, uint
     }
 }
 
-// Narrow teams (TW <= 4: many rows per block, short row slices) are latency bound: the
+// Narrow teams (TW <= 4, or accesses under 16 bytes: many rows per block, short row
+// slices) are latency bound: the
 // block's rows are consecutive, so their gather lists form one contiguous range of `src`,
 // which is copied to LDS with coalesced loads (when it fits) before the teams walk their
-// lists with 8 row loads in flight (16, or 32 for 4- and 8-byte slices, for single-lane
-// teams).  The summation order is the same in every variant.
+// lists with 8 row loads in flight for 16-byte slices and 16 for narrower ones (16, or 32
+// for 4- and 8-byte slices, for single-lane teams).  The summation order is the same in
+// every variant.
 constexpr uint32_t kStageEntries = 4096;
 
 struct GlobalIdx {
@@ -303,24 +309,53 @@ __global__ __launch_bounds__(kBlock) void pull_level_kernel(PullArgs a) {
     const uint64_t colByte = ((uint64_t)by * TW + lane) * VB;
     const bool active = colByte < a.rowBytes;
     constexpr uint32_t NT = kBlock / TW;
-    if (bx < a.heavyCount) {
-        pull_heavy<T, VB, TW, XOR>(a, a.levelBegin + bx, colByte, active, team, lane);
-    } else if constexpr (TW <= 4) {
-        constexpr int UNR = TW == 1 ? (VB <= 8 ? 32 : 16) : 8;
+    if (bx < a.heavyBlocks) {
+        if (a.heavyPerWave) {
+            // Short rows: one wavefront per heavy row, each lane one partial holding the
+            // whole row (the single-lane-team arithmetic: same partials, same tree), so a
+            // block takes 4 heavy rows and no lane idles.
+            const uint32_t h = bx * (kBlock / 64) + threadIdx.x / 64;
+            if (h >= a.heavyCount) return;
+            const uint32_t row = a.levelBegin + h, part = threadIdx.x % 64;
+            if (a.rowBytes == 4) {
+                if constexpr (sizeof(T) <= 4) pull_heavy<T, 4, 1, XOR>(a, row, 0, true, part, 0);
+            } else if (a.rowBytes == 8) {
+                if constexpr (sizeof(T) <= 8) pull_heavy<T, 8, 1, XOR>(a, row, 0, true, part, 0);
+            } else {
+                pull_heavy<T, 16, 1, XOR>(a, row, 0, true, part, 0);
+            }
+        } else {
+            pull_heavy<T, VB, TW, XOR>(a, a.levelBegin + bx, colByte, active, team, lane);
+        }
+    } else if constexpr (TW <= 4 || VB < 16) {
+        constexpr int UNR = TW == 1 ? (VB <= 8 ? 32 : 16) : (VB == 16 ? 8 : 16);
         __shared__ uint32_t staged[kStageEntries];
-        const uint32_t r0 = a.levelBegin + a.heavyCount + (bx - a.heavyCount) * NT;
+        const uint32_t r0 = a.levelBegin + a.heavyCount + (bx - a.heavyBlocks) * NT;
         const uint32_t r1 = r0 + NT < a.levelEnd ? r0 + NT : a.levelEnd;
         const uint32_t e0 = a.off[r0], n = a.off[r1] - e0;
         const uint32_t row = r0 + team;
         if (n <= kStageEntries) { // block-uniform
-            for (uint32_t t = threadIdx.x; t < n; t += kBlock) staged[t] = a.src[e0 + t];
+            // Every index load of a thread is issued before its first LDS store, so the
+            // copy is one round trip whatever n is (a load-store loop costs one per pass).
+            constexpr int SU = (int)(kStageEntries / kBlock);
+            uint32_t v[SU];
+#pragma unroll
+            for (int u = 0; u < SU; u++) {
+                const uint32_t t = threadIdx.x + (uint32_t)u * kBlock;
+                if (t < n) v[u] = a.src[e0 + t];
+            }
+#pragma unroll
+            for (int u = 0; u < SU; u++) {
+                const uint32_t t = threadIdx.x + (uint32_t)u * kBlock;
+                if (t < n) staged[t] = v[u];
+            }
             __syncthreads();
             if (row < r1) pull_light<T, VB, TW, XOR, UNR>(a, row, colByte, active, StagedIdx{staged, e0});
         } else if (row < r1) {
             pull_light<T, VB, TW, XOR, UNR>(a, row, colByte, active, GlobalIdx{a.src});
         }
     } else {
-        const uint32_t row = a.levelBegin + a.heavyCount + (bx - a.heavyCount) * NT + team;
+        const uint32_t row = a.levelBegin + a.heavyCount + (bx - a.heavyBlocks) * NT + team;
         if (row < a.levelEnd) pull_light<T, VB, TW, XOR, 4>(a, row, colByte, active, GlobalIdx{a.src});
     }
 }
@@ -357,6 +392,8 @@ __global__ __launch_bounds__(kBlock) void pull_tail_kernel(TailArgs t) {
         a.src = t.src;
         a.levelBegin = t.levelStart[l];
         a.heavyCount = t.noHeavy ? 0 : t.heavyCount[l];
+        a.heavyBlocks = a.heavyCount;
+        a.heavyPerWave = 0;
         a.levelEnd = t.levelStart[l + 1];
         a.rowBytes = t.rowBytes;
         a.readSelf = t.readSelf;

@@ -150,8 +150,11 @@ grgl_status grgl_hip_matmul_batch(grgl_graph* graph, grgl_dtype dtype, grgl_dire
  * gather lists are split 64 ways (float rounding of those rows) for every later call in that
  * direction; set it before traversals start, not while others run on the graph. */
 typedef struct grgl_tuning {
-    uint32_t team_lanes;     /* lanes that share one node's row slice: 1..64, power of two */
-    uint32_t heavy_fanin;    /* gather lists longer than this are split over a whole block */
+    uint32_t team_lanes;     /* lanes that share one node's row slice: a power of two up to the
+                                row's 16-byte slices rounded up, or up to 8 for short rows (which
+                                then take 8- or 4-byte accesses and idle lanes); larger values are
+                                clamped */
+    uint32_t heavy_fanin;    /* gather lists longer than this are split into 64 partials */
     uint32_t use_graph;      /* 1 = replay the level sequence as a captured hipGraph */
     uint32_t xcd_map;        /* block order of the level kernels when the row splits into a multiple
                                 of 8 column slices: 0 = default (plain 2-D grid), 1 = XCD-aware (XCD i

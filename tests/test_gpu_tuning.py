@@ -25,6 +25,16 @@ def _graphs():
     return _G
 
 
+def _team_widths(K, es):
+    """Every team width the library accepts for K columns of es bytes: powers of two up to
+    the row's 16-byte slices rounded up, and at least up to 8 (narrower accesses, then idle
+    lanes, on short rows)."""
+    raw = K * es
+    lanes = -(-raw // 16) if raw > 8 else 1
+    top = min(64, max(8, 1 << (lanes - 1).bit_length()))
+    return [t for t in (1, 2, 4, 8, 16, 32, 64) if t <= top]
+
+
 def _run(cg, dt, x, K, direction, cin, cout, stream):
     import torch
 
@@ -34,7 +44,7 @@ def _run(cg, dt, x, K, direction, cin, cout, stream):
 
 
 @pytest.mark.parametrize("name", ["heavy", "tree"])
-@pytest.mark.parametrize("dtype", [np.float32, np.int32, np.float64])
+@pytest.mark.parametrize("dtype", [np.float32, np.int32, np.float64, np.int8])
 def test_every_tuning_gives_the_same_bits(gpu, name, dtype):
     import torch
 
@@ -48,14 +58,13 @@ def test_every_tuning_gives_the_same_bits(gpu, name, dtype):
     for direction in (capi.UP, capi.DOWN):
         cin = cg.num_samples if direction == capi.UP else cg.num_mutations
         cout = cg.num_mutations if direction == capi.UP else cg.num_samples
-        for K in (1, 5, 64, 200, 511):
+        for K in (1, 2, 3, 4, 5, 64, 200, 511):
             x = torch.from_numpy(make_input(rng, dtype, (K, cin))).cuda()
             cg.set_tuning(direction, dt, K)  # library default
             ref = _run(cg, dt, x, K, direction, cin, cout, stream).cpu().numpy()
             if np.dtype(dtype).kind == "i":
                 assert bits_equal(ref, og.matmul(x.cpu().numpy(), oracle.UP if direction == capi.UP else oracle.DOWN))
-            lanes = max(1, -(-K * es // 16))
-            tws = [1] if K * es <= 8 else [t for t in (1, 2, 4, 8, 16, 32, 64) if t <= max(1, 1 << (lanes - 1).bit_length())]
+            tws = _team_widths(K, es)
             for tw in tws:
                 for use_graph in (0, 1):
                     for xcd in (1, 2):
