@@ -88,6 +88,7 @@ struct DeviceDirPlan {
     uint32_t heavyFanin = 64;
     std::vector<uint32_t> heavyCount;   // per level, for heavyFanin
     std::vector<uint64_t> levelEntries; // per level, gather entries
+    std::vector<uint64_t> heavyEntries; // per level, gather entries of the heavy rows
     uint32_t* dLevelStart = nullptr;    // device copies read by the tail kernel
     uint32_t* dHeavyCount = nullptr;
     void computeHeavy();

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <stdexcept>
 #include <memory>
 
 #include "device_graph.h"
@@ -96,6 +97,7 @@ uint32_t maxTeamLanes(const RowGeometry& geo) {
 void DeviceDirPlan::computeHeavy() {
     heavyCount.assign(numLevels, 0);
     levelEntries.assign(numLevels, 0);
+    heavyEntries.assign(numLevels, 0);
     for (uint32_t l = firstPulledLevel; l < numLevels; l++) {
         uint32_t lo = levelStart[l], hi = levelStart[l + 1];
         for (uint32_t r = lo; r < hi; r++) levelEntries[l] += fanIn[r];
@@ -106,6 +108,7 @@ void DeviceDirPlan::computeHeavy() {
             else hi = mid;
         }
         heavyCount[l] = lo - begin;
+        for (uint32_t r = begin; r < lo; r++) heavyEntries[l] += fanIn[r];
     }
     if (numLevels > 0) {
         if (!dHeavyCount) GRGL_HIP_CHECK(hipMalloc(&dHeavyCount, numLevels * sizeof(uint32_t)));
@@ -146,9 +149,22 @@ void validateShapes(const grgl_graph* g, grgl_direction dir, size_t inCols, size
 // ---------------------------------------------------------------------------------------
 namespace {
 
-// Accesses narrower than 16 bytes come with teams of at most 8 lanes (levelVb).
-template <typename T, bool XOR> void launchPullT(uint32_t vb, uint32_t tw, dim3 grid, const PullArgs& a, hipStream_t s) {
+// Accesses narrower than 16 bytes come with teams of at most 8 lanes (levelVb).  h32: the
+// variant with per-wavefront heavy rows of 32 bytes, for (vb, tw) of a 32-byte row.
+template <typename T, bool XOR>
+void launchPullT(uint32_t vb, uint32_t tw, bool h32, dim3 grid, const PullArgs& a, hipStream_t s) {
 #define GRGL_PULL(VB, TW) hipLaunchKernelGGL((pull_level_kernel<T, VB, TW, XOR>), grid, dim3(kBlock), 0, s, a)
+#define GRGL_PULL32(VB, TW) hipLaunchKernelGGL((pull_level_kernel<T, VB, TW, XOR, true>), grid, dim3(kBlock), 0, s, a)
+    if (h32) {
+        if (vb == 16 && tw == 1) GRGL_PULL32(16, 1);
+        else if (vb == 16 && tw == 2) GRGL_PULL32(16, 2);
+        else if (vb == 8 && tw == 4) GRGL_PULL32(8, 4);
+        else if (vb == 8 && tw == 8) GRGL_PULL32(8, 8);
+        else if (vb == 4 && tw == 8) { if constexpr (sizeof(T) <= 4) GRGL_PULL32(4, 8); }
+        else throw std::logic_error("no 32-byte heavy-row variant for this team shape");
+        return;
+    }
+#undef GRGL_PULL32
     if (vb == 4) {
         if constexpr (sizeof(T) <= 4) {
             if (tw == 1) GRGL_PULL(4, 1);
@@ -219,15 +235,15 @@ void launchTail(TypeClass tc, uint32_t vb, uint32_t tw, const TailArgs& a, hipSt
     }
 }
 
-void launchPull(TypeClass tc, uint32_t vb, uint32_t tw, dim3 grid, const PullArgs& a, hipStream_t s) {
+void launchPull(TypeClass tc, uint32_t vb, uint32_t tw, bool h32, dim3 grid, const PullArgs& a, hipStream_t s) {
     switch (tc) {
-    case kF64: launchPullT<double, false>(vb, tw, grid, a, s); break;
-    case kF32: launchPullT<float, false>(vb, tw, grid, a, s); break;
-    case kW64: launchPullT<uint64_t, false>(vb, tw, grid, a, s); break;
-    case kW32: launchPullT<uint32_t, false>(vb, tw, grid, a, s); break;
-    case kW16: launchPullT<uint16_t, false>(vb, tw, grid, a, s); break;
-    case kW8: launchPullT<uint8_t, false>(vb, tw, grid, a, s); break;
-    case kX8: launchPullT<uint8_t, true>(vb, tw, grid, a, s); break;
+    case kF64: launchPullT<double, false>(vb, tw, h32, grid, a, s); break;
+    case kF32: launchPullT<float, false>(vb, tw, h32, grid, a, s); break;
+    case kW64: launchPullT<uint64_t, false>(vb, tw, h32, grid, a, s); break;
+    case kW32: launchPullT<uint32_t, false>(vb, tw, h32, grid, a, s); break;
+    case kW16: launchPullT<uint16_t, false>(vb, tw, h32, grid, a, s); break;
+    case kW8: launchPullT<uint8_t, false>(vb, tw, h32, grid, a, s); break;
+    case kX8: launchPullT<uint8_t, true>(vb, tw, h32, grid, a, s); break;
     }
 }
 
@@ -472,7 +488,10 @@ void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const
     const uint32_t tiles = levelTiles(geo, tw, elemSize);
     const uint32_t vb = levelVb(geo, tw, elemSize);
     const uint32_t teamsPerBlock = kBlock / tw;
-    const uint32_t tailFrom = g->timing ? plan.numLevels : tailStart(plan, geo, tw, orderPreserving);
+    // The single-block tail gains nothing from spreading rows over idle lanes: rows of one
+    // access take single-lane teams there (256 rows per pass instead of 256 / tw).
+    const uint32_t tailTw = geo.lanes == 1 ? 1 : tw;
+    const uint32_t tailFrom = g->timing ? plan.numLevels : tailStart(plan, geo, tailTw, orderPreserving);
     for (uint32_t l = plan.firstPulledLevel; l < tailFrom; l++) {
         const uint32_t begin = plan.levelStart[l], end = plan.levelStart[l + 1];
         if (end == begin) continue;
@@ -485,8 +504,15 @@ void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const
         a.src = plan.src;
         a.levelBegin = begin;
         a.heavyCount = heavy;
-        // Rows of one access (tiles == 1): a wavefront per heavy row, 4 per block.
-        a.heavyPerWave = geo.rowBytes <= 16 ? 1 : 0;
+        // Short rows: a wavefront per heavy row, 4 per block (see PullArgs::heavyPerWave),
+        // for levels with many heavy rows (at least 512: a few hundred rows of thousands of
+        // entries, the DOWN levels near the roots, go faster with a whole block each).
+        // 32-byte rows go through the H32 kernel variant, and only where heavy rows hold at
+        // least a quarter of the level's gather entries (its extra registers slow the light
+        // rows of the launch).
+        const bool manyHeavy = heavy >= 512;
+        const bool h32 = geo.rowBytes == 32 && tw <= 8 && manyHeavy && plan.heavyEntries[l] * 4 >= plan.levelEntries[l];
+        a.heavyPerWave = ((geo.rowBytes <= 16 && manyHeavy) || h32) ? 1 : 0;
         a.heavyBlocks = a.heavyPerWave ? (heavy + kBlock / 64 - 1) / (kBlock / 64) : heavy;
         a.levelEnd = end;
         a.rowBytes = geo.rowBytes;
@@ -500,7 +526,7 @@ void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const
             GRGL_HIP_CHECK(hipEventCreate(&ev1));
             GRGL_HIP_CHECK(hipEventRecord(ev0, stream));
         }
-        launchPull(tc, vb, tw, grid, a, stream);
+        launchPull(tc, vb, tw, h32, grid, a, stream);
         GRGL_HIP_CHECK(hipGetLastError());
         if (g->timing) {
             GRGL_HIP_CHECK(hipEventRecord(ev1, stream));
@@ -522,12 +548,12 @@ void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const
         t.src = plan.src;
         t.rowBytes = geo.rowBytes;
         t.readSelf = readSelf ? 1 : 0;
-        t.tiles = tiles;
+        t.tiles = levelTiles(geo, tailTw, elemSize);
         t.levelStart = plan.dLevelStart + tailFrom;
         t.heavyCount = plan.dHeavyCount + tailFrom;
         t.numLevels = plan.numLevels - tailFrom;
         t.noHeavy = orderPreserving ? 1 : 0;
-        launchTail(tc, vb, tw, t, stream);
+        launchTail(tc, levelVb(geo, tailTw, elemSize), tailTw, t, stream);
         GRGL_HIP_CHECK(hipGetLastError());
     }
 }

@@ -44,6 +44,10 @@ template <int VB> struct RawVec;
 template <> struct RawVec<4> { using type = uint32_t; };
 template <> struct RawVec<8> { using type = uint2; };
 template <> struct RawVec<16> { using type = uint4; };
+// A whole 32-byte row held by one lane (heavy rows of 32-byte rows, one wavefront per row):
+// two 16-byte accesses.
+struct Raw32 { uint4 a, b; };
+template <> struct RawVec<32> { using type = Raw32; };
 
 template <typename T, int VB> struct Pack {
     static constexpr int N = VB / (int)sizeof(T);
@@ -115,8 +119,9 @@ struct PullArgs {
     uint32_t heavyCount;  // heavy rows at the front of the level
     uint32_t heavyBlocks; // blocks on heavy rows: heavyCount, or heavyCount / 4 rounded up
                           // when each wavefront takes a heavy row of its own (heavyPerWave)
-    uint32_t heavyPerWave; // 1: rows of one access (at most 16 bytes, one column slice);
-                           // a wavefront's 64 lanes are a heavy row's 64 partials
+    uint32_t heavyPerWave; // 1: rows of 4, 8, 16 bytes (32 bytes in the H32 variant); a
+                           // wavefront's 64 lanes are a heavy row's 64 partials, each lane
+                           // holding the whole row
     uint32_t levelEnd;
     uint32_t rowBytes;    // bytes of each row holding values (multiple of VB)
     uint32_t readSelf;    // 1: start from the row's current value (node initialisation)
@@ -199,7 +204,7 @@ __device__ __forceinline__ void pull_heavy(const PullArgs& a, uint32_t row, uint
         // are issued before the adds, which run step by step, so each partial's order is
         // unchanged.  The last, partial iteration loads clamped (in-list) entries and adds
         // only the real ones.
-        constexpr int STEPS = PPT >= 4 ? 1 : (VB * PPT <= 4 ? 32 : (VB * PPT <= 8 ? 16 : 4 / PPT));
+        constexpr int STEPS = PPT >= 4 ? 1 : (VB * PPT <= 4 ? 32 : (VB * PPT <= 8 ? 16 : (VB <= 16 ? 4 / PPT : 4)));
         constexpr uint32_t STRIDE = (uint32_t)(STEPS * PPT) * A;
         uint32_t i = b + team;
         for (; i + STRIDE - A < e; i += STRIDE) {
@@ -300,7 +305,10 @@ struct StagedIdx {
     __device__ uint32_t operator()(uint32_t i) const { return lds[i - base]; }
 };
 
-template <typename T, int VB, int TW, bool XOR>
+// H32: the variant launched for levels with heavy rows when rows are 32 bytes; its
+// per-wavefront heavy path holds 32 bytes per lane, whose registers would lower the
+// occupancy of the levels without heavy rows, so they use the plain variant.
+template <typename T, int VB, int TW, bool XOR, bool H32 = false>
 __global__ __launch_bounds__(kBlock) void pull_level_kernel(PullArgs a) {
     const uint32_t lane = threadIdx.x % TW;
     const uint32_t team = threadIdx.x / TW;
@@ -321,8 +329,10 @@ __global__ __launch_bounds__(kBlock) void pull_level_kernel(PullArgs a) {
                 if constexpr (sizeof(T) <= 4) pull_heavy<T, 4, 1, XOR>(a, row, 0, true, part, 0);
             } else if (a.rowBytes == 8) {
                 if constexpr (sizeof(T) <= 8) pull_heavy<T, 8, 1, XOR>(a, row, 0, true, part, 0);
-            } else {
+            } else if (a.rowBytes == 16) {
                 pull_heavy<T, 16, 1, XOR>(a, row, 0, true, part, 0);
+            } else if constexpr (H32) {
+                pull_heavy<T, 32, 1, XOR>(a, row, 0, true, part, 0);
             }
         } else {
             pull_heavy<T, VB, TW, XOR>(a, a.levelBegin + bx, colByte, active, team, lane);
