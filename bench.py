@@ -499,6 +499,38 @@ def pca_benchmark():
     return out
 
 
+def latency_trace(args):
+    """One request per traversal, FP32, K = 1, 4, 16, DOWN then UP per K, library defaults:
+    3 untimed calls, then 20 timed calls per shape; one JSON line of milliseconds per call."""
+    import torch
+
+    import grgl_amd
+    from grgl_amd import capi
+
+    g = grgl_amd.grg_from_coalescent(args.haplotypes, args.sites, args.trees, seed=SEED)
+    cg = capi.Graph(g.arrays())
+    sp = torch.cuda.current_stream().cuda_stream
+    N, M = g.num_samples, g.num_mutations
+    out = {"workload": f"{args.haplotypes} haplotypes x {args.sites} sites, {args.trees} trees; FP32, one request "
+                       "per traversal", "ms": {}}
+    for K in (1, 4, 16):
+        for d, dn, cin, cout in ((capi.DOWN, "DOWN", M, N), (capi.UP, "UP", N, M)):
+            x = torch.randn((K, cin), device="cuda")
+            y = torch.zeros((K, cout), device="cuda")
+            for _ in range(3):
+                cg.matmul_ptr(capi.F32, x.data_ptr(), cin, K, d, y.data_ptr(), y.numel(), stream=sp)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(20):
+                cg.matmul_ptr(capi.F32, x.data_ptr(), cin, K, d, y.data_ptr(), y.numel(), stream=sp)
+            torch.cuda.synchronize()
+            out["ms"][f"{dn}_K{K}"] = round((time.perf_counter() - t) / 20 * 1e3, 4)
+            log(f"{dn} K={K}: {out['ms'][f'{dn}_K{K}']} ms")
+            time.sleep(0.3)
+    cg.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -515,6 +547,10 @@ def main():
     ap.add_argument("--no-pca", action="store_true", help="skip the standardized PCA scenario")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the option, .grg file and construction measurements")
+    ap.add_argument("--latency-trace", action="store_true",
+                    help="only time one-request FP32 traversals of the benchmark graph at K = 1, 4, 16 (UP and "
+                         "DOWN, library default tuning), 20 calls per shape with 0.3 s between shapes, so a "
+                         "kernel trace splits by shape")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -527,6 +563,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4}))
+        return
+    if args.latency_trace:
+        latency_trace(args)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
