@@ -75,3 +75,37 @@ def test_row_limits(gpu):
         grgl_amd.matmul(_pg(a), np.ones((65537, 4), np.int8), UP)
     with pytest.raises(RuntimeError):
         grgl_amd.matmul(_pg(a), np.ones((0, 4), np.int8), UP)
+
+
+@pytest.mark.parametrize("ploidy", [3, 4])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.int32, np.int8, np.bool_])
+def test_polyploid_by_individual_and_xtx(gpu, ploidy, dtype):
+    """Individuals of 3 or 4 haplotypes: by_individual sums `ploidy` consecutive haplotypes
+    (grg.h:1448,1463), xtx initialises with 2 x the coalescence counts (grg.h:1369-1385)."""
+    from helpers import random_grg
+
+    a = random_grg(40 + ploidy, num_samples=ploidy * 21, num_internal=300, heavy=(90, 150), num_mutations=240,
+                   ploidy=ploidy, with_coal=True)
+    g = _pg_coal(a)
+    og = oracle.OracleGraph(a)
+    rng = np.random.default_rng(ploidy)
+    exact = np.dtype(dtype).kind in "biu"
+    for direction, od in ((UP, oracle.UP), (DOWN, oracle.DOWN)):
+        cols = g.num_individuals if direction == UP else g.num_mutations
+        for K in (1, 5, 40):
+            x = make_input(rng, dtype, (K, cols))
+            want = og.matmul(x, od, by_individual=True)
+            got = grgl_amd.matmul(g, x, direction, by_individual=True, order_preserving=not exact)
+            assert bits_equal(got, want), (direction, K)
+        xf = make_input(rng, dtype, (2, g.num_samples if direction == UP else g.num_mutations))
+        if dtype == np.bool_:  # as the reference (src/python/_grgl.cpp:236)
+            with pytest.raises(RuntimeError):
+                grgl_amd.matmul(g, xf, direction, init="xtx")
+            continue
+        got = grgl_amd.matmul(g, xf, direction, init="xtx", order_preserving=not exact)
+        assert bits_equal(got, og.matmul(xf, od, init="xtx")), direction
+
+
+def _pg_coal(a):
+    return grgl_amd.grg_from_arrays(a["num_samples"], a["ploidy"], a["down_offsets"], a["down_children"],
+                                    a["mutation_nodes"], coal_counts=a.get("coal_counts"))

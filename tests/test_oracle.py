@@ -172,6 +172,10 @@ GRAPHS = {
     "random_miss_coal": lambda: random_grg(12, num_samples=48, num_internal=150, num_mutations=300, with_miss=True,
                                            with_coal=True),
     "vcf": lambda: direct_map_grg(vcf_fixture()),
+    # polyploid individuals: by_individual sums `ploidy` consecutive haplotypes (grg.h:1448,1463)
+    "ploidy3": lambda: random_grg(13, num_samples=63, num_internal=200, heavy=(80,), num_mutations=250, ploidy=3,
+                                  with_coal=True),
+    "ploidy4": lambda: random_grg(14, num_samples=64, num_internal=150, num_mutations=200, ploidy=4),
 }
 
 
