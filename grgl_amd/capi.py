@@ -112,7 +112,7 @@ class Request(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("team_lanes", ctypes.c_uint32), ("heavy_fanin", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
-                ("xcd_map", ctypes.c_uint32)]
+                ("xcd_map", ctypes.c_uint32), ("max_launch_entries", ctypes.c_uint32)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -305,8 +305,9 @@ class Graph:
             arr[i].output = op
         check(lib().grgl_hip_matmul_batch(self.handle, dtype, direction, arr, len(reqs), flags, stream))
 
-    def set_tuning(self, direction, dtype, rows, team_lanes=0, heavy_fanin=0, use_graph=0, xcd_map=0):
-        t = Tuning(team_lanes, heavy_fanin, use_graph, xcd_map)
+    def set_tuning(self, direction, dtype, rows, team_lanes=0, heavy_fanin=0, use_graph=0, xcd_map=0,
+                   max_launch_entries=0):
+        t = Tuning(team_lanes, heavy_fanin, use_graph, xcd_map, max_launch_entries)
         check(lib().grgl_hip_set_tuning(self.handle, direction, dtype, rows, ctypes.byref(t)))
 
     def autotune(self, direction, dtype, rows):
@@ -314,7 +315,7 @@ class Graph:
         ms = ctypes.c_float()
         check(lib().grgl_hip_autotune(self.handle, direction, dtype, rows, ctypes.byref(t), ctypes.byref(ms)))
         return {"team_lanes": t.team_lanes, "heavy_fanin": t.heavy_fanin, "use_graph": t.use_graph,
-                "xcd_map": t.xcd_map}, ms.value
+                "xcd_map": t.xcd_map, "max_launch_entries": t.max_launch_entries}, ms.value
 
     def kernel_timing(self, enable):
         check(lib().grgl_hip_kernel_timing(self.handle, int(enable)))

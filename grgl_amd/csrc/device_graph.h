@@ -73,12 +73,23 @@ inline uint32_t levelTiles(const RowGeometry& geo, uint32_t tw, size_t elemSize)
     return (geo.rowBytes / levelVb(geo, tw, elemSize) + tw - 1) / tw;
 }
 
+// A launch's share of a level: consecutive rows whose gather lists hold at most
+// kChunkEntries entries (or one longer list), so that list offsets relative to the chunk's
+// first entry fit 32 bits whatever the graph's total.  Graphs under 2^30 entries per level
+// have one chunk per level.
+struct LevelChunk {
+    uint32_t rowBegin, rowEnd;
+    uint32_t heavy;       // heavy rows at the front of the chunk (heavyFanin)
+    uint64_t entryBegin;  // first gather entry of rowBegin
+};
+constexpr uint64_t kChunkEntries = 1ULL << 30;
+
 struct DeviceDirPlan {
     uint32_t numNodes = 0, numLevels = 0, firstPulledLevel = 0, maxFanIn = 0;
     uint64_t numEntries = 0;
     std::vector<uint32_t> levelStart;
     std::vector<uint32_t> fanIn;   // host: per row list length (heavy split points)
-    uint32_t* off = nullptr;
+    uint32_t* off = nullptr;       // list offsets modulo 2^32 (kernels subtract a chunk's base)
     uint32_t* src = nullptr;
     uint32_t* order = nullptr;     // row -> node
     uint32_t* rank = nullptr;      // node -> row
@@ -89,6 +100,10 @@ struct DeviceDirPlan {
     std::vector<uint32_t> heavyCount;   // per level, for heavyFanin
     std::vector<uint64_t> levelEntries; // per level, gather entries
     std::vector<uint64_t> heavyEntries; // per level, gather entries of the heavy rows
+    std::vector<uint64_t> levelEntryBegin; // numLevels + 1: first gather entry of each level
+    std::vector<LevelChunk> chunks;     // launches of every level, in level order
+    std::vector<uint32_t> levelChunk;   // numLevels + 1 offsets into chunks
+    uint64_t chunkEntries = kChunkEntries; // gather entries per launch at most (tuning)
     uint32_t* dLevelStart = nullptr;    // device copies read by the tail kernel
     uint32_t* dHeavyCount = nullptr;
     void computeHeavy();
@@ -135,7 +150,9 @@ struct grgl_graph {
     uint64_t timedLaunches = 0;
     double timedBytes = 0.0;
     // Captured level sequences: (dir, typeClass, vb, tw, ld, rowBytes, readSelf, rows ptr)
-    std::map<std::tuple<int, int, uint32_t, uint32_t, uint64_t, uint32_t, int, uintptr_t>, grgl_amd::CapturedLevels> graphs;
+    std::map<std::tuple<int, int, uint32_t, uint32_t, uint64_t, uint32_t, int, uintptr_t, uint64_t>,
+             grgl_amd::CapturedLevels>
+        graphs;
 };
 
 namespace grgl_amd {

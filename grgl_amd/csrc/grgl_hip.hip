@@ -98,7 +98,13 @@ void DeviceDirPlan::computeHeavy() {
     heavyCount.assign(numLevels, 0);
     levelEntries.assign(numLevels, 0);
     heavyEntries.assign(numLevels, 0);
-    for (uint32_t l = firstPulledLevel; l < numLevels; l++) {
+    chunks.clear();
+    levelChunk.assign(numLevels + 1, 0);
+    levelEntryBegin.assign(numLevels + 1, 0);
+    uint64_t entry = 0; // first entry of level l
+    for (uint32_t l = 0; l < numLevels; l++) {
+        levelChunk[l] = (uint32_t)chunks.size();
+        levelEntryBegin[l] = entry;
         uint32_t lo = levelStart[l], hi = levelStart[l + 1];
         for (uint32_t r = lo; r < hi; r++) levelEntries[l] += fanIn[r];
         const uint32_t begin = lo;
@@ -107,9 +113,35 @@ void DeviceDirPlan::computeHeavy() {
             if (fanIn[mid] > heavyFanin) lo = mid + 1;
             else hi = mid;
         }
-        heavyCount[l] = lo - begin;
-        for (uint32_t r = begin; r < lo; r++) heavyEntries[l] += fanIn[r];
+        const uint32_t heavyEnd = lo;
+        heavyCount[l] = heavyEnd - begin;
+        for (uint32_t r = begin; r < heavyEnd; r++) heavyEntries[l] += fanIn[r];
+        if (l >= firstPulledLevel && levelStart[l + 1] > begin) {
+            // Cut the level into chunks of at most chunkEntries entries (a longer list
+            // gets a chunk of its own); one chunk unless the level is that large.
+            LevelChunk c{begin, begin, 0, entry};
+            uint64_t inChunk = 0, e = entry;
+            for (uint32_t r = begin; r < levelStart[l + 1]; r++) {
+                if (r > c.rowBegin && inChunk + fanIn[r] > chunkEntries) {
+                    c.rowEnd = r;
+                    chunks.push_back(c);
+                    c = LevelChunk{r, r, 0, e};
+                    inChunk = 0;
+                }
+                inChunk += fanIn[r];
+                e += fanIn[r];
+            }
+            c.rowEnd = levelStart[l + 1];
+            chunks.push_back(c);
+            for (uint32_t k = levelChunk[l]; k < chunks.size(); k++) {
+                LevelChunk& ck = chunks[k];
+                ck.heavy = heavyEnd > ck.rowBegin ? std::min(heavyEnd, ck.rowEnd) - ck.rowBegin : 0;
+            }
+        }
+        entry += levelEntries[l];
     }
+    levelChunk[numLevels] = (uint32_t)chunks.size();
+    levelEntryBegin[numLevels] = entry;
     if (numLevels > 0) {
         if (!dHeavyCount) GRGL_HIP_CHECK(hipMalloc(&dHeavyCount, numLevels * sizeof(uint32_t)));
         GRGL_HIP_CHECK(hipMemcpy(dHeavyCount, heavyCount.data(), numLevels * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -493,15 +525,18 @@ void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const
     const uint32_t tailTw = geo.lanes == 1 ? 1 : tw;
     const uint32_t tailFrom = g->timing ? plan.numLevels : tailStart(plan, geo, tailTw, orderPreserving);
     for (uint32_t l = plan.firstPulledLevel; l < tailFrom; l++) {
-        const uint32_t begin = plan.levelStart[l], end = plan.levelStart[l + 1];
+      for (uint32_t ci = plan.levelChunk[l]; ci < plan.levelChunk[l + 1]; ci++) {
+        const LevelChunk& ck = plan.chunks[ci];
+        const uint32_t begin = ck.rowBegin, end = ck.rowEnd;
         if (end == begin) continue;
-        const uint32_t heavy = orderPreserving ? 0 : plan.heavyCount[l];
+        const uint32_t heavy = orderPreserving ? 0 : ck.heavy;
         const uint32_t light = end - begin - heavy;
         PullArgs a;
         a.rows = rows;
         a.ld = geo.ld;
         a.off = plan.off;
-        a.src = plan.src;
+        a.src = plan.src + ck.entryBegin;
+        a.entryBase = (uint32_t)ck.entryBegin;
         a.levelBegin = begin;
         a.heavyCount = heavy;
         // Short rows: a wavefront per heavy row, 4 per block (see PullArgs::heavyPerWave),
@@ -539,13 +574,18 @@ void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const
             g->timingEvents.push_back({ev0, ev1});
             g->timingBytes.push_back(bytes);
         }
+      }
     }
     if (tailFrom < plan.numLevels) {
         TailArgs t;
         t.rows = rows;
         t.ld = geo.ld;
+        // Tail levels are small (at most 64 KB of rows each), so one base covers them all;
+        // the tail kernel takes whole levels.
+        const uint64_t tailEntry = plan.levelEntryBegin[tailFrom];
         t.off = plan.off;
-        t.src = plan.src;
+        t.src = plan.src + tailEntry;
+        t.entryBase = (uint32_t)tailEntry;
         t.rowBytes = geo.rowBytes;
         t.readSelf = readSelf ? 1 : 0;
         t.tiles = levelTiles(geo, tailTw, elemSize);
@@ -571,7 +611,7 @@ void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometr
     const auto key = std::make_tuple((int)dir, (int)tc, geo.vb, t.teamLanes, geo.ld, geo.rowBytes,
                                      (int)readSelf * 2 + (int)orderPreserving + 4 * (int)(t.xcdMap == 1) +
                                          8 * (int)plan.heavyFanin,
-                                     (uintptr_t)rows);
+                                     (uintptr_t)rows, plan.chunkEntries);
     hipGraphExec_t exec = nullptr;
     {
         std::lock_guard<std::mutex> lk(g->mu);
@@ -1161,9 +1201,10 @@ GRGL_EXPORT grgl_status grgl_hip_set_tuning(grgl_graph* g, grgl_direction direct
     if (tuning->xcd_map > 2) throw MisuseError("xcd_map must be 0, 1 or 2");
     t.xcdMap = tuning->xcd_map;
     g->tuning[std::make_tuple((int)direction, (int)tc, rows)] = t;
-    if (tuning->heavy_fanin) {
+    if (tuning->heavy_fanin || tuning->max_launch_entries) {
         DeviceDirPlan& plan = direction == GRGL_DIRECTION_UP ? g->up : g->down;
-        plan.heavyFanin = tuning->heavy_fanin;
+        if (tuning->heavy_fanin) plan.heavyFanin = tuning->heavy_fanin;
+        if (tuning->max_launch_entries) plan.chunkEntries = std::min<uint64_t>(tuning->max_launch_entries, kChunkEntries);
         plan.computeHeavy();
     }
     GRGL_API_END
@@ -1287,6 +1328,7 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
         chosen->heavy_fanin = (direction == GRGL_DIRECTION_UP ? g->up : g->down).heavyFanin;
         chosen->use_graph = best.useGraph;
         chosen->xcd_map = best.xcdMap;
+        chosen->max_launch_entries = (uint32_t)(direction == GRGL_DIRECTION_UP ? g->up : g->down).chunkEntries;
     }
     if (best_ms) *best_ms = bestMs;
     GRGL_API_END

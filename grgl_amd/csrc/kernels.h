@@ -113,9 +113,11 @@ template <typename T, int VB> __device__ __forceinline__ Pack<T, VB> shfl_down_p
 struct PullArgs {
     char* rows;
     uint64_t ld;          // row stride, bytes
-    const uint32_t* off;
-    const uint32_t* src;
-    uint32_t levelBegin;  // first row of the level
+    const uint32_t* off;  // list offsets modulo 2^32
+    const uint32_t* src;  // the launch's first gather entry
+    uint32_t entryBase;   // off[] value (modulo 2^32) of that entry; lists are read at
+                          // src[off[row] - entryBase], < 2^32 within a launch
+    uint32_t levelBegin;  // first row of the launch (a level, or a chunk of a large level)
     uint32_t heavyCount;  // heavy rows at the front of the level
     uint32_t heavyBlocks; // blocks on heavy rows: heavyCount, or heavyCount / 4 rounded up
                           // when each wavefront takes a heavy row of its own (heavyPerWave)
@@ -152,7 +154,7 @@ template <typename T, int VB, int TW, bool XOR, int UNR, typename Idx>
 __device__ __forceinline__ void pull_light(const PullArgs& a, uint32_t row, uint64_t colByte, bool active,
                                            Idx idx) {
     using P = Pack<T, VB>;
-    const uint32_t b = a.off[row], e = a.off[row + 1];
+    const uint32_t b = a.off[row] - a.entryBase, e = a.off[row + 1] - a.entryBase;
     P acc = zero_pack<T, VB>();
     if (!active) return;
     if (a.readSelf) acc = load_pack<T, VB>(a.rows + (uint64_t)row * a.ld + colByte);
@@ -190,7 +192,7 @@ __device__ __forceinline__ void pull_heavy(const PullArgs& a, uint32_t row, uint
     constexpr int PPT = NT >= kHeavyPartials ? 1 : kHeavyPartials / NT; // partials per team
     constexpr int ACTIVE_TEAMS = NT >= kHeavyPartials ? kHeavyPartials : NT;
     __shared__ char lds[ACTIVE_TEAMS * TW * VB];
-    const uint32_t b = a.off[row], e = a.off[row + 1];
+    const uint32_t b = a.off[row] - a.entryBase, e = a.off[row + 1] - a.entryBase;
     P part[PPT];
 #pragma unroll
     for (int q = 0; q < PPT; q++) part[q] = zero_pack<T, VB>();
@@ -342,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void pull_level_kernel(PullArgs a) {
         __shared__ uint32_t staged[kStageEntries];
         const uint32_t r0 = a.levelBegin + a.heavyCount + (bx - a.heavyBlocks) * NT;
         const uint32_t r1 = r0 + NT < a.levelEnd ? r0 + NT : a.levelEnd;
-        const uint32_t e0 = a.off[r0], n = a.off[r1] - e0;
+        const uint32_t e0 = a.off[r0] - a.entryBase, n = a.off[r1] - a.off[r0];
         const uint32_t row = r0 + team;
         if (n <= kStageEntries) { // block-uniform
             // Every index load of a thread is issued before its first LDS store, so the
@@ -380,6 +382,7 @@ struct TailArgs {
     uint64_t ld;
     const uint32_t* off;
     const uint32_t* src;
+    uint32_t entryBase;         // as PullArgs (src points at the first level's first entry)
     const uint32_t* levelStart; // numLevels + 1 row boundaries (device)
     const uint32_t* heavyCount; // numLevels heavy-row counts (device)
     uint32_t rowBytes;
@@ -400,6 +403,7 @@ __global__ __launch_bounds__(kBlock) void pull_tail_kernel(TailArgs t) {
         a.ld = t.ld;
         a.off = t.off;
         a.src = t.src;
+        a.entryBase = t.entryBase;
         a.levelBegin = t.levelStart[l];
         a.heavyCount = t.noHeavy ? 0 : t.heavyCount[l];
         a.heavyBlocks = a.heavyCount;

@@ -34,8 +34,10 @@ void orderByLevel(DirPlan& p, const std::vector<uint32_t>& level, const std::vec
     for (uint32_t j = 0; j < V; j++) p.rank[p.order[j]] = j;
 }
 
-void checkEntryCount(uint64_t n) {
-    if (n >= 0xffffffffULL) throw std::runtime_error("gather lists exceed 2^32 entries; graph too large for u32 offsets");
+// List offsets are kept modulo 2^32 (the traversal reads them relative to a launch's
+// first entry), so the total may exceed 2^32; one list may not exceed 2^31 entries.
+void checkListLength(uint64_t n) {
+    if (n >= (1ULL << 31)) throw std::runtime_error("a gather list exceeds 2^31 entries");
 }
 
 std::vector<uint32_t> sortedMutationOrder(const HostGrg& g) {
@@ -96,16 +98,16 @@ DirPlan buildUpPlan(const HostGrg& g) {
     uint64_t total = 0;
     for (uint32_t j = 0; j < V; j++) {
         const uint32_t n = p.order[j];
+        checkListLength(fanIn[n]);
         total += fanIn[n];
-        checkEntryCount(total);
-        p.off[j + 1] = (uint32_t)total;
+        p.off[j + 1] = (uint32_t)total; // modulo 2^32
         p.maxFanIn = std::max<uint32_t>(p.maxFanIn, (uint32_t)fanIn[n]);
     }
     p.src.resize(total);
+    uint64_t o = 0;
     for (uint32_t j = 0; j < V; j++) {
         const uint32_t n = p.order[j];
-        uint32_t o = p.off[j];
-        if (n < N) continue;
+        if (n < N) continue; // samples have no list (fanIn 0)
         for (uint64_t i = g.downOff[n]; i < g.downOff[n + 1]; i++) p.src[o++] = p.rank[g.downChild[i]];
     }
     const uint32_t M = g.numMutations();
@@ -143,9 +145,14 @@ DirPlan buildDownPlan(const HostGrg& g) {
         if (hasMiss && g.mutMiss[m] != kInvalidNode) cnt[g.mutMiss[m]]++;
     }
     for (uint64_t i = 0; i < g.numEdges(); i++) cnt[g.downChild[i]]++;
+    // Seed rows follow the V node rows: V + m (and V + M + m for missingness), u32 indices.
+    if ((uint64_t)V + (uint64_t)M * (hasMiss ? 2 : 1) >= (1ULL << 32))
+        throw std::runtime_error("nodes plus mutation seed rows exceed 2^32");
     std::vector<uint64_t> nodeOff(V + 1, 0);
-    for (uint32_t v = 0; v < V; v++) nodeOff[v + 1] = nodeOff[v] + cnt[v];
-    checkEntryCount(nodeOff[V]);
+    for (uint32_t v = 0; v < V; v++) {
+        checkListLength(cnt[v]);
+        nodeOff[v + 1] = nodeOff[v] + cnt[v];
+    }
     std::vector<uint32_t> nodeSrc(nodeOff[V]);
     std::vector<uint64_t> cur(nodeOff.begin(), nodeOff.end() - 1);
     const std::vector<uint32_t> tuples = sortedMutationOrder(g);
@@ -167,7 +174,7 @@ DirPlan buildDownPlan(const HostGrg& g) {
             const uint32_t s = nodeSrc[i];
             p.src[o++] = (s & kParentTag) ? p.rank[s & ~kParentTag] : s;
         }
-        p.off[j + 1] = (uint32_t)o;
+        p.off[j + 1] = (uint32_t)o; // modulo 2^32
         p.maxFanIn = std::max<uint32_t>(p.maxFanIn, (uint32_t)cnt[n]);
     }
     (void)N;

@@ -31,7 +31,8 @@ struct DirPlan {
     std::vector<uint32_t> levelStart;    // numLevels + 1 row offsets
     std::vector<uint32_t> order;         // row -> node ID
     std::vector<uint32_t> rank;          // node ID -> row
-    std::vector<uint32_t> off;           // numNodes + 1 gather-list offsets
+    std::vector<uint32_t> off;           // numNodes + 1 gather-list offsets, modulo 2^32 (a
+                                         // list's length is off[j + 1] - off[j] in u32)
     std::vector<uint32_t> src;           // gather-list entries (row indices)
     uint32_t maxFanIn = 0;
     // Readout: UP maps each mutation to its node's row (or invalid) and its missingness
@@ -41,7 +42,8 @@ struct DirPlan {
     std::vector<uint32_t> coalByRow;     // per row: individual coalescence count (samples 0)
 };
 
-// Throws std::runtime_error if the graph is not an ordered GRG or lists exceed 2^32 entries.
+// Throws std::runtime_error if the graph is not an ordered GRG, one gather list exceeds 2^31
+// entries, or (DOWN) nodes plus mutation seed rows exceed 2^32.
 DirPlan buildUpPlan(const HostGrg& g);
 DirPlan buildDownPlan(const HostGrg& g);
 

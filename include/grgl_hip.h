@@ -161,6 +161,10 @@ typedef struct grgl_tuning {
                                 takes slices i, i+8, ...), 2 = plain 2-D grid.  The plain grid runs the
                                 column slices one after another, so one slice of every row (V x 256 B)
                                 stays in the memory-side cache shared by all XCDs; measured faster. */
+    uint32_t max_launch_entries; /* gather entries per level-kernel launch at most (0 = library default,
+                                    2^30, also the upper limit); larger levels are split into several
+                                    launches.  Never changes results; like heavy_fanin it applies to
+                                    every later call in that direction. */
 } grgl_tuning;
 
 grgl_status grgl_hip_set_tuning(grgl_graph* graph, grgl_direction direction, grgl_dtype dtype, uint32_t rows,

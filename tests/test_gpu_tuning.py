@@ -99,3 +99,37 @@ def test_autotune_choice_is_used_and_exact(gpu):
     with pytest.raises(RuntimeError):
         cg.set_tuning(capi.UP, capi.I32, 8, xcd_map=7)
     cg.close()
+
+
+@pytest.mark.parametrize("name", ["heavy", "tree"])
+def test_split_levels_give_the_same_bits(gpu, name):
+    """Levels cut into several launches (max_launch_entries; the library does this for levels
+    of more than 2^30 gather entries, whose list offsets are then read relative to each
+    launch's first entry) give the same bits, with and without hipGraph replay."""
+    import torch
+
+    arrays = _graphs()[name]
+    cg = capi.Graph(arrays)
+    og = oracle.OracleGraph(arrays)
+    rng = np.random.default_rng(12)
+    stream = torch.cuda.current_stream().cuda_stream
+    for dtype in (np.float32, np.int32, np.float64):
+        dt = capi.NUMPY_TO_DTYPE[np.dtype(dtype)]
+        for direction in (capi.UP, capi.DOWN):
+            cin = cg.num_samples if direction == capi.UP else cg.num_mutations
+            cout = cg.num_mutations if direction == capi.UP else cg.num_samples
+            for K in (1, 3, 40):
+                x = torch.from_numpy(make_input(rng, dtype, (K, cin))).cuda()
+                cg.set_tuning(direction, dt, K, max_launch_entries=1 << 30)
+                ref = _run(cg, dt, x, K, direction, cin, cout, stream).cpu().numpy()
+                if np.dtype(dtype).kind == "i":
+                    assert bits_equal(ref, og.matmul(x.cpu().numpy(), oracle.UP if direction == capi.UP else oracle.DOWN))
+                for chunk in (1, 7, 100, 5000):
+                    for use_graph in (0, 1):
+                        cg.set_tuning(direction, dt, K, use_graph=use_graph, max_launch_entries=chunk)
+                        for rep in range(2 if use_graph else 1):
+                            got = _run(cg, dt, x, K, direction, cin, cout, stream)
+                            torch.cuda.synchronize()
+                            assert bits_equal(got.cpu().numpy(), ref), (direction, K, chunk, use_graph, rep)
+                cg.set_tuning(direction, dt, K, max_launch_entries=1 << 30)
+    cg.close()
