@@ -1018,26 +1018,15 @@ GRGL_EXPORT grgl_status grgl_hip_graph_info(const grgl_graph* g, grgl_graph_info
     GRGL_API_END
 }
 
-GRGL_EXPORT grgl_status grgl_hip_matmul(grgl_graph* g, grgl_dtype dtype, const void* input, size_t input_cols,
-                                        size_t input_rows, grgl_direction direction, void* output, size_t output_size,
-                                        int emit_all_nodes, int by_individual, const void* init,
-                                        grgl_init_mode init_mode, void* miss, uint32_t flags, void* stream) {
-    GRGL_API_BEGIN
-    if (!g) throw MisuseError("null graph");
-    const TypeClass tc = typeClassOf(dtype);
-    validateShapes(g, direction, input_cols, input_rows, output_size, emit_all_nodes != 0, by_individual != 0);
-    if (!input || !output) throw MisuseError("null input or output");
-    if (input_rows > 65536) throw MisuseError("at most 65536 input rows per call");
-    if (tc == kX8 && init_mode != GRGL_INIT_ZERO)
-        throw MisuseError("Node initialization and bit vector type cannot be mixed");
-    if (init_mode == GRGL_INIT_XTX && !g->hasCoal)
-        throw MisuseError("Coalescent counts not available for this GRG (init=XTX won't work)");
-    if ((init_mode == GRGL_INIT_VECTOR || init_mode == GRGL_INIT_MATRIX) && !init)
-        throw MisuseError("init mode requires an init buffer");
-    if (miss && emit_all_nodes) throw MisuseError("The \"miss\" parameter cannot be mixed with the \"emit_all_nodes\" parameter");
-    const uint32_t K = (uint32_t)input_rows;
+namespace {
+
+// One traversal of up to 65536 input rows (grgl_hip_matmul splits larger inputs).  Shapes and
+// arguments are validated by the caller.
+void matmulRows(grgl_graph* g, grgl_dtype dtype, const void* input, size_t input_cols, uint32_t K,
+                grgl_direction direction, void* output, size_t output_size, int emit_all_nodes, int by_individual,
+                const void* init, grgl_init_mode init_mode, void* miss, uint32_t flags, void* stream) {
+    const size_t input_rows = K;
     const size_t es = dtypeSize(dtype);
-    GRGL_HIP_CHECK(hipSetDevice(g->device));
     const bool host = (flags & GRGL_FLAG_HOST_PTRS) != 0;
     hipStream_t s = (hipStream_t)stream;
     bool ownStream = false;
@@ -1107,6 +1096,58 @@ GRGL_EXPORT grgl_status grgl_hip_matmul(grgl_graph* g, grgl_dtype dtype, const v
     }
     releaseWorkspace(g, ws, s);
     if (ownStream) releaseInternalStream(g, s);
+}
+
+// Input rows per traversal: 65536, or fewer when the row buffer (plus staging of host
+// buffers) would take more than half of the free device memory.  Rows are independent, so a
+// larger input is computed block by block with identical results.
+uint32_t rowsPerTraversal(grgl_graph* g, grgl_direction dir, size_t es, bool host, size_t inCols, size_t outCols) {
+    uint32_t kb = 65536;
+    auto need = [&](uint32_t k) {
+        size_t b = rowsBufferBytes(g, dir, k, es);
+        if (host) b += (size_t)k * (inCols + outCols) * es;
+        return b;
+    };
+    if (need(kb) <= (1ULL << 30)) return kb; // small graphs: no device query
+    size_t freeBytes = 0, totalBytes = 0;
+    if (hipMemGetInfo(&freeBytes, &totalBytes) != hipSuccess) return kb;
+    while (kb > 1 && need(kb) > freeBytes / 2) kb /= 2;
+    return kb;
+}
+
+} // namespace
+
+GRGL_EXPORT grgl_status grgl_hip_matmul(grgl_graph* g, grgl_dtype dtype, const void* input, size_t input_cols,
+                                        size_t input_rows, grgl_direction direction, void* output, size_t output_size,
+                                        int emit_all_nodes, int by_individual, const void* init,
+                                        grgl_init_mode init_mode, void* miss, uint32_t flags, void* stream) {
+    GRGL_API_BEGIN
+    if (!g) throw MisuseError("null graph");
+    const TypeClass tc = typeClassOf(dtype);
+    validateShapes(g, direction, input_cols, input_rows, output_size, emit_all_nodes != 0, by_individual != 0);
+    if (!input || !output) throw MisuseError("null input or output");
+    if (tc == kX8 && init_mode != GRGL_INIT_ZERO)
+        throw MisuseError("Node initialization and bit vector type cannot be mixed");
+    if (init_mode == GRGL_INIT_XTX && !g->hasCoal)
+        throw MisuseError("Coalescent counts not available for this GRG (init=XTX won't work)");
+    if ((init_mode == GRGL_INIT_VECTOR || init_mode == GRGL_INIT_MATRIX) && !init)
+        throw MisuseError("init mode requires an init buffer");
+    if (miss && emit_all_nodes) throw MisuseError("The \"miss\" parameter cannot be mixed with the \"emit_all_nodes\" parameter");
+    GRGL_HIP_CHECK(hipSetDevice(g->device));
+    const size_t es = dtypeSize(dtype);
+    const size_t outCols = output_size / input_rows;
+    const uint32_t kb = (uint32_t)std::min<size_t>(
+        input_rows, rowsPerTraversal(g, direction, es, (flags & GRGL_FLAG_HOST_PTRS) != 0, input_cols, outCols));
+    // Row blocks: input, output, init and miss are row-major with one row per input row.
+    for (size_t k0 = 0; k0 < input_rows; k0 += kb) {
+        const uint32_t rows = (uint32_t)std::min<size_t>(kb, input_rows - k0);
+        const void* blockInit = init;
+        if (init_mode == GRGL_INIT_VECTOR) blockInit = (const char*)init + k0 * es;
+        else if (init_mode == GRGL_INIT_MATRIX) blockInit = (const char*)init + k0 * g->V * es;
+        matmulRows(g, dtype, (const char*)input + k0 * input_cols * es, input_cols, rows, direction,
+                   (char*)output + k0 * outCols * es, (size_t)rows * outCols, emit_all_nodes, by_individual, blockInit,
+                   init_mode, miss ? (char*)miss + k0 * g->M * es : nullptr, flags, stream);
+    }
     GRGL_API_END
 }
 

@@ -121,6 +121,9 @@ grgl_status grgl_hip_graph_info(const grgl_graph* graph, grgl_graph_info* info);
  *                UP: an output, accumulated like `output`.
  * Without GRGL_FLAG_HOST_PTRS every pointer is device memory and the work is enqueued on
  * `stream` (a hipStream_t; NULL = the device's null stream) without a host sync.
+ * Any K: inputs of more than 65536 rows, or whose row buffer would take more than half of
+ * the free device memory, run as several traversals of consecutive row blocks (rows are
+ * independent; results are the same).
  */
 grgl_status grgl_hip_matmul(grgl_graph* graph, grgl_dtype dtype, const void* input, size_t input_cols,
                             size_t input_rows, grgl_direction direction, void* output, size_t output_size,

@@ -67,14 +67,65 @@ def test_single_haploid_sample(gpu):
 
 
 def test_row_limits(gpu):
+    """Any number of input rows, as on the CPU path: more than 65536 rows run as several
+    traversals of independent row blocks.  Zero rows is misuse (_grgl.cpp:100-110)."""
     a = csr_from_lists(4, [[], [], [], [], [0, 1], [2, 3], [4, 5]], np.array([4, 6, 1], np.int64))
-    x = np.ones((65536, 4), np.int8)
-    got, want = _both(a, x, UP)
-    assert bits_equal(got, want)
-    with pytest.raises(RuntimeError):
-        grgl_amd.matmul(_pg(a), np.ones((65537, 4), np.int8), UP)
+    rng = np.random.default_rng(9)
+    for rows in (65536, 65537, 140001):
+        x = rng.integers(-100, 100, size=(rows, 4)).astype(np.int8)
+        got, want = _both(a, x, UP)
+        assert bits_equal(got, want), rows
     with pytest.raises(RuntimeError):
         grgl_amd.matmul(_pg(a), np.ones((0, 4), np.int8), UP)
+
+
+def test_many_rows_with_init_and_miss(gpu):
+    """Row blocks slice the init vector / matrix and the miss matrix with the input."""
+    from helpers import random_grg
+
+    a = random_grg(51, num_samples=20, num_internal=30, num_mutations=25, with_miss=True, with_coal=True)
+    g = _pg_miss(a)
+    og = oracle.OracleGraph(a)
+    rng = np.random.default_rng(10)
+    K = 70000
+    for direction, od in ((UP, oracle.UP), (DOWN, oracle.DOWN)):
+        cols = g.num_samples if direction == UP else g.num_mutations
+        x = make_input(rng, np.int32, (K, cols))
+        for init in (make_input(rng, np.int32, (K,)), make_input(rng, np.int32, (K, g.num_nodes)), "xtx"):
+            assert bits_equal(grgl_amd.matmul(g, x, direction, init=init), og.matmul(x, od, init=init))
+        m1 = make_input(rng, np.int32, (K, g.num_mutations))
+        m2 = m1.copy()
+        assert bits_equal(grgl_amd.matmul(g, x, direction, miss=m1), og.matmul(x, od, miss=m2))
+        assert bits_equal(m1, m2)
+
+
+def test_many_rows_device_pointers(gpu):
+    import torch
+
+    from grgl_amd import capi
+    from helpers import random_grg
+
+    a = random_grg(52, num_samples=30, num_internal=60, heavy=(70,), num_mutations=40)
+    cg = capi.Graph(a)
+    og = oracle.OracleGraph(a)
+    rng = np.random.default_rng(11)
+    K = 66000
+    for direction, od, cin, cout in ((capi.UP, oracle.UP, cg.num_samples, cg.num_mutations),
+                                     (capi.DOWN, oracle.DOWN, cg.num_mutations, cg.num_samples)):
+        xh = make_input(rng, np.float64, (K, cin))
+        x = torch.from_numpy(xh).cuda()
+        y = torch.zeros((K, cout), dtype=torch.float64, device="cuda")
+        cg.matmul_ptr(capi.F64, x.data_ptr(), cin, K, direction, y.data_ptr(), y.numel(),
+                      flags=capi.FLAG_ORDER_PRESERVING)
+        torch.cuda.synchronize()
+        assert bits_equal(y.cpu().numpy(), og.matmul(xh, od))
+    cg.close()
+
+
+def _pg_miss(a):
+    return grgl_amd.grg_from_arrays(a["num_samples"], a["ploidy"], a["down_offsets"], a["down_children"],
+                                    a["mutation_nodes"], mutation_miss_nodes=a.get("mutation_miss_nodes"),
+                                    coal_counts=a.get("coal_counts"))
 
 
 @pytest.mark.parametrize("ploidy", [3, 4])
