@@ -206,7 +206,7 @@ __device__ __forceinline__ void pull_heavy(const PullArgs& a, uint32_t row, uint
         // are issued before the adds, which run step by step, so each partial's order is
         // unchanged.  The last, partial iteration loads clamped (in-list) entries and adds
         // only the real ones.
-        constexpr int STEPS = PPT >= 4 ? 1 : (VB * PPT <= 4 ? 32 : (VB * PPT <= 8 ? 16 : (VB <= 16 ? 4 / PPT : 4)));
+        constexpr int STEPS = PPT >= 4 ? 1 : (VB * PPT <= 4 ? 16 : (VB * PPT <= 8 ? 8 : (VB <= 16 ? 4 / PPT : 4)));
         constexpr uint32_t STRIDE = (uint32_t)(STEPS * PPT) * A;
         uint32_t i = b + team;
         for (; i + STRIDE - A < e; i += STRIDE) {
