@@ -645,11 +645,12 @@ def main():
     log(f"batched step: {ms_per_step:.3f} ms, {value / 1e9:.1f} G edge*RHS/s, "
         f"{step_alg_bytes / (t_total / args.steps) / 1e9:.0f} GB/s algorithmic")
 
-    # Plain path: one request per traversal, f32, each K of the ladder.
+    # Plain path: one request per traversal, each K of the ladder, f32 (and i32 beside it).
     plain = {}
+    plain_i32 = {}
     plain_ms_total = 0.0
     for cdt, d, cin, cout, reqs in work:
-        if cdt != capi.F32 or args.no_plain:
+        if args.no_plain:
             continue
         name = "UP" if d == capi.UP else "DOWN"
         for K, x, y in reqs:
@@ -663,12 +664,16 @@ def main():
                 cg.matmul_ptr(cdt, x.data_ptr(), cin, K, d, y.data_ptr(), y.numel(), stream=sp)
             torch.cuda.synchronize()
             ms = (time.perf_counter() - t) / reps * 1e3
-            plain_ms_total += ms
-            plain[f"{name}_K{K}"] = {
+            rec = {
                 "ms": round(ms, 4),
                 "edge_rhs_per_s": E * K / (ms / 1e3),
                 "alg_GBps": algorithmic_bytes(V, E, K, 4, cin, cout) / (ms / 1e3) / 1e9,
             }
+            if cdt == capi.F32:
+                plain_ms_total += ms
+                plain[f"{name}_K{K}"] = rec
+            else:
+                plain_i32[f"{name}_K{K}"] = rec
     batched_f32_ms = None
     # Batched f32 UP+DOWN alone, to compare with the plain per-K sum on equal work.
     f32_work = [w for w in work if w[0] == capi.F32]
@@ -746,6 +751,7 @@ def main():
         },
         "step_alg_GBps": step_alg_bytes / (t_total / args.steps) / 1e9,
         "plain": plain,
+        "plain_i32": plain_i32,
         "batched_vs_plain_f32": round(plain_ms_total / batched_f32_ms, 3) if plain_ms_total else None,
         "roofline": roofline,
         "plain_100k_x_500k": at_scale,
