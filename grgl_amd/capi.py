@@ -58,6 +58,9 @@ EXPORTED_SYMBOLS = [
     "grgl_hip_matmul_batch",
     "grgl_hip_set_tuning",
     "grgl_hip_autotune",
+    "grgl_hip_get_tuning",
+    "grgl_hip_tuning_save",
+    "grgl_hip_tuning_load",
     "grgl_hip_kernel_timing",
     "grgl_hip_kernel_timing_read",
     "grgl_hip_server_create",
@@ -182,6 +185,12 @@ def lib():
     if hasattr(L, "grgl_hip_autotune"):
         L.grgl_hip_autotune.restype = ci
         L.grgl_hip_autotune.argtypes = [vp, ci, ci, u32, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_float)]
+        L.grgl_hip_get_tuning.restype = ci
+        L.grgl_hip_get_tuning.argtypes = [vp, ci, ci, u32, ctypes.POINTER(Tuning)]
+        L.grgl_hip_tuning_save.restype = ci
+        L.grgl_hip_tuning_save.argtypes = [vp, ctypes.c_char_p]
+        L.grgl_hip_tuning_load.restype = ci
+        L.grgl_hip_tuning_load.argtypes = [vp, ctypes.c_char_p]
     if hasattr(L, "grgl_hip_server_create"):
         L.grgl_hip_server_create.restype = ci
         L.grgl_hip_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
@@ -316,6 +325,18 @@ class Graph:
         check(lib().grgl_hip_autotune(self.handle, direction, dtype, rows, ctypes.byref(t), ctypes.byref(ms)))
         return {"team_lanes": t.team_lanes, "heavy_fanin": t.heavy_fanin, "use_graph": t.use_graph,
                 "xcd_map": t.xcd_map, "max_launch_entries": t.max_launch_entries}, ms.value
+
+    def get_tuning(self, direction, dtype, rows):
+        t = Tuning()
+        check(lib().grgl_hip_get_tuning(self.handle, direction, dtype, rows, ctypes.byref(t)))
+        return {"team_lanes": t.team_lanes, "heavy_fanin": t.heavy_fanin, "use_graph": t.use_graph,
+                "xcd_map": t.xcd_map, "max_launch_entries": t.max_launch_entries}
+
+    def save_tuning(self, path):
+        check(lib().grgl_hip_tuning_save(self.handle, os.fsencode(path)))
+
+    def load_tuning(self, path):
+        check(lib().grgl_hip_tuning_load(self.handle, os.fsencode(path)))
 
     def kernel_timing(self, enable):
         check(lib().grgl_hip_kernel_timing(self.handle, int(enable)))

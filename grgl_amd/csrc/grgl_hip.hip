@@ -2,7 +2,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <stdexcept>
 #include <memory>
 
@@ -1246,8 +1249,132 @@ GRGL_EXPORT grgl_status grgl_hip_set_tuning(grgl_graph* g, grgl_direction direct
         DeviceDirPlan& plan = direction == GRGL_DIRECTION_UP ? g->up : g->down;
         if (tuning->heavy_fanin) plan.heavyFanin = tuning->heavy_fanin;
         if (tuning->max_launch_entries) plan.chunkEntries = std::min<uint64_t>(tuning->max_launch_entries, kChunkEntries);
+        GRGL_HIP_CHECK(hipSetDevice(g->device)); // computeHeavy uploads the heavy counts
         plan.computeHeavy();
     }
+    GRGL_API_END
+}
+
+GRGL_EXPORT grgl_status grgl_hip_get_tuning(grgl_graph* g, grgl_direction direction, grgl_dtype dtype, uint32_t rows,
+                                            grgl_tuning* out) {
+    GRGL_API_BEGIN
+    if (!g || !out) throw MisuseError("null argument");
+    if (direction != GRGL_DIRECTION_UP && direction != GRGL_DIRECTION_DOWN) throw MisuseError("Invalid traversal direction");
+    if (rows == 0) throw MisuseError("rows must be positive");
+    const TypeClass tc = typeClassOf(dtype);
+    const Tuning t = chooseTuning(g, direction, tc, rows, rowGeometry(rows, dtypeSize(dtype)));
+    std::lock_guard<std::mutex> lk(g->mu);
+    const DeviceDirPlan& plan = direction == GRGL_DIRECTION_UP ? g->up : g->down;
+    out->team_lanes = t.teamLanes;
+    out->heavy_fanin = plan.heavyFanin;
+    out->use_graph = t.useGraph;
+    out->xcd_map = t.xcdMap;
+    out->max_launch_entries = (uint32_t)plan.chunkEntries;
+    GRGL_API_END
+}
+
+namespace {
+
+// Fingerprint of what tuning depends on: sizes, level boundaries and list lengths of both
+// directions (FNV-1a, 64 bit).
+uint64_t tuningFingerprint(const grgl_graph* g) {
+    uint64_t h = 1469598103934665603ULL;
+    auto mix = [&h](uint64_t v) {
+        for (int i = 0; i < 8; i++) {
+            h ^= (v >> (8 * i)) & 0xff;
+            h *= 1099511628211ULL;
+        }
+    };
+    mix(g->V);
+    mix(g->E);
+    mix(g->N);
+    mix(g->M);
+    for (const DeviceDirPlan* p : {&g->up, &g->down}) {
+        mix(p->numLevels);
+        for (uint32_t x : p->levelStart) mix(x);
+        for (uint32_t x : p->fanIn) mix(x);
+    }
+    return h;
+}
+
+const char* kTuningMagic = "grgl_hip_tuning";
+
+} // namespace
+
+GRGL_EXPORT grgl_status grgl_hip_tuning_save(grgl_graph* g, const char* path) {
+    GRGL_API_BEGIN
+    if (!g || !path) throw MisuseError("null argument");
+    std::ostringstream os;
+    os << kTuningMagic << " 1\n";
+    char fp[32];
+    std::snprintf(fp, sizeof(fp), "%016llx", (unsigned long long)tuningFingerprint(g));
+    os << "graph " << fp << "\n";
+    {
+        std::lock_guard<std::mutex> lk(g->mu);
+        os << "direction " << (int)GRGL_DIRECTION_UP << " " << g->up.heavyFanin << " " << g->up.chunkEntries << "\n";
+        os << "direction " << (int)GRGL_DIRECTION_DOWN << " " << g->down.heavyFanin << " " << g->down.chunkEntries << "\n";
+        for (const auto& kv : g->tuning) {
+            const Tuning& t = kv.second;
+            os << "entry " << std::get<0>(kv.first) << " " << std::get<1>(kv.first) << " " << std::get<2>(kv.first) << " "
+               << t.teamLanes << " " << t.useGraph << " " << t.xcdMap << "\n";
+        }
+    }
+    std::ofstream f(path, std::ios::trunc);
+    if (!f) throw MisuseError(std::string("cannot write tuning file ") + path);
+    f << os.str();
+    if (!f.good()) throw MisuseError(std::string("cannot write tuning file ") + path);
+    GRGL_API_END
+}
+
+GRGL_EXPORT grgl_status grgl_hip_tuning_load(grgl_graph* g, const char* path) {
+    GRGL_API_BEGIN
+    if (!g || !path) throw MisuseError("null argument");
+    std::ifstream f(path);
+    if (!f) throw MisuseError(std::string("cannot read tuning file ") + path);
+    std::string magic, word, fp;
+    int version = 0;
+    if (!(f >> magic >> version) || magic != kTuningMagic || version != 1) throw MisuseError("not a tuning file");
+    if (!(f >> word >> fp) || word != "graph") throw MisuseError("malformed tuning file");
+    char want[32];
+    std::snprintf(want, sizeof(want), "%016llx", (unsigned long long)tuningFingerprint(g));
+    if (fp != want) throw MisuseError("tuning file was written for a graph of another structure");
+    // Parse everything before applying anything.
+    struct Dir { int dir; uint64_t heavy, chunk; };
+    std::vector<Dir> dirs;
+    std::vector<std::pair<std::tuple<int, int, uint32_t>, Tuning>> entries;
+    while (f >> word) {
+        if (word == "direction") {
+            Dir d{};
+            if (!(f >> d.dir >> d.heavy >> d.chunk) || (d.dir != GRGL_DIRECTION_UP && d.dir != GRGL_DIRECTION_DOWN) ||
+                d.heavy == 0 || d.heavy > 0xffffffffULL || d.chunk == 0 || d.chunk > kChunkEntries)
+                throw MisuseError("malformed tuning file (direction)");
+            dirs.push_back(d);
+        } else if (word == "entry") {
+            int dir = 0, tc = 0;
+            uint64_t rows = 0, tw = 0, ug = 0, xcd = 0;
+            if (!(f >> dir >> tc >> rows >> tw >> ug >> xcd) || (dir != GRGL_DIRECTION_UP && dir != GRGL_DIRECTION_DOWN) ||
+                tc < 0 || tc > (int)kX8 || rows == 0 || rows > 65536 || tw > 64 || ug > 1 || xcd > 2)
+                throw MisuseError("malformed tuning file (entry)");
+            Tuning t;
+            t.teamLanes = (uint32_t)tw;
+            t.useGraph = (uint32_t)ug;
+            t.xcdMap = (uint32_t)xcd;
+            entries.push_back({std::make_tuple(dir, tc, (uint32_t)rows), t});
+        } else {
+            throw MisuseError("malformed tuning file");
+        }
+    }
+    GRGL_HIP_CHECK(hipSetDevice(g->device)); // computeHeavy uploads the heavy counts
+    std::lock_guard<std::mutex> lk(g->mu);
+    for (const Dir& d : dirs) {
+        DeviceDirPlan& plan = d.dir == GRGL_DIRECTION_UP ? g->up : g->down;
+        if (plan.heavyFanin != (uint32_t)d.heavy || plan.chunkEntries != d.chunk) {
+            plan.heavyFanin = (uint32_t)d.heavy;
+            plan.chunkEntries = d.chunk;
+            plan.computeHeavy();
+        }
+    }
+    for (const auto& e : entries) g->tuning[e.first] = e.second;
     GRGL_API_END
 }
 

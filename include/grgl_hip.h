@@ -175,6 +175,18 @@ grgl_status grgl_hip_set_tuning(grgl_graph* graph, grgl_direction direction, grg
 /* Time candidate tunings on the device and keep the fastest; writes it to *chosen. */
 grgl_status grgl_hip_autotune(grgl_graph* graph, grgl_direction direction, grgl_dtype dtype, uint32_t rows,
                               grgl_tuning* chosen, float* best_ms);
+/* The tuning a call of (direction, dtype, rows) uses now: the stored choice with library
+ * defaults filled in (team_lanes as launched, use_graph, xcd_map) and the direction's
+ * heavy_fanin and max_launch_entries. */
+grgl_status grgl_hip_get_tuning(grgl_graph* graph, grgl_direction direction, grgl_dtype dtype, uint32_t rows,
+                                grgl_tuning* out);
+/* Tuning cache file (text).  save writes every stored (direction, dtype class, rows) choice
+ * and each direction's heavy_fanin / max_launch_entries with a fingerprint of the graph's
+ * level structure; load applies a file written for a graph of the same structure and fails
+ * with GRGL_ERR_MISUSE for any other graph or a malformed file.  Autotuning once and loading
+ * the file in later processes skips the timing runs. */
+grgl_status grgl_hip_tuning_save(grgl_graph* graph, const char* path);
+grgl_status grgl_hip_tuning_load(grgl_graph* graph, const char* path);
 
 /*
  * Per-launch timing of the level (pull-gather) kernels.  When enabled, each level launch

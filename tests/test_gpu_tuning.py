@@ -133,3 +133,46 @@ def test_split_levels_give_the_same_bits(gpu, name):
                             assert bits_equal(got.cpu().numpy(), ref), (direction, K, chunk, use_graph, rep)
                 cg.set_tuning(direction, dt, K, max_launch_entries=1 << 30)
     cg.close()
+
+
+def test_tuning_cache_file(gpu, tmp_path):
+    """Autotuned choices saved to a file and loaded into a new handle of the same graph give
+    the same tuning (and results); a graph of another structure refuses the file."""
+    import torch
+
+    arrays = _graphs()["tree"]
+    cg = capi.Graph(arrays)
+    chosen_up, _ = cg.autotune(capi.UP, capi.F32, 300)
+    chosen_down, _ = cg.autotune(capi.DOWN, capi.I32, 7)
+    cg.set_tuning(capi.DOWN, capi.F64, 40, team_lanes=2, use_graph=1, heavy_fanin=100, max_launch_entries=5000)
+    path = str(tmp_path / "tree.tuning")
+    cg.save_tuning(path)
+    want = {key: cg.get_tuning(*key) for key in ((capi.UP, capi.F32, 300), (capi.DOWN, capi.I32, 7),
+                                                 (capi.DOWN, capi.F64, 40))}
+    assert want[(capi.UP, capi.F32, 300)]["team_lanes"] == chosen_up["team_lanes"]
+    assert want[(capi.DOWN, capi.F64, 40)]["heavy_fanin"] == 100
+    assert want[(capi.DOWN, capi.F64, 40)]["max_launch_entries"] == 5000
+
+    fresh = capi.Graph(arrays)
+    assert fresh.get_tuning(capi.DOWN, capi.F64, 40)["heavy_fanin"] == 64
+    fresh.load_tuning(path)
+    for key, t in want.items():
+        assert fresh.get_tuning(*key) == t, key
+    # same results on both handles after loading
+    rng = np.random.default_rng(4)
+    stream = torch.cuda.current_stream().cuda_stream
+    x = torch.from_numpy(make_input(rng, np.float64, (40, cg.num_mutations))).cuda()
+    a = _run(cg, capi.F64, x, 40, capi.DOWN, cg.num_mutations, cg.num_samples, stream)
+    b = _run(fresh, capi.F64, x, 40, capi.DOWN, cg.num_mutations, cg.num_samples, stream)
+    torch.cuda.synchronize()
+    assert bits_equal(a.cpu().numpy(), b.cpu().numpy())
+
+    other = capi.Graph(_graphs()["heavy"])
+    with pytest.raises(RuntimeError):
+        other.load_tuning(path)
+    bad = tmp_path / "bad.tuning"
+    bad.write_text("grgl_hip_tuning 1\ngraph 0000\n")
+    with pytest.raises(RuntimeError):
+        fresh.load_tuning(str(bad))
+    for h in (cg, fresh, other):
+        h.close()
