@@ -15,8 +15,10 @@
 //   missingness node per mutation (u32) if header flag 0x4
 #include <algorithm>
 #include <cstring>
+#include <exception>
 #include <fstream>
 #include <stdexcept>
+#include <thread>
 
 #include "host_graph.h"
 
@@ -118,6 +120,31 @@ uint32_t getVarint(const uint8_t* p, size_t avail, size_t& used) {
     return 0;
 }
 
+// Runs body(lo, hi) over [0, n) in contiguous ranges on up to 16 threads (one for small n);
+// the first exception thrown by any range is rethrown after all threads finish.
+template <typename F> void parallelRanges(uint64_t n, F body) {
+    unsigned threads = std::min(16U, std::max(1U, std::thread::hardware_concurrency()));
+    if (n < (1U << 16)) threads = 1;
+    if (threads == 1) {
+        body(0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    std::vector<std::exception_ptr> errors(threads);
+    for (unsigned t = 0; t < threads; t++) {
+        pool.emplace_back([&, t] {
+            try {
+                body(n * t / threads, n * (t + 1) / threads);
+            } catch (...) {
+                errors[t] = std::current_exception();
+            }
+        });
+    }
+    for (auto& th : pool) th.join();
+    for (auto& e : errors)
+        if (e) std::rethrow_exception(e);
+}
+
 } // namespace
 
 HostGrg readGrgFile(const std::string& path) {
@@ -144,30 +171,43 @@ HostGrg readGrgFile(const std::string& path) {
     } else {
         index = r.array<uint64_t>(V + 1);
     }
+    // Two passes over node ranges in parallel (each list is self-contained at its byte
+    // offset): list lengths, then the lists decoded into place.
     g.downOff.assign(V + 1, 0);
-    g.downChild.reserve(h.edgeCount);
-    for (uint64_t n = 0; n < V; n++) {
-        const uint64_t b = index[n], e = index[n + 1];
-        if (e < b || e > edgeBytes) fail("node index");
-        if (e > b) {
+    parallelRanges(V, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t n = lo; n < hi; n++) {
+            const uint64_t b = index[n], e = index[n + 1];
+            if (e < b || e > edgeBytes) fail("node index");
+            if (e > b) {
+                size_t used = 0;
+                g.downOff[n + 1] = getVarint(edges.data() + b, (size_t)(e - b), used);
+            }
+        }
+    });
+    for (uint64_t n = 0; n < V; n++) g.downOff[n + 1] += g.downOff[n];
+    if (g.downOff[V] != h.edgeCount) fail("edge count does not match the header");
+    g.downChild.resize(h.edgeCount);
+    parallelRanges(V, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t n = lo; n < hi; n++) {
+            const uint64_t b = index[n], e = index[n + 1];
+            if (e == b) continue;
             size_t used = 0;
             const uint8_t* p = edges.data() + b;
             size_t avail = (size_t)(e - b);
             const uint32_t count = getVarint(p, avail, used);
             p += used;
             avail -= used;
+            uint32_t* out = g.downChild.data() + g.downOff[n];
             uint32_t prev = 0;
             for (uint32_t i = 0; i < count; i++) {
                 const uint32_t d = getVarint(p, avail, used);
                 p += used;
                 avail -= used;
                 prev += d;
-                g.downChild.push_back(prev);
+                out[i] = prev;
             }
         }
-        g.downOff[n + 1] = g.downChild.size();
-    }
-    if (g.downChild.size() != h.edgeCount) fail("edge count does not match the header");
+    });
     // Mutations.
     g.mutNode = r.array<uint32_t>(M);
     const std::vector<MutationRecord> recs = r.array<MutationRecord>(M);
