@@ -72,4 +72,39 @@ def test_more_than_2_32_gather_entries(gpu):
     wantf = base[:, None] + extra
     scale = np.abs(w).sum(axis=1)[:, None]
     assert (np.abs(downf - wantf) <= 1e-6 * scale).all()
+    _timing(g, N, M)
     _say(f"done; peak host memory {resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 1e6:.1f} GB")
+
+
+def _timing(g, N, M):
+    """Device-resident traversals on the same graph (reported, not asserted): at this size a
+    traversal streams 4 bytes of index per edge from HBM."""
+    import time
+
+    import torch
+
+    from grgl_amd import capi
+
+    L = capi.lib()
+    handle = g.device_handle()
+    stream = torch.cuda.current_stream().cuda_stream
+    E = g.num_edges
+    for direction, name, cin, cout in ((UP, "UP", N, M), (DOWN, "DOWN", M, N)):
+        for K in (1, 8):
+            x = torch.randn((K, cin), device="cuda")
+            y = torch.zeros((K, cout), device="cuda")
+
+            def call():
+                capi.check(L.grgl_hip_matmul(handle, capi.F32, x.data_ptr(), cin, K, int(direction), y.data_ptr(),
+                                             y.numel(), 0, 0, None, capi.INIT_ZERO, None, 0, stream))
+
+            call()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(3):
+                call()
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t) / 3 * 1e3
+            entries = E + (M if direction == DOWN else 0)
+            _say(f"timing {name} K={K}: {ms:.2f} ms, {E * K / ms / 1e9:.3f}e12 edge*RHS/s, "
+                 f"index stream {4 * entries / ms / 1e9:.2f} TB/s")
