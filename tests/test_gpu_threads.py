@@ -89,3 +89,43 @@ def test_threads_same_graph(gpu):
     for (x, d), out in zip(calls, together):
         if x.dtype.kind == "i":
             assert bits_equal(out, og.matmul(x, oracle.UP if d == UP else oracle.DOWN))
+
+
+def test_threads_device_pointers_on_own_streams(gpu):
+    """Device-pointer calls on one graph from several threads, each on its own HIP stream
+    (the library keeps a workspace per stream), equal the same calls made one by one."""
+    import torch
+
+    from grgl_amd import capi
+
+    _, arrays = _graph()
+    cg = capi.Graph(arrays)
+    rng = np.random.default_rng(9)
+    jobs = []
+    for i in range(24):
+        direction = capi.UP if i % 2 == 0 else capi.DOWN
+        K = [1, 4, 16, 100][i % 4]
+        cin = cg.num_samples if direction == capi.UP else cg.num_mutations
+        cout = cg.num_mutations if direction == capi.UP else cg.num_samples
+        x = torch.from_numpy(make_input(rng, np.float32, (K, cin))).cuda()
+        jobs.append((direction, K, cin, cout, x))
+    torch.cuda.synchronize()
+
+    def run(job, stream):
+        direction, K, cin, cout, x = job
+        with torch.cuda.stream(stream):  # the zero fill is ordered before the traversal
+            y = torch.zeros((K, cout), dtype=torch.float32, device="cuda")
+        cg.matmul_ptr(capi.F32, x.data_ptr(), cin, K, direction, y.data_ptr(), y.numel(),
+                      stream=stream.cuda_stream)
+        stream.synchronize()
+        return y
+
+    default = torch.cuda.current_stream()
+    alone = [run(j, default).cpu().numpy() for j in jobs]
+    streams = [torch.cuda.Stream() for _ in range(6)]
+    with cf.ThreadPoolExecutor(6) as ex:
+        futs = [ex.submit(run, j, streams[i % 6]) for i, j in enumerate(jobs)]
+        together = [f.result().cpu().numpy() for f in futs]
+    for i, (a, b) in enumerate(zip(alone, together)):
+        assert bits_equal(a, b), i
+    cg.close()
