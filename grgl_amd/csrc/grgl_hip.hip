@@ -4,9 +4,11 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
+#include <thread>
 #include <memory>
 
 #include "device_graph.h"
@@ -943,8 +945,24 @@ GRGL_EXPORT grgl_status grgl_hip_graph_create(const grgl_graph_desc* desc, int d
     }
     if ((uint64_t)h.numNodes() + 2ULL * h.numMutations() >= 0x7fffffffULL)
         throw MisuseError("graph too large: nodes + 2*mutations must stay below 2^31");
-    DirPlan up = buildUpPlan(h);
-    DirPlan down = buildDownPlan(h);
+    // The two directions' plans are independent: build DOWN on a second thread.
+    DirPlan up, down;
+    std::exception_ptr downError;
+    std::thread downThread([&] {
+        try {
+            down = buildDownPlan(h);
+        } catch (...) {
+            downError = std::current_exception();
+        }
+    });
+    try {
+        up = buildUpPlan(h);
+    } catch (...) {
+        downThread.join();
+        throw;
+    }
+    downThread.join();
+    if (downError) std::rethrow_exception(downError);
     int ndev = 0;
     GRGL_HIP_CHECK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) throw MisuseError("invalid device ordinal " + std::to_string(device));
