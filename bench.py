@@ -350,6 +350,31 @@ def options_benchmark(cg, N, M, V, E, P=2, K=16):
     return out
 
 
+def host_path_benchmark(g, E):
+    """pygrgl-style calls with host numpy input and output (grgl_amd.matmul), f32: the time per
+    call including the staging copies over PCIe, beside the device-resident traversal."""
+    import numpy as np
+
+    import grgl_amd
+
+    rng = np.random.default_rng(SEED)
+    out = {}
+    for d, name, cin, cout in ((grgl_amd.UP, "UP", g.num_samples, g.num_mutations),
+                               (grgl_amd.DOWN, "DOWN", g.num_mutations, g.num_samples)):
+        for K in (1, 16, 256):
+            x = rng.standard_normal((K, cin)).astype(np.float32)
+            grgl_amd.matmul(g, x, d)
+            reps = 5 if K < 256 else 3
+            t = time.perf_counter()
+            for _ in range(reps):
+                grgl_amd.matmul(g, x, d)
+            ms = (time.perf_counter() - t) / reps * 1e3
+            moved = 4 * K * (cin + cout)
+            out[f"{name}_K{K}"] = {"ms": round(ms, 3), "edge_rhs_per_s": E * K / (ms / 1e3),
+                                   "host_bytes": moved, "host_GBps": round(moved / (ms / 1e3) / 1e9, 2)}
+    return out
+
+
 def grg_io_benchmark(g):
     """Write and read the benchmark graph as a .grg file (format 5.3) in a temporary directory."""
     import tempfile
@@ -716,8 +741,8 @@ def main():
     extras = None
     if not args.no_extras:
         log("options, .grg file and construction ...")
-        extras = {"options_K16_f32": options_benchmark(cg, N, M, V, E), "grg_io": grg_io_benchmark(g),
-                  "construction": construction_benchmark()}
+        extras = {"options_K16_f32": options_benchmark(cg, N, M, V, E), "host_path_f32": host_path_benchmark(g, E),
+                  "grg_io": grg_io_benchmark(g), "construction": construction_benchmark()}
         log(f"extras: {json.dumps(extras)}")
     cpu = None
     if not args.no_cpu_baseline:

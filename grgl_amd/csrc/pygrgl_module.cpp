@@ -187,8 +187,13 @@ py::array dispatchMult(PyGRG& grg, grgl_dtype dt, const py::array& arr, size_t r
                        Direction direction, bool emitAllNodes, bool byIndividual, py::handle init, py::handle miss,
                        uint32_t extraFlags) {
     const size_t outSize = rows * outCols;
+    // The reference zero-fills the result and accumulates into it (_grgl.cpp:112-114).  The
+    // device path writes every output element as 0 + value, which is the same, so the result
+    // is filled only where the call accumulates anyway: UP with `miss`, whose missingness
+    // output is accumulated into the caller's array.
+    const bool accumulate = !miss.is_none() && direction == Direction::UP;
     py::array result(numpyDtype(dt), std::vector<py::ssize_t>{(py::ssize_t)rows, (py::ssize_t)outCols});
-    std::memset(result.mutable_data(), 0, outSize * result.itemsize());
+    if (accumulate) std::memset(result.mutable_data(), 0, outSize * result.itemsize());
 
     py::array initArr;
     const void* initPtr = nullptr;
@@ -245,7 +250,7 @@ py::array dispatchMult(PyGRG& grg, grgl_dtype dt, const py::array& arr, size_t r
         py::gil_scoped_release release;
         st = grgl_hip_matmul(g, dt, in, inCols, rows, (grgl_direction)direction, out, outSize, emitAllNodes ? 1 : 0,
                              byIndividual ? 1 : 0, initPtr, initMode, missPtr,
-                             GRGL_FLAG_HOST_PTRS | GRGL_FLAG_ACCUMULATE | extraFlags, nullptr);
+                             GRGL_FLAG_HOST_PTRS | (accumulate ? GRGL_FLAG_ACCUMULATE : 0U) | extraFlags, nullptr);
     }
     checkStatus(st);
     return result;
@@ -282,8 +287,7 @@ py::array dotProduct(PyGRGPtr& grg, py::handle input, Direction direction) {
     const grgl_dtype dt = dtypeOf(input, ok);
     API_CHECK(ok && (dt == GRGL_F64 || dt == GRGL_F32 || dt == GRGL_I64 || dt == GRGL_I32),
               "Unsupported numpy dtype: " << std::string(py::str(arr.dtype())));
-    py::array result(numpyDtype(dt), std::vector<py::ssize_t>{(py::ssize_t)outCols});
-    std::memset(result.mutable_data(), 0, outCols * result.itemsize());
+    py::array result(numpyDtype(dt), std::vector<py::ssize_t>{(py::ssize_t)outCols}); // every element written
     grgl_graph* g = grg->deviceGraph();
     const void* in = arr.data();
     void* out = result.mutable_data();
@@ -291,7 +295,7 @@ py::array dotProduct(PyGRGPtr& grg, py::handle input, Direction direction) {
     {
         py::gil_scoped_release release;
         st = grgl_hip_matmul(g, dt, in, cols, 1, (grgl_direction)direction, out, outCols, 0, 0, nullptr, GRGL_INIT_ZERO,
-                             nullptr, GRGL_FLAG_HOST_PTRS | GRGL_FLAG_ACCUMULATE, nullptr);
+                             nullptr, GRGL_FLAG_HOST_PTRS, nullptr);
     }
     checkStatus(st);
     return result;
