@@ -74,6 +74,8 @@ EXPORTED_SYMBOLS = [
     "grgl_hip_server_destroy",
     "grgl_hip_last_error",
     "grgl_hip_device_count",
+    "grgl_hip_host_alloc",
+    "grgl_hip_host_free",
 ]
 
 
@@ -165,6 +167,10 @@ def lib():
     L.grgl_hip_last_error.argtypes = []
     L.grgl_hip_device_count.restype = ci
     L.grgl_hip_device_count.argtypes = []
+    L.grgl_hip_host_alloc.restype = ci
+    L.grgl_hip_host_alloc.argtypes = [sz, ctypes.POINTER(vp)]
+    L.grgl_hip_host_free.restype = ci
+    L.grgl_hip_host_free.argtypes = [vp]
     L.grgl_hip_graph_create.restype = ci
     L.grgl_hip_graph_create.argtypes = [ctypes.POINTER(GraphDesc), ci, ctypes.POINTER(vp)]
     L.grgl_hip_graph_destroy.restype = ci

@@ -914,6 +914,21 @@ void freePlan(DeviceDirPlan& d) {
 
 GRGL_EXPORT const char* grgl_hip_last_error(void) { return g_lastError.c_str(); }
 
+GRGL_EXPORT grgl_status grgl_hip_host_alloc(size_t bytes, void** out) {
+    GRGL_API_BEGIN
+    if (!out) throw MisuseError("null argument");
+    *out = nullptr;
+    if (bytes == 0) throw MisuseError("zero-byte allocation");
+    GRGL_HIP_CHECK(hipHostMalloc(out, bytes, hipHostMallocDefault));
+    GRGL_API_END
+}
+
+GRGL_EXPORT grgl_status grgl_hip_host_free(void* ptr) {
+    GRGL_API_BEGIN
+    if (ptr) GRGL_HIP_CHECK(hipHostFree(ptr));
+    GRGL_API_END
+}
+
 GRGL_EXPORT int grgl_hip_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;

@@ -183,6 +183,70 @@ bool sameDtype(py::handle h, grgl_dtype dt) {
 }
 
 // dispatchMult (src/python/_grgl.cpp:100-185): zeroed result, init/miss parsing, GIL released.
+// Page-locked host blocks that back large results, so the device-to-host copy of the output
+// runs at link speed instead of through the runtime's pageable staging (UP K = 16 on a
+// 1M-mutation graph copies 64 MB back).  A result's block returns to the pool when numpy
+// frees the array; the pool keeps at most kMaxFree bytes of idle blocks, and at most
+// kMaxLive bytes are handed out (beyond that results are ordinary numpy memory).  The pool
+// is never destroyed (blocks may outlive the module's statics at interpreter exit).
+class PinnedPool {
+public:
+    static PinnedPool& get() {
+        static PinnedPool* pool = new PinnedPool();
+        return *pool;
+    }
+    static constexpr size_t kMinBytes = 1ULL << 20;
+    void* acquire(size_t bytes) {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = free_.lower_bound(bytes);
+        if (it != free_.end() && it->first <= 2 * bytes) {
+            void* p = it->second;
+            freeBytes_ -= it->first;
+            live_ += it->first;
+            free_.erase(it);
+            return p;
+        }
+        if (live_ + bytes > kMaxLive) return nullptr;
+        void* p = nullptr;
+        if (grgl_hip_host_alloc(bytes, &p) != GRGL_OK) return nullptr;
+        size_[p] = bytes;
+        live_ += bytes;
+        return p;
+    }
+    void release(void* p) {
+        std::lock_guard<std::mutex> lk(mu_);
+        const size_t b = size_[p];
+        live_ -= b;
+        if (freeBytes_ + b <= kMaxFree) {
+            free_.emplace(b, p);
+            freeBytes_ += b;
+        } else {
+            size_.erase(p);
+            (void)grgl_hip_host_free(p);
+        }
+    }
+
+private:
+    static constexpr size_t kMaxFree = 2ULL << 30, kMaxLive = 16ULL << 30;
+    std::mutex mu_;
+    std::multimap<size_t, void*> free_;
+    std::map<void*, size_t> size_;
+    size_t freeBytes_ = 0, live_ = 0;
+};
+
+// A C-contiguous array for a result; large ones live in the pinned pool.
+py::array resultArray(grgl_dtype dt, std::vector<py::ssize_t> shape, size_t elems) {
+    const py::dtype d = numpyDtype(dt);
+    const size_t bytes = elems * (size_t)d.itemsize();
+    if (bytes >= PinnedPool::kMinBytes && grgl_hip_device_count() > 0) {
+        if (void* p = PinnedPool::get().acquire(bytes)) {
+            py::capsule owner(p, [](void* q) { PinnedPool::get().release(q); });
+            return py::array(d, shape, p, owner);
+        }
+    }
+    return py::array(d, shape);
+}
+
 py::array dispatchMult(PyGRG& grg, grgl_dtype dt, const py::array& arr, size_t rows, size_t inCols, size_t outCols,
                        Direction direction, bool emitAllNodes, bool byIndividual, py::handle init, py::handle miss,
                        uint32_t extraFlags) {
@@ -192,7 +256,7 @@ py::array dispatchMult(PyGRG& grg, grgl_dtype dt, const py::array& arr, size_t r
     // is filled only where the call accumulates anyway: UP with `miss`, whose missingness
     // output is accumulated into the caller's array.
     const bool accumulate = !miss.is_none() && direction == Direction::UP;
-    py::array result(numpyDtype(dt), std::vector<py::ssize_t>{(py::ssize_t)rows, (py::ssize_t)outCols});
+    py::array result = resultArray(dt, {(py::ssize_t)rows, (py::ssize_t)outCols}, outSize);
     if (accumulate) std::memset(result.mutable_data(), 0, outSize * result.itemsize());
 
     py::array initArr;

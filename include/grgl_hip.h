@@ -259,6 +259,12 @@ const char* grgl_hip_last_error(void);
 /* Number of visible devices (0 when no GPU is present). */
 int grgl_hip_device_count(void);
 
+/* Page-locked host memory (hipHostMalloc) for callers that stage GRGL_FLAG_HOST_PTRS buffers:
+ * copies to and from it run at full link speed.  Free with grgl_hip_host_free.  No
+ * counterpart in the reference. */
+grgl_status grgl_hip_host_alloc(size_t bytes, void** out);
+grgl_status grgl_hip_host_free(void* ptr);
+
 #ifdef __cplusplus
 }
 #endif
