@@ -160,3 +160,33 @@ def test_polyploid_by_individual_and_xtx(gpu, ploidy, dtype):
 def _pg_coal(a):
     return grgl_amd.grg_from_arrays(a["num_samples"], a["ploidy"], a["down_offsets"], a["down_children"],
                                     a["mutation_nodes"], coal_counts=a.get("coal_counts"))
+
+
+def test_large_results_in_pinned_pool(gpu):
+    """Results of 1 MB or more come from a pool of page-locked blocks: they are ordinary,
+    writable numpy arrays, correct, and their blocks are reused after the arrays are freed
+    (values never leak from one result into the next, with or without accumulation)."""
+    import gc
+
+    g = grgl_amd.grg_from_coalescent(2000, 120000, 4, seed=3)
+    a = g.arrays()
+    og = oracle.OracleGraph(a)
+    rng = np.random.default_rng(12)
+    kept = []
+    for i in range(12):
+        K = [3, 5, 8][i % 3]
+        x = rng.integers(-9, 9, size=(K, g.num_samples)).astype(np.int32)
+        out = grgl_amd.matmul(g, x, UP)
+        assert out.nbytes >= 1 << 20 and out.flags.writeable and out.flags.c_contiguous
+        assert bits_equal(out, og.matmul(x, oracle.UP)), i
+        out[0, 0] += 1  # writable
+        if i % 2:
+            kept.append(out)
+        del out
+        gc.collect()
+    w = rng.integers(-9, 9, size=(4, g.num_mutations)).astype(np.int32)
+    miss = np.zeros((4, g.num_mutations), np.int32)
+    out = grgl_amd.matmul(g, rng.integers(0, 2, size=(4, g.num_samples)).astype(np.int32), UP, miss=miss)
+    assert out.shape == (4, g.num_mutations)
+    down = grgl_amd.matmul(g, w, DOWN)
+    assert bits_equal(down, og.matmul(w, oracle.DOWN))
