@@ -1056,6 +1056,57 @@ GRGL_EXPORT grgl_status grgl_hip_graph_info(const grgl_graph* g, grgl_graph_info
 
 namespace {
 
+// Host-to-device copy from pageable memory.  The runtime stages such copies through page-
+// locked buffers in the calling thread, so a large buffer is split over 2 or 4 threads, each
+// copying its slice on an internal stream that first waits for the work already on `s`;
+// `s` then waits for every slice.
+void uploadPageable(grgl_graph* g, void* dst, const void* src, size_t bytes, hipStream_t s) {
+    constexpr size_t kSlice = 16ULL << 20;
+    const unsigned T = bytes >= 4 * kSlice ? 4 : (bytes >= 2 * kSlice ? 2 : 1);
+    if (T == 1) {
+        GRGL_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+        return;
+    }
+    hipEvent_t before = nullptr;
+    GRGL_HIP_CHECK(hipEventCreateWithFlags(&before, hipEventDisableTiming));
+    std::vector<hipStream_t> st(T, nullptr);
+    std::vector<hipEvent_t> ev(T, nullptr);
+    std::vector<hipError_t> err(T, hipSuccess);
+    hipError_t first = hipEventRecord(before, s);
+    for (unsigned t = 0; t < T && first == hipSuccess; t++) {
+        st[t] = acquireInternalStream(g);
+        first = hipEventCreateWithFlags(&ev[t], hipEventDisableTiming);
+        if (first == hipSuccess) first = hipStreamWaitEvent(st[t], before, 0);
+    }
+    if (first == hipSuccess) {
+        std::vector<std::thread> threads;
+        for (unsigned t = 0; t < T; t++) {
+            threads.emplace_back([&, t] {
+                const size_t lo = bytes * t / T, hi = bytes * (t + 1) / T;
+                hipError_t e = hipSetDevice(g->device);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync((char*)dst + lo, (const char*)src + lo, hi - lo, hipMemcpyHostToDevice, st[t]);
+                if (e == hipSuccess) e = hipEventRecord(ev[t], st[t]);
+                err[t] = e;
+            });
+        }
+        for (auto& th : threads) th.join();
+        for (unsigned t = 0; t < T; t++) {
+            if (err[t] != hipSuccess && first == hipSuccess) first = err[t];
+            if (err[t] == hipSuccess) {
+                const hipError_t w = hipStreamWaitEvent(s, ev[t], 0);
+                if (w != hipSuccess && first == hipSuccess) first = w;
+            }
+        }
+    }
+    for (unsigned t = 0; t < T; t++) {
+        if (ev[t]) (void)hipEventDestroy(ev[t]);
+        if (st[t]) releaseInternalStream(g, st[t]);
+    }
+    (void)hipEventDestroy(before);
+    GRGL_HIP_CHECK(first);
+}
+
 // One traversal of up to 65536 input rows (grgl_hip_matmul splits larger inputs).  Shapes and
 // arguments are validated by the caller.
 void matmulRows(grgl_graph* g, grgl_dtype dtype, const void* input, size_t input_cols, uint32_t K,
@@ -1096,7 +1147,7 @@ void matmulRows(grgl_graph* g, grgl_dtype dtype, const void* input, size_t input
         void* dMiss = miss;
         const bool accumulate = (flags & GRGL_FLAG_ACCUMULATE) != 0;
         if (host) {
-            GRGL_HIP_CHECK(hipMemcpyAsync(base + inOff, input, inBytes, hipMemcpyHostToDevice, s));
+            uploadPageable(g, base + inOff, input, inBytes, s);
             dIn = base + inOff;
             dOut = base + outOff;
             if (accumulate) GRGL_HIP_CHECK(hipMemcpyAsync(dOut, output, outBytes, hipMemcpyHostToDevice, s));

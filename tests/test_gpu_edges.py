@@ -190,3 +190,16 @@ def test_large_results_in_pinned_pool(gpu):
     assert out.shape == (4, g.num_mutations)
     down = grgl_amd.matmul(g, w, DOWN)
     assert bits_equal(down, og.matmul(w, oracle.DOWN))
+
+
+def test_large_host_inputs_split_over_threads(gpu):
+    """Host inputs of 32 MB or more are uploaded in 2 or 4 slices from as many threads, each
+    on its own stream; the traversal waits for all of them (results bit-exact)."""
+    g = grgl_amd.grg_from_coalescent(2000, 120000, 4, seed=4)
+    og = oracle.OracleGraph(g.arrays())
+    rng = np.random.default_rng(13)
+    for K in (80, 150):  # 38 MB and 72 MB of int32 input
+        w = rng.integers(-50, 50, size=(K, g.num_mutations)).astype(np.int32)
+        assert w.nbytes >= 32 << 20
+        for _ in range(2):
+            assert bits_equal(grgl_amd.matmul(g, w, DOWN), og.matmul(w, oracle.DOWN)), K
