@@ -1056,15 +1056,26 @@ GRGL_EXPORT grgl_status grgl_hip_graph_info(const grgl_graph* g, grgl_graph_info
 
 namespace {
 
-// Host-to-device copy from pageable memory.  The runtime stages such copies through page-
-// locked buffers in the calling thread, so a large buffer is split over 2 or 4 threads, each
-// copying its slice on an internal stream that first waits for the work already on `s`;
-// `s` then waits for every slice.
-void uploadPageable(grgl_graph* g, void* dst, const void* src, size_t bytes, hipStream_t s) {
+// Copy between host and device memory where the host side may be pageable.  The runtime
+// stages pageable copies through page-locked buffers in the calling thread, so a large
+// pageable buffer is split over 2 or 4 threads, each copying its slice on an internal stream
+// that first waits for the work already on `s`; `s` then waits for every slice.
+bool pageLocked(const void* host) {
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, host) != hipSuccess) {
+        (void)hipGetLastError(); // pageable memory is unknown to the runtime
+        return false;
+    }
+    return attr.type == hipMemoryTypeHost;
+}
+
+void copyHostSplit(grgl_graph* g, void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s) {
     constexpr size_t kSlice = 16ULL << 20;
-    const unsigned T = bytes >= 4 * kSlice ? 4 : (bytes >= 2 * kSlice ? 2 : 1);
+    unsigned T = bytes >= 4 * kSlice ? 4 : (bytes >= 2 * kSlice ? 2 : 1);
+    // Page-locked host memory is copied directly at link speed: no split.
+    if (T > 1 && pageLocked(kind == hipMemcpyHostToDevice ? src : dst)) T = 1;
     if (T == 1) {
-        GRGL_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+        GRGL_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, kind, s));
         return;
     }
     hipEvent_t before = nullptr;
@@ -1084,8 +1095,7 @@ void uploadPageable(grgl_graph* g, void* dst, const void* src, size_t bytes, hip
             threads.emplace_back([&, t] {
                 const size_t lo = bytes * t / T, hi = bytes * (t + 1) / T;
                 hipError_t e = hipSetDevice(g->device);
-                if (e == hipSuccess)
-                    e = hipMemcpyAsync((char*)dst + lo, (const char*)src + lo, hi - lo, hipMemcpyHostToDevice, st[t]);
+                if (e == hipSuccess) e = hipMemcpyAsync((char*)dst + lo, (const char*)src + lo, hi - lo, kind, st[t]);
                 if (e == hipSuccess) e = hipEventRecord(ev[t], st[t]);
                 err[t] = e;
             });
@@ -1147,16 +1157,16 @@ void matmulRows(grgl_graph* g, grgl_dtype dtype, const void* input, size_t input
         void* dMiss = miss;
         const bool accumulate = (flags & GRGL_FLAG_ACCUMULATE) != 0;
         if (host) {
-            uploadPageable(g, base + inOff, input, inBytes, s);
+            copyHostSplit(g, base + inOff, input, inBytes, hipMemcpyHostToDevice, s);
             dIn = base + inOff;
             dOut = base + outOff;
-            if (accumulate) GRGL_HIP_CHECK(hipMemcpyAsync(dOut, output, outBytes, hipMemcpyHostToDevice, s));
+            if (accumulate) copyHostSplit(g, dOut, output, outBytes, hipMemcpyHostToDevice, s);
             if (initBytes) {
-                GRGL_HIP_CHECK(hipMemcpyAsync(base + initOff, init, initBytes, hipMemcpyHostToDevice, s));
+                copyHostSplit(g, base + initOff, init, initBytes, hipMemcpyHostToDevice, s);
                 dInit = base + initOff;
             }
             if (missBytes) {
-                GRGL_HIP_CHECK(hipMemcpyAsync(base + missOff, miss, missBytes, hipMemcpyHostToDevice, s));
+                copyHostSplit(g, base + missOff, miss, missBytes, hipMemcpyHostToDevice, s);
                 dMiss = base + missOff;
             }
         }
@@ -1171,9 +1181,9 @@ void matmulRows(grgl_graph* g, grgl_dtype dtype, const void* input, size_t input
         std::vector<TravRequest> reqs{{K, dIn, dOut, 0}};
         enqueueTraversal(g, dtype, direction, reqs, K, opt, base, s);
         if (host) {
-            GRGL_HIP_CHECK(hipMemcpyAsync(output, dOut, outBytes, hipMemcpyDeviceToHost, s));
+            copyHostSplit(g, output, dOut, outBytes, hipMemcpyDeviceToHost, s);
             if (missBytes && direction == GRGL_DIRECTION_UP)
-                GRGL_HIP_CHECK(hipMemcpyAsync(miss, dMiss, missBytes, hipMemcpyDeviceToHost, s));
+                copyHostSplit(g, miss, dMiss, missBytes, hipMemcpyDeviceToHost, s);
             GRGL_HIP_CHECK(hipStreamSynchronize(s));
         }
     } catch (...) {
