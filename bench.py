@@ -147,7 +147,7 @@ SERVE_K = [1, 8, 32, 128]
 SERVE_P = [0.4, 0.3, 0.2, 0.1]
 
 
-def serve_benchmark(num_requests=2000, load=0.6):
+def serve_benchmark(num_requests=2000, load=0.6, num_streams=2):
     """Resident graphs, mixed requests: continuous batching vs one request per traversal.
 
     Requests pick a graph uniformly, K from {1,8,32,128} (p .4/.3/.2/.1), a direction
@@ -203,7 +203,7 @@ def serve_benchmark(num_requests=2000, load=0.6):
                 for K in SERVE_K:
                     plain_one((gi, K, d, cdt))
     torch.cuda.synchronize()
-    srv = capi.Server(num_streams=2, max_batch_rows=512, hbm_budget_bytes=48 << 30)
+    srv = capi.Server(num_streams=num_streams, max_batch_rows=512, hbm_budget_bytes=48 << 30)
     gids = [srv.add_graph(cg) for cg in graphs]
 
     def served_submit(r):
