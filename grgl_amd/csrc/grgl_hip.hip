@@ -376,7 +376,8 @@ bool aligned16(const void* p) { return ((uintptr_t)p & 15U) == 0; }
 // Tile height: TK = min(64, pow2ceil(K)); TC = tile elements / TK.
 uint32_t tileShift(uint32_t K) { return std::min<uint32_t>(6, log2u(pow2ceil(K))); }
 
-uint32_t tileElems(size_t es) { return es == 8 ? 2048U : 4096U; }
+// Elements of the transpose tile (kTileElems<T> of the kernels).
+uint32_t tileElems(size_t es) { return es == 1 ? 4096U : 8192U / (uint32_t)es; }
 
 // Row-side 16-byte accesses need a 16-byte aligned row stride and request offset, and a
 // request at least 16 bytes wide: only such requests own whole 16-byte slots (stackOffsets
