@@ -13,7 +13,9 @@ Also reported:
   plain       one request per traversal for each K of the ladder (f32, UP and DOWN), the
               per-K edge*RHS/s and algorithmic GB/s, and the batched/plain speedup;
   roofline    the float32 level (pull-gather) kernel: algorithmic bytes per launch over
-              its HIP-event-timed duration on the launch stream, against 8 TB/s HBM;
+              its HIP-event-timed duration on the launch stream, against 8 TB/s HBM; beside
+              it the rocprofv3-measured HBM bytes per launch over the same duration and the
+              device's measured streaming read / copy rates (grgl_hip_stream_ceiling);
   cpu_baseline the reference's own CPU matmul (oracle/_ref) on a bounded sample of the
               same workload: all host cores (one independent call per thread) as value,
               one thread beside it.
@@ -713,6 +715,11 @@ def main():
     log(f"plain per-K f32 UP+DOWN total {plain_ms_total:.3f} ms vs batched {batched_f32_ms:.3f} ms")
 
     achieved = k_bytes / (k_ms / 1e3) / 1e9 if k_ms > 0 else 0.0
+    avg_launch_us = k_ms / max(1, k_launches) * 1e3
+    traffic = traffic_from_profile()
+    # The practical streaming limit of this GPU (one 16-byte access per thread), measured now.
+    ceiling = capi.stream_ceiling()
+    log(f"streaming ceiling: read {ceiling['read_GBps']:.0f} GB/s, copy {ceiling['copy_GBps']:.0f} GB/s")
     roofline = {
         "bound": "hbm",
         "kernel": "pull_level_kernel<float,16,*,false> (level gather, f32, 511 stacked RHS)",
@@ -720,9 +727,13 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic_from_profile(),
+        "traffic": traffic,
         "launches": k_launches,
-        "avg_launch_us": round(k_ms / max(1, k_launches) * 1e3, 2),
+        "avg_launch_us": round(avg_launch_us, 2),
+        # rocprofv3 HBM bytes per launch over the live average launch time
+        "traffic_GBps": round(traffic / (avg_launch_us * 1e3), 1) if traffic and k_launches else None,
+        "measured_read_ceiling_GBps": round(ceiling["read_GBps"], 1),
+        "measured_copy_ceiling_GBps": round(ceiling["copy_GBps"], 1),
     }
     serving = None
     if not args.no_serving:

@@ -76,6 +76,7 @@ EXPORTED_SYMBOLS = [
     "grgl_hip_device_count",
     "grgl_hip_host_alloc",
     "grgl_hip_host_free",
+    "grgl_hip_stream_ceiling",
 ]
 
 
@@ -171,6 +172,8 @@ def lib():
     L.grgl_hip_host_alloc.argtypes = [sz, ctypes.POINTER(vp)]
     L.grgl_hip_host_free.restype = ci
     L.grgl_hip_host_free.argtypes = [vp]
+    L.grgl_hip_stream_ceiling.restype = ci
+    L.grgl_hip_stream_ceiling.argtypes = [ci, sz, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.grgl_hip_graph_create.restype = ci
     L.grgl_hip_graph_create.argtypes = [ctypes.POINTER(GraphDesc), ci, ctypes.POINTER(vp)]
     L.grgl_hip_graph_destroy.restype = ci
@@ -233,6 +236,14 @@ def check(status):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data
+
+
+def stream_ceiling(device=0, nbytes=2 << 30):
+    """grgl_hip_stream_ceiling: measured streaming read and copy bandwidth of the device in
+    GB/s, as {"read_GBps": r, "copy_GBps": c}."""
+    r, c = ctypes.c_double(), ctypes.c_double()
+    check(lib().grgl_hip_stream_ceiling(device, nbytes, ctypes.byref(r), ctypes.byref(c)))
+    return {"read_GBps": r.value, "copy_GBps": c.value}
 
 
 class Graph:

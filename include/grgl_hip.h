@@ -265,6 +265,12 @@ int grgl_hip_device_count(void);
 grgl_status grgl_hip_host_alloc(size_t bytes, void** out);
 grgl_status grgl_hip_host_free(void* ptr);
 
+/* Measured streaming bandwidth of `device`, in GB/s: a kernel reading `bytes` (at least 4096,
+ * under 32 GiB) with one 16-byte load per thread, and a kernel copying them (read + write
+ * bytes counted).  Allocates and frees two buffers of `bytes`.  No counterpart in the
+ * reference: it gives the practical ceiling the level kernels' HBM traffic is compared with. */
+grgl_status grgl_hip_stream_ceiling(int device, size_t bytes, double* read_gbps, double* copy_gbps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -203,3 +203,16 @@ def test_large_host_inputs_split_over_threads(gpu):
         assert w.nbytes >= 32 << 20
         for _ in range(2):
             assert bits_equal(grgl_amd.matmul(g, w, DOWN), og.matmul(w, oracle.DOWN)), K
+
+
+def test_stream_ceiling(gpu):
+    """grgl_hip_stream_ceiling: measured read and copy rates are positive and under the
+    published 8 TB/s (with 5% margin); out-of-range arguments are misuse errors."""
+    from grgl_amd import capi
+
+    c = capi.stream_ceiling(0, 256 << 20)
+    assert 0 < c["read_GBps"] < 8400 and 0 < c["copy_GBps"] < 8400
+    with pytest.raises(RuntimeError):
+        capi.stream_ceiling(0, 1024)
+    with pytest.raises(RuntimeError):
+        capi.stream_ceiling(99, 1 << 20)
