@@ -376,8 +376,9 @@ bool aligned16(const void* p) { return ((uintptr_t)p & 15U) == 0; }
 // Tile height: TK = min(64, pow2ceil(K)); TC = tile elements / TK.
 uint32_t tileShift(uint32_t K) { return std::min<uint32_t>(6, log2u(pow2ceil(K))); }
 
-// Elements of the transpose tile (kTileElems<T> of the kernels).
+// Elements of the transpose tiles (kTileElems<T> / kSeedTileElems<T> of the kernels).
 uint32_t tileElems(size_t es) { return es == 1 ? 4096U : 8192U / (uint32_t)es; }
+uint32_t seedTileElems(size_t es) { return es == 8 ? 2048U : 4096U; }
 
 // Row-side 16-byte accesses need a 16-byte aligned row stride and request offset, and a
 // request at least 16 bytes wide: only such requests own whole 16-byte slots (stackOffsets
@@ -408,7 +409,7 @@ void seedRows(TypeClass tc, size_t es, const void* in, uint64_t inCols, uint32_t
     // The kernels' fixed access counts assume a tile line of at least one row-side vector.
     const bool vr = rowVectorOk(ld, kOffset, K, es) && (1ULL << a.tkShift) * es >= 16;
     const bool vm = colMap == nullptr && per == 1 && (inCols * es) % 16 == 0 && aligned16(in);
-    const uint32_t TK = 1U << a.tkShift, TC = tileElems(es) / TK;
+    const uint32_t TK = 1U << a.tkShift, TC = seedTileElems(es) / TK;
     dim3 grid((D + TC - 1) / TC, (K + TK - 1) / TK);
     launchSeed(tc, grid, a, vr, vm, s);
 }
@@ -467,7 +468,7 @@ void stackedTranspose(TypeClass tc, size_t es, bool seed, const std::vector<Trav
     vm = vm && (cols * es) % 16 == 0;
     if (seed) vm = vm && per == 1;
     const bool vr = geo.ld % 16 == 0 && geo.rowBytes % 16 == 0 && (1ULL << a.tkShift) * es >= 16;
-    const uint32_t TK = 1U << a.tkShift, TC = tileElems(es) / TK;
+    const uint32_t TK = 1U << a.tkShift, TC = (seed ? seedTileElems(es) : tileElems(es)) / TK;
     dim3 grid((uint32_t)((C + TC - 1) / TC), (a.kCols + TK - 1) / TK);
     launchStacked(tc, seed, grid, a, vr, vm, s);
 }

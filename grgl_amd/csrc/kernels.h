@@ -433,14 +433,17 @@ __global__ __launch_bounds__(kBlock) void pull_tail_kernel(TailArgs t) {
 // start 16-byte aligned, so that padding belongs to no other request); matrix-side
 // accesses never cross the last column.
 // ---------------------------------------------------------------------------------------
-// Tiles of 8 KB for 2- to 8-byte types (4 KB for bytes): two 16-byte accesses per thread and
-// direction.  A plain copy on MI355X runs at 6.1 TB/s with one 16-byte access per thread of a
-// 256-thread block, 5.8 with two and 5.6 with four (the stacked read-out of 511 FP32 rows to
-// 1M columns: 762 us with 16 KB tiles, 650 us with 8 KB).  The tile must still give every
-// thread at least one 16-byte access (TE >= kBlock * 16 / sizeof(T)).
+// Read-out tiles are 8 KB for 2- to 8-byte types (4 KB for bytes): two 16-byte accesses per
+// thread and direction.  A plain copy on MI355X runs at 6.1 TB/s with one 16-byte access per
+// thread of a 256-thread block, 5.8 with two and 5.6 with four (the stacked read-out of 511
+// FP32 rows to 1M columns: 762 us with 16 KB tiles, 650 us with 8 KB).  Seed tiles stay at
+// 16 KB for 4- and 8-byte types (the matching seed: 802 us with 16 KB, 809-838 with 8 KB).
+// A tile must give every thread at least one 16-byte access (TE >= kBlock * 16 / sizeof(T)).
 template <typename T> constexpr int kTileElems() { return sizeof(T) == 1 ? 4096 : 8192 / (int)sizeof(T); }
+template <typename T> constexpr int kSeedTileElems() { return sizeof(T) == 8 ? 2048 : 4096; }
 static_assert(kTileElems<uint8_t>() >= kBlock * 16 && kTileElems<uint16_t>() * 2 >= kBlock * 16 &&
-                  kTileElems<float>() * 4 >= kBlock * 16 && kTileElems<double>() * 8 >= kBlock * 16,
+                  kTileElems<float>() * 4 >= kBlock * 16 && kTileElems<double>() * 8 >= kBlock * 16 &&
+                  kSeedTileElems<uint8_t>() >= kBlock * 16,
               "every thread needs at least one 16-byte access per tile");
 constexpr uint32_t ilog2c(uint32_t v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
 // Tile index math uses shifts: TK, TC, VR and VM are powers of two, so every "vectors per
@@ -468,7 +471,7 @@ struct SeedArgs {
 
 template <typename T, bool XOR, int VR, int VM>
 __global__ __launch_bounds__(kBlock) void seed_rows_kernel(SeedArgs a) {
-    constexpr int TE = kTileElems<T>();
+    constexpr int TE = kSeedTileElems<T>();
     __shared__ T tile[TE + 64];
     const uint32_t TK = 1U << a.tkShift;
     const uint32_t TC = (uint32_t)TE >> a.tkShift;
@@ -817,7 +820,7 @@ __device__ __forceinline__ void stack_resolve(const StackArgs& a, uint32_t k0, u
 
 template <typename T, bool XOR, int VR, int VM>
 __global__ __launch_bounds__(kBlock) void seed_stacked_kernel(StackArgs a) {
-    constexpr int TE = kTileElems<T>();
+    constexpr int TE = kSeedTileElems<T>();
     __shared__ T tile[TE + 64];
     __shared__ char* kPtr[64];
     const uint32_t TK = 1U << a.tkShift;
