@@ -946,8 +946,10 @@ GRGL_EXPORT grgl_status grgl_hip_graph_create(const grgl_graph_desc* desc, int d
     h.ploidy = desc->ploidy;
     if (!desc->down_offsets) throw MisuseError("down_offsets is required");
     h.downOff.assign(desc->down_offsets, desc->down_offsets + (size_t)desc->num_nodes + 1);
-    h.downChild.assign(desc->down_children, desc->down_children + h.downOff.back());
-    h.mutNode.assign(desc->mutation_nodes, desc->mutation_nodes + desc->num_mutations);
+    if (h.downOff.back() > 0 && !desc->down_children) throw MisuseError("down_children is required");
+    if (desc->num_mutations > 0 && !desc->mutation_nodes) throw MisuseError("mutation_nodes is required");
+    if (h.downOff.back() > 0) h.downChild.assign(desc->down_children, desc->down_children + h.downOff.back());
+    if (desc->num_mutations > 0) h.mutNode.assign(desc->mutation_nodes, desc->mutation_nodes + desc->num_mutations);
     if (desc->mutation_miss_nodes) {
         h.mutMiss.assign(desc->mutation_miss_nodes, desc->mutation_miss_nodes + desc->num_mutations);
         bool any = false;

@@ -230,3 +230,20 @@ def _build_shim_test(tmp_path):
 def test_cpp_shim_compiles_and_links(tmp_path):
     """include/grgl_hip.hpp (GRG::matMul / matrixMultiplication surface) builds against the library."""
     assert _build_shim_test(tmp_path).exists()
+
+
+def test_graph_create_rejects_missing_arrays():
+    """grgl_hip_graph_create checks its descriptor before any device work: missing offsets,
+    children (when there are edges) or mutation nodes (when there are mutations) are misuse."""
+    off = np.array([0, 0, 0, 2], np.uint64)
+    child = np.array([0, 1], np.uint32)
+    muts = np.array([2], np.uint32)
+    L = capi.lib()
+    cases = [(None, child, muts, "down_offsets"), (off, None, muts, "down_children"), (off, child, None, "mutation_nodes")]
+    for o, c, m, what in cases:
+        d = capi.GraphDesc(2, 3, 2, 1, None if o is None else o.ctypes.data, None if c is None else c.ctypes.data,
+                           None if m is None else m.ctypes.data, None, None)
+        out = ctypes.c_void_p()
+        assert L.grgl_hip_graph_create(ctypes.byref(d), 0, ctypes.byref(out)) == capi.ERR_MISUSE
+        assert what in L.grgl_hip_last_error().decode()
+        assert not out.value
