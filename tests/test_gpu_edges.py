@@ -216,3 +216,37 @@ def test_stream_ceiling(gpu):
         capi.stream_ceiling(0, 1024)
     with pytest.raises(RuntimeError):
         capi.stream_ceiling(99, 1 << 20)
+
+
+def test_graph_without_mutations_or_edges(gpu):
+    """Degenerate graphs, as pygrgl.matmul treats them (src/python/_grgl.cpp:196-203): with no
+    mutations an UP result has zero columns and a DOWN input cannot be given (zero columns is
+    misuse); with no edges, UP reads the mutated samples and DOWN writes them."""
+    tree = csr_from_lists(4, [[], [], [], [], [0, 1], [2, 3], [4, 5]], np.zeros(0, np.uint32))
+    g = _pg(tree)
+    assert g.num_mutations == 0
+    for dt in (np.int32, np.float64):
+        out = grgl_amd.matmul(g, np.ones((2, 4), dt), UP)
+        assert out.shape == (2, 0) and out.dtype == dt
+    with pytest.raises(RuntimeError):
+        grgl_amd.matmul(g, np.ones((2, 0), np.int32), DOWN)
+    # every node's subtree sample count
+    nodes = grgl_amd.matmul(g, np.ones((1, 4), np.int32), UP, emit_all_nodes=True)
+    assert nodes.tolist() == [[1, 1, 1, 1, 2, 2, 4]]
+
+    flat = csr_from_lists(4, [[], [], [], []], [2, 0, INVALID, 2])
+    g2 = _pg(flat)
+    og = oracle.OracleGraph(flat)
+    rng = np.random.default_rng(8)
+    x = rng.integers(-9, 9, size=(3, 4)).astype(np.int32)
+    up = grgl_amd.matmul(g2, x, UP)
+    assert up.tolist() == np.stack([x[:, 2], x[:, 0], np.zeros(3, np.int32), x[:, 2]], axis=1).tolist()
+    assert bits_equal(up, og.matmul(x, oracle.UP))
+    w = rng.integers(-9, 9, size=(3, 4)).astype(np.int32)
+    down = grgl_amd.matmul(g2, w, DOWN)
+    zero = np.zeros(3, np.int32)
+    want = np.stack([w[:, 1], zero, w[:, 0] + w[:, 3], zero], axis=1)
+    assert down.tolist() == want.tolist()
+    assert bits_equal(down, og.matmul(w, oracle.DOWN))
+    f = rng.standard_normal((2, 4)).astype(np.float32)
+    assert bits_equal(grgl_amd.matmul(g2, f, DOWN), og.matmul(f, oracle.DOWN))
