@@ -7,7 +7,7 @@ import pytest
 
 import grgl_amd
 import oracle
-from helpers import bits_equal, csr_from_lists, make_input
+from helpers import bits_equal, csr_from_lists, make_input, random_grg
 
 pytestmark = pytest.mark.gpu
 INVALID = grgl_amd.INVALID_NODE
@@ -250,3 +250,22 @@ def test_graph_without_mutations_or_edges(gpu):
     assert bits_equal(down, og.matmul(w, oracle.DOWN))
     f = rng.standard_normal((2, 4)).astype(np.float32)
     assert bits_equal(grgl_amd.matmul(g2, f, DOWN), og.matmul(f, oracle.DOWN))
+
+
+def test_strided_and_fortran_inputs_are_made_contiguous(gpu):
+    """matmul() takes its input through py::array::ensure(c_style | forcecast), as the reference
+    does (src/python/_grgl.cpp:194): Fortran-ordered and strided views give the results of
+    their contiguous copies; the dtype is the input's own, so a Python list is rejected."""
+    a = random_grg(61, num_samples=40, num_internal=150, num_mutations=220)
+    g = _pg(a)
+    rng = np.random.default_rng(21)
+    for dt in (np.float64, np.int32):
+        x = make_input(rng, dt, (5, 2 * g.num_samples))[:, ::2]
+        assert not x.flags.c_contiguous
+        want = grgl_amd.matmul(g, np.ascontiguousarray(x), UP)
+        assert bits_equal(grgl_amd.matmul(g, x, UP), want)
+        w = np.asfortranarray(make_input(rng, dt, (6, g.num_mutations)))
+        assert not w.flags.c_contiguous
+        assert bits_equal(grgl_amd.matmul(g, w, DOWN), grgl_amd.matmul(g, np.ascontiguousarray(w), DOWN))
+    with pytest.raises(RuntimeError):
+        grgl_amd.matmul(g, [[1.0] * g.num_samples], UP)
