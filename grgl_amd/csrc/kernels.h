@@ -150,11 +150,11 @@ __device__ __forceinline__ void pull_block_coords(const PullArgs& a, uint32_t& b
 
 // Light rows: one team per row, list summed front to back with UNR row loads in flight.
 // `idx(i)` returns gather entry i (from global memory or from an LDS copy).
+// `b`, `e`: the row's list bounds relative to the launch's first entry.
 template <typename T, int VB, int TW, bool XOR, int UNR, typename Idx>
-__device__ __forceinline__ void pull_light(const PullArgs& a, uint32_t row, uint64_t colByte, bool active,
-                                           Idx idx) {
+__device__ __forceinline__ void pull_light_range(const PullArgs& a, uint32_t row, uint32_t b, uint32_t e,
+                                                 uint64_t colByte, bool active, Idx idx) {
     using P = Pack<T, VB>;
-    const uint32_t b = a.off[row] - a.entryBase, e = a.off[row + 1] - a.entryBase;
     P acc = zero_pack<T, VB>();
     if (!active) return;
     if (a.readSelf) acc = load_pack<T, VB>(a.rows + (uint64_t)row * a.ld + colByte);
@@ -181,6 +181,13 @@ __device__ __forceinline__ void pull_light(const PullArgs& a, uint32_t row, uint
             if (i + u < e) acc_pack<T, VB, XOR>(acc, x[u]);
     }
     store_pack<T, VB>(a.rows + (uint64_t)row * a.ld + colByte, acc);
+}
+
+template <typename T, int VB, int TW, bool XOR, int UNR, typename Idx>
+__device__ __forceinline__ void pull_light(const PullArgs& a, uint32_t row, uint64_t colByte, bool active,
+                                           Idx idx) {
+    pull_light_range<T, VB, TW, XOR, UNR>(a, row, a.off[row] - a.entryBase, a.off[row + 1] - a.entryBase, colByte,
+                                          active, idx);
 }
 
 // Heavy rows: the whole block on one row, 64 interleaved partials, fixed reduction tree.
@@ -346,6 +353,13 @@ __global__ __launch_bounds__(kBlock) void pull_level_kernel(PullArgs a) {
         const uint32_t r1 = r0 + NT < a.levelEnd ? r0 + NT : a.levelEnd;
         const uint32_t e0 = a.off[r0] - a.entryBase, n = a.off[r1] - a.off[r0];
         const uint32_t row = r0 + team;
+        // The row's list bounds are loaded with the block's range, before the index copy, so
+        // they cost no round trip of their own.
+        uint32_t rb = 0, re = 0;
+        if (row < r1) {
+            rb = a.off[row] - a.entryBase;
+            re = a.off[row + 1] - a.entryBase;
+        }
         if (n <= kStageEntries) { // block-uniform
             // Every index load of a thread is issued before its first LDS store, so the
             // copy is one round trip whatever n is (a load-store loop costs one per pass).
@@ -362,9 +376,9 @@ __global__ __launch_bounds__(kBlock) void pull_level_kernel(PullArgs a) {
                 if (t < n) staged[t] = v[u];
             }
             __syncthreads();
-            if (row < r1) pull_light<T, VB, TW, XOR, UNR>(a, row, colByte, active, StagedIdx{staged, e0});
+            if (row < r1) pull_light_range<T, VB, TW, XOR, UNR>(a, row, rb, re, colByte, active, StagedIdx{staged, e0});
         } else if (row < r1) {
-            pull_light<T, VB, TW, XOR, UNR>(a, row, colByte, active, GlobalIdx{a.src});
+            pull_light_range<T, VB, TW, XOR, UNR>(a, row, rb, re, colByte, active, GlobalIdx{a.src});
         }
     } else {
         const uint32_t row = a.levelBegin + a.heavyCount + (bx - a.heavyBlocks) * NT + team;
