@@ -611,11 +611,14 @@ def main():
     t0 = time.time()
     g = grgl_amd.grg_from_coalescent(args.haplotypes, args.sites, args.trees, seed=SEED)
     arrays = g.arrays()
-    cg = capi.Graph(arrays)
+    t_create = time.time()
+    cg = capi.Graph(arrays)  # levelization of both directions + upload (grgl_hip_graph_create)
+    graph_create_s = time.time() - t_create
     info = cg.info()
     V, E, N, M = info["num_nodes"], info["num_edges"], info["num_samples"], info["num_mutations"]
     log(f"graph: V={V} E={E} levels up/down={info['up_levels']}/{info['down_levels']} "
-        f"max fan-in up/down={info['max_fanin_up']}/{info['max_fanin_down']} built in {time.time() - t0:.1f}s")
+        f"max fan-in up/down={info['max_fanin_up']}/{info['max_fanin_down']} built in {time.time() - t0:.1f}s "
+        f"(device graph {graph_create_s:.2f}s)")
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     gen = torch.Generator(device="cuda")
@@ -785,6 +788,8 @@ def main():
             "levels_down": info["down_levels"],
             "parallelism": "single GPU",
         },
+        # host levelization of both directions + upload, once per graph (not in the step)
+        "graph_create_s": round(graph_create_s, 3),
         "step_alg_GBps": step_alg_bytes / (t_total / args.steps) / 1e9,
         "plain": plain,
         "plain_i32": plain_i32,
