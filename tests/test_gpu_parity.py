@@ -231,6 +231,40 @@ def test_batch_alone_equals_mixed(gpu, dtype, accumulate, graph):
     cg.close()
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.int32, np.float64])
+def test_batch_host_buffers_and_order_preserving(gpu, dtype):
+    """grgl_hip_matmul_batch with host buffers (FLAG_HOST_PTRS: the batch is staged through
+    the workspace), with and without accumulation and order preservation, gives each request
+    the bits of the same request alone; integers equal the oracle, and order-preserving floats
+    equal the oracle's own float arithmetic."""
+    arrays, _, og = _graph("random_heavy")
+    cg = capi.Graph(arrays)
+    dt = capi.NUMPY_TO_DTYPE[np.dtype(dtype)]
+    rng = np.random.default_rng(31)
+    for direction in (capi.UP, capi.DOWN):
+        cin = cg.num_samples if direction == capi.UP else cg.num_mutations
+        cout = cg.num_mutations if direction == capi.UP else cg.num_samples
+        Ks = [4, 1, 17, 64, 2]
+        xs = [np.ascontiguousarray(make_input(rng, dtype, (k, cin))) for k in Ks]
+        for flags in (0, capi.FLAG_ACCUMULATE, capi.FLAG_ORDER_PRESERVING):
+            start = [make_input(rng, dtype, (k, cout)) if flags & capi.FLAG_ACCUMULATE else np.zeros((k, cout), dtype)
+                     for k in Ks]
+            outs = [np.ascontiguousarray(b.copy()) for b in start]
+            cg.matmul_batch_ptr(dt, direction, [(k, x.ctypes.data, o.ctypes.data) for k, x, o in zip(Ks, xs, outs)],
+                                flags=flags | capi.FLAG_HOST_PTRS)
+            for k, x, b, o in zip(Ks, xs, start, outs):
+                alone = np.ascontiguousarray(b.copy())
+                cg.matmul_ptr(dt, x.ctypes.data, cin, k, direction, alone.ctypes.data, alone.size,
+                              flags=flags | capi.FLAG_HOST_PTRS)
+                assert bits_equal(o, alone), (direction, flags, k)
+                if np.dtype(dtype).kind == "i" or flags & capi.FLAG_ORDER_PRESERVING:
+                    want = og.matmul(x, _odir(UP if direction == capi.UP else DOWN))
+                    if flags & capi.FLAG_ACCUMULATE:
+                        want = want + b
+                    assert bits_equal(o, want.astype(dtype)), (direction, flags, k)
+    cg.close()
+
+
 def test_deterministic_repeat(gpu):
     arrays, g, _ = _graph("random_heavy")
     x = make_input(np.random.default_rng(2), np.float32, (16, g.num_samples))
