@@ -269,3 +269,37 @@ def test_strided_and_fortran_inputs_are_made_contiguous(gpu):
         assert bits_equal(grgl_amd.matmul(g, w, DOWN), grgl_amd.matmul(g, np.ascontiguousarray(w), DOWN))
     with pytest.raises(RuntimeError):
         grgl_amd.matmul(g, [[1.0] * g.num_samples], UP)
+
+
+def test_host_alloc_buffers_through_the_c_abi(gpu):
+    """grgl_hip_host_alloc / grgl_hip_host_free: page-locked buffers used as host input and
+    output of a matmul give the same bits as ordinary numpy buffers; zero bytes is misuse."""
+    import ctypes
+
+    from grgl_amd import capi
+
+    a = random_grg(63, num_samples=50, num_internal=200, num_mutations=300)
+    cg = capi.Graph(a)
+    L = capi.lib()
+    rng = np.random.default_rng(4)
+    K = 6
+    x = make_input(rng, np.float32, (K, cg.num_samples))
+    want = cg.matmul_host(x, capi.UP)
+    pin_in, pin_out = ctypes.c_void_p(), ctypes.c_void_p()
+    assert L.grgl_hip_host_alloc(x.nbytes, ctypes.byref(pin_in)) == capi.OK and pin_in.value
+    assert L.grgl_hip_host_alloc(want.nbytes, ctypes.byref(pin_out)) == capi.OK and pin_out.value
+    try:
+        xin = np.ctypeslib.as_array((ctypes.c_float * x.size).from_address(pin_in.value)).reshape(x.shape)
+        yout = np.ctypeslib.as_array((ctypes.c_float * want.size).from_address(pin_out.value)).reshape(want.shape)
+        xin[...] = x
+        yout[...] = 0
+        cg.matmul_ptr(capi.F32, pin_in.value, cg.num_samples, K, capi.UP, pin_out.value, yout.size,
+                      flags=capi.FLAG_HOST_PTRS)
+        assert bits_equal(yout.copy(), want)
+    finally:
+        assert L.grgl_hip_host_free(pin_in) == capi.OK
+        assert L.grgl_hip_host_free(pin_out) == capi.OK
+    bad = ctypes.c_void_p()
+    assert L.grgl_hip_host_alloc(0, ctypes.byref(bad)) == capi.ERR_MISUSE
+    assert L.grgl_hip_host_free(None) == capi.OK
+    cg.close()
