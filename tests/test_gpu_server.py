@@ -139,3 +139,29 @@ def test_device_submit_zero_copy_equals_alone(graphs):
         assert bits_equal(yd.cpu().numpy(), cg.matmul_host(x, d))
     assert srv.stats()["batches"] < 40
     srv.close()
+
+
+def test_submit_misuse_and_close_with_work_queued(graphs):
+    """Submissions the server cannot serve are misuse errors at submit time (unknown graph id,
+    more rows than max_batch_rows, invalid direction, zero rows) as are unknown request ids,
+    and closing a server with requests still queued or running returns cleanly."""
+    import ctypes
+
+    _, cg, _ = graphs[0]
+    srv = capi.Server(num_streams=1, max_batch_rows=16)
+    gid = srv.add_graph(cg)
+    L = capi.lib()
+    x = np.ones((17, cg.num_samples), np.float32)
+    out = np.zeros((17, cg.num_mutations), np.float32)
+    rid = ctypes.c_uint64()
+    for graph_id, direction, rows in ((gid + 7, capi.UP, 4), (gid, capi.UP, 17), (gid, 3, 4), (gid, capi.UP, 0)):
+        st = L.grgl_hip_server_submit(srv.handle, graph_id, capi.F32, direction, rows, x.ctypes.data, out.ctypes.data,
+                                      ctypes.byref(rid))
+        assert st == capi.ERR_MISUSE, (graph_id, direction, rows)
+    with pytest.raises(capi.GrglError) as ei:
+        srv.wait(123456789)
+    assert ei.value.status == capi.ERR_MISUSE
+    y = np.ones((4, cg.num_samples), np.float32)
+    keep = [srv.submit(gid, y, capi.UP) for _ in range(30)]
+    assert len(keep) == 30
+    srv.close()  # cancels what is queued, waits for running batches
