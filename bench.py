@@ -326,7 +326,8 @@ def serve_benchmark(num_requests=2000, load=0.6, num_streams=2):
 
 def options_benchmark(cg, N, M, V, E, P=2, K=16):
     """The remaining matmul options on the benchmark graph, f32, K = 16, one request per call:
-    by_individual (UP reads K x N/P, DOWN writes K x N/P) and emit_all_nodes (K x V out)."""
+    by_individual (UP reads K x N/P, DOWN writes K x N/P), emit_all_nodes (K x V out) and
+    order_preserving (every list summed in the reference's order, heavy rows included)."""
     import torch
 
     from grgl_amd import capi
@@ -335,7 +336,9 @@ def options_benchmark(cg, N, M, V, E, P=2, K=16):
     cases = [("by_individual_UP", capi.UP, N // P, M, dict(by_individual=True)),
              ("by_individual_DOWN", capi.DOWN, M, N // P, dict(by_individual=True)),
              ("emit_all_nodes_UP", capi.UP, N, V, dict(emit_all=True)),
-             ("emit_all_nodes_DOWN", capi.DOWN, M, V, dict(emit_all=True))]
+             ("emit_all_nodes_DOWN", capi.DOWN, M, V, dict(emit_all=True)),
+             ("order_preserving_UP", capi.UP, N, M, dict(flags=capi.FLAG_ORDER_PRESERVING)),
+             ("order_preserving_DOWN", capi.DOWN, M, N, dict(flags=capi.FLAG_ORDER_PRESERVING))]
     out = {}
     for name, d, cin, cout, kw in cases:
         x = torch.randn((K, cin), device="cuda")
