@@ -59,10 +59,21 @@ static_assert(sizeof(MutationRecord) == 24, "mutation record size");
 
 void fail(const std::string& m) { throw std::runtime_error("Malformed GRG file: " + m); }
 
+// Every array and string is checked against the bytes left in the file before anything is
+// allocated for it, so a corrupt count fails as a malformed file instead of a huge allocation.
 struct Reader {
     std::ifstream in;
+    uint64_t size = 0;
     explicit Reader(const std::string& path) : in(path, std::ios::binary) {
         if (!in) throw std::runtime_error("Could not open GRG file " + path);
+        in.seekg(0, std::ios::end);
+        const std::streamoff end = in.tellg();
+        in.seekg(0, std::ios::beg);
+        size = end > 0 ? (uint64_t)end : 0;
+    }
+    uint64_t remaining() {
+        const std::streamoff at = in.tellg();
+        return (at >= 0 && (uint64_t)at <= size) ? size - (uint64_t)at : 0;
     }
     void bytes(void* dst, size_t n) {
         in.read(reinterpret_cast<char*>(dst), (std::streamsize)n);
@@ -73,7 +84,8 @@ struct Reader {
         bytes(&v, sizeof(T));
         return v;
     }
-    template <typename T> std::vector<T> array(size_t n) {
+    template <typename T> std::vector<T> array(uint64_t n) {
+        if (n > remaining() / sizeof(T)) fail("unexpected end of file");
         std::vector<T> v(n);
         if (n) bytes(v.data(), n * sizeof(T));
         return v;
@@ -90,6 +102,7 @@ struct Reader {
     }
     std::string str() {
         const uint32_t n = varint();
+        if (n > remaining()) fail("unexpected end of file");
         std::string s(n, '\0');
         if (n) bytes(&s[0], n);
         return s;
@@ -164,10 +177,10 @@ HostGrg readGrgFile(const std::string& path) {
     // Edges.
     const uint64_t edgeBytes = r.scalar<uint64_t>();
     const std::vector<uint8_t> edges = r.array<uint8_t>(edgeBytes);
-    std::vector<uint64_t> index(V + 1);
+    std::vector<uint64_t> index;
     if (h.versionMinor <= 1) {
         const std::vector<uint32_t> small = r.array<uint32_t>(V + 1);
-        for (uint64_t i = 0; i <= V; i++) index[i] = small[i];
+        index.assign(small.begin(), small.end());
     } else {
         index = r.array<uint64_t>(V + 1);
     }
@@ -180,7 +193,10 @@ HostGrg readGrgFile(const std::string& path) {
             if (e < b || e > edgeBytes) fail("node index");
             if (e > b) {
                 size_t used = 0;
-                g.downOff[n + 1] = getVarint(edges.data() + b, (size_t)(e - b), used);
+                const uint32_t count = getVarint(edges.data() + b, (size_t)(e - b), used);
+                // every child takes at least one byte of the list
+                if (count > (e - b) - used) fail("node list length");
+                g.downOff[n + 1] = count;
             }
         }
     });

@@ -122,6 +122,40 @@ def test_corrupt_files_raise(tmp_path):
         grgl_amd.load_immutable_grg(str(tmp_path / "missing.grg"))
 
 
+def test_corrupt_counts_fail_before_allocating(tmp_path):
+    """Counts that the file cannot hold (the edge section's byte count, the node count, a list
+    longer than its bytes) are malformed-file errors, raised before any buffer of that size is
+    allocated."""
+    import struct
+
+    g = grgl_amd.grg_from_coalescent(100, 500, 1, seed=1)
+    fn = str(tmp_path / "ok.grg")
+    grgl_amd.save_grg(g, fn)
+    data = bytearray(open(fn, "rb").read())
+    bad = str(tmp_path / "bad.grg")
+    # edge section byte count (first field after the 128-byte header): 2^60
+    d = bytearray(data)
+    d[128:136] = struct.pack("<Q", 1 << 60)
+    open(bad, "wb").write(bytes(d))
+    with pytest.raises(RuntimeError, match="Malformed"):
+        grgl_amd.load_immutable_grg(bad)
+    # node count (header offset 32): 2^31 - 2 nodes, far more index entries than the file holds
+    d = bytearray(data)
+    d[32:40] = struct.pack("<Q", (1 << 31) - 2)
+    open(bad, "wb").write(bytes(d))
+    with pytest.raises(RuntimeError, match="Malformed"):
+        grgl_amd.load_immutable_grg(bad)
+    # the first non-empty list claims 2^28 children in a few bytes (varint 0x80 0x80 0x80 0x80 0x01)
+    edge_bytes = struct.unpack("<Q", data[128:136])[0]
+    edges = data[136:136 + edge_bytes]
+    first = next(i for i in range(len(edges)) if edges[i] != 0)
+    d = bytearray(data)
+    d[136 + first:136 + first + 5] = bytes([0x80, 0x80, 0x80, 0x80, 0x01])
+    open(bad, "wb").write(bytes(d))
+    with pytest.raises(RuntimeError, match="Malformed"):
+        grgl_amd.load_immutable_grg(bad)
+
+
 @pytest.mark.gpu
 def test_gpu_matmul_on_loaded_file(gpu, tmp_path):
     g = grgl_amd.grg_from_coalescent(2000, 20000, 4, seed=12)
