@@ -156,6 +156,34 @@ def test_corrupt_counts_fail_before_allocating(tmp_path):
         grgl_amd.load_immutable_grg(bad)
 
 
+def test_truncated_and_flipped_files_fail_cleanly(tmp_path):
+    """Every prefix of a small file, and random byte flips of it, either load as a valid graph or
+    raise RuntimeError: no crash and no other exception type.  (The same corpus was run through
+    the reader, validation and levelization built with -fsanitize=address,undefined.)"""
+    g = grgl_amd.grg_from_coalescent(60, 300, 2, seed=4)
+    fn = str(tmp_path / "ok.grg")
+    grgl_amd.save_grg(g, fn)
+    data = open(fn, "rb").read()
+    rng = np.random.default_rng(5)
+    bad = str(tmp_path / "bad.grg")
+    variants = [data[:c] for c in range(0, len(data), max(1, len(data) // 300))]
+    for _ in range(300):
+        b = bytearray(data)
+        for _ in range(int(rng.integers(1, 4))):
+            b[int(rng.integers(0, len(b)))] = int(rng.integers(0, 256))
+        variants.append(bytes(b))
+    loaded = 0
+    for v in variants:
+        open(bad, "wb").write(v)
+        try:
+            h = grgl_amd.load_immutable_grg(bad)
+        except RuntimeError:
+            continue
+        loaded += 1
+        assert h.num_nodes >= h.num_samples
+    assert 0 < loaded < len(variants)
+
+
 @pytest.mark.gpu
 def test_gpu_matmul_on_loaded_file(gpu, tmp_path):
     g = grgl_amd.grg_from_coalescent(2000, 20000, 4, seed=12)
