@@ -83,6 +83,23 @@ struct LevelChunk {
     uint64_t entryBegin;  // first gather entry of rowBegin
 };
 constexpr uint64_t kChunkEntries = 1ULL << 30;
+// Captured level sequences kept per graph before idle ones are dropped.
+constexpr size_t kMaxCapturedGraphs = 64;
+
+// Launch table of one direction for one heavy threshold and launch size: which rows at the
+// front of each level are heavy, and how each level is cut into launches.  Immutable once
+// built: traversals take a reference-counted snapshot, so retuning a direction swaps in a new
+// table and never changes one that a traversal on another thread is reading.
+struct LaunchTable {
+    uint32_t heavyFanin = 64;
+    uint64_t chunkEntries = kChunkEntries; // gather entries per launch at most
+    std::vector<uint32_t> heavyCount;      // per level, rows with more than heavyFanin entries
+    std::vector<uint64_t> levelEntries;    // per level, gather entries
+    std::vector<uint64_t> heavyEntries;    // per level, gather entries of the heavy rows
+    std::vector<uint64_t> levelEntryBegin; // numLevels + 1: first gather entry of each level
+    std::vector<LevelChunk> chunks;        // launches of every level, in level order
+    std::vector<uint32_t> levelChunk;      // numLevels + 1 offsets into chunks
+};
 
 struct DeviceDirPlan {
     uint32_t numNodes = 0, numLevels = 0, firstPulledLevel = 0, maxFanIn = 0;
@@ -96,17 +113,9 @@ struct DeviceDirPlan {
     uint32_t* outRow = nullptr;
     uint32_t* missRow = nullptr;
     uint32_t* coalByRow = nullptr;
-    uint32_t heavyFanin = 64;
-    std::vector<uint32_t> heavyCount;   // per level, for heavyFanin
-    std::vector<uint64_t> levelEntries; // per level, gather entries
-    std::vector<uint64_t> heavyEntries; // per level, gather entries of the heavy rows
-    std::vector<uint64_t> levelEntryBegin; // numLevels + 1: first gather entry of each level
-    std::vector<LevelChunk> chunks;     // launches of every level, in level order
-    std::vector<uint32_t> levelChunk;   // numLevels + 1 offsets into chunks
-    uint64_t chunkEntries = kChunkEntries; // gather entries per launch at most (tuning)
-    uint32_t* dLevelStart = nullptr;    // device copies read by the tail kernel
-    uint32_t* dHeavyCount = nullptr;
-    void computeHeavy();
+    // Current launch table; replaced (never modified) under grgl_graph::mu.
+    std::shared_ptr<const LaunchTable> table;
+    std::shared_ptr<const LaunchTable> buildTable(uint32_t heavyFanin, uint64_t chunkEntries) const;
 };
 
 struct Workspace {
@@ -124,9 +133,21 @@ struct Tuning {
     uint32_t xcdMap = 0;    // 0 = default (off), 1 = XCD-aware block order, 2 = off
 };
 
+// A level sequence as an instantiated hipGraph.  Held by shared_ptr: a traversal keeps its reference from the
+// cache lookup until its launch is recorded, and the cache drops only entries that nothing
+// else holds and whose last launch has completed.
 struct CapturedLevels {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipEvent_t lastLaunch = nullptr; // recorded after every launch of `exec`
+    CapturedLevels() = default;
+    CapturedLevels(const CapturedLevels&) = delete;
+    CapturedLevels& operator=(const CapturedLevels&) = delete;
+    ~CapturedLevels() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (lastLaunch) (void)hipEventDestroy(lastLaunch);
+    }
 };
 
 } // namespace grgl_amd
@@ -149,9 +170,10 @@ struct grgl_graph {
     double timedMs = 0.0;
     uint64_t timedLaunches = 0;
     double timedBytes = 0.0;
-    // Captured level sequences: (dir, typeClass, vb, tw, ld, rowBytes, readSelf, rows ptr)
+    // Captured level sequences: (dir, typeClass, vb, tw, ld, rowBytes, flags and heavy threshold,
+    // rows ptr, launch size)
     std::map<std::tuple<int, int, uint32_t, uint32_t, uint64_t, uint32_t, int, uintptr_t, uint64_t>,
-             grgl_amd::CapturedLevels>
+             std::shared_ptr<grgl_amd::CapturedLevels>>
         graphs;
 };
 

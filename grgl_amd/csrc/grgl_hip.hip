@@ -99,19 +99,22 @@ uint32_t maxTeamLanes(const RowGeometry& geo) {
     return std::min<uint32_t>(64, std::max<uint32_t>(8, pow2ceil(geo.lanes)));
 }
 
-void DeviceDirPlan::computeHeavy() {
-    heavyCount.assign(numLevels, 0);
-    levelEntries.assign(numLevels, 0);
-    heavyEntries.assign(numLevels, 0);
-    chunks.clear();
-    levelChunk.assign(numLevels + 1, 0);
-    levelEntryBegin.assign(numLevels + 1, 0);
+std::shared_ptr<const LaunchTable> DeviceDirPlan::buildTable(uint32_t heavyFanin, uint64_t chunkEntries) const {
+    auto t = std::make_shared<LaunchTable>();
+    t->heavyFanin = heavyFanin;
+    t->chunkEntries = chunkEntries;
+    t->heavyCount.assign(numLevels, 0);
+    t->levelEntries.assign(numLevels, 0);
+    t->heavyEntries.assign(numLevels, 0);
+    t->levelChunk.assign(numLevels + 1, 0);
+    t->levelEntryBegin.assign(numLevels + 1, 0);
+    std::vector<LevelChunk>& chunks = t->chunks;
     uint64_t entry = 0; // first entry of level l
     for (uint32_t l = 0; l < numLevels; l++) {
-        levelChunk[l] = (uint32_t)chunks.size();
-        levelEntryBegin[l] = entry;
+        t->levelChunk[l] = (uint32_t)chunks.size();
+        t->levelEntryBegin[l] = entry;
         uint32_t lo = levelStart[l], hi = levelStart[l + 1];
-        for (uint32_t r = lo; r < hi; r++) levelEntries[l] += fanIn[r];
+        for (uint32_t r = lo; r < hi; r++) t->levelEntries[l] += fanIn[r];
         const uint32_t begin = lo;
         while (lo < hi) {
             const uint32_t mid = lo + (hi - lo) / 2;
@@ -119,8 +122,8 @@ void DeviceDirPlan::computeHeavy() {
             else hi = mid;
         }
         const uint32_t heavyEnd = lo;
-        heavyCount[l] = heavyEnd - begin;
-        for (uint32_t r = begin; r < heavyEnd; r++) heavyEntries[l] += fanIn[r];
+        t->heavyCount[l] = heavyEnd - begin;
+        for (uint32_t r = begin; r < heavyEnd; r++) t->heavyEntries[l] += fanIn[r];
         if (l >= firstPulledLevel && levelStart[l + 1] > begin) {
             // Cut the level into chunks of at most chunkEntries entries (a longer list
             // gets a chunk of its own); one chunk unless the level is that large.
@@ -138,19 +141,22 @@ void DeviceDirPlan::computeHeavy() {
             }
             c.rowEnd = levelStart[l + 1];
             chunks.push_back(c);
-            for (uint32_t k = levelChunk[l]; k < chunks.size(); k++) {
+            for (uint32_t k = t->levelChunk[l]; k < chunks.size(); k++) {
                 LevelChunk& ck = chunks[k];
                 ck.heavy = heavyEnd > ck.rowBegin ? std::min(heavyEnd, ck.rowEnd) - ck.rowBegin : 0;
             }
         }
-        entry += levelEntries[l];
+        entry += t->levelEntries[l];
     }
-    levelChunk[numLevels] = (uint32_t)chunks.size();
-    levelEntryBegin[numLevels] = entry;
-    if (numLevels > 0) {
-        if (!dHeavyCount) GRGL_HIP_CHECK(hipMalloc(&dHeavyCount, numLevels * sizeof(uint32_t)));
-        GRGL_HIP_CHECK(hipMemcpy(dHeavyCount, heavyCount.data(), numLevels * sizeof(uint32_t), hipMemcpyHostToDevice));
-    }
+    t->levelChunk[numLevels] = (uint32_t)chunks.size();
+    t->levelEntryBegin[numLevels] = entry;
+    return t;
+}
+
+// Snapshot of a direction's current launch table.
+std::shared_ptr<const LaunchTable> launchTable(grgl_graph* g, grgl_direction dir) {
+    std::lock_guard<std::mutex> lk(g->mu);
+    return (dir == GRGL_DIRECTION_UP ? g->up : g->down).table;
 }
 
 size_t rowsBufferBytes(const grgl_graph* g, grgl_direction dir, uint32_t Ktotal, size_t elemSize) {
@@ -187,101 +193,107 @@ void validateShapes(const grgl_graph* g, grgl_direction dir, size_t inCols, size
 namespace {
 
 // Accesses narrower than 16 bytes come with teams of at most 8 lanes (levelVb).  h32: the
-// variant with per-wavefront heavy rows of 32 bytes, for (vb, tw) of a 32-byte row.
-template <typename T, bool XOR>
-void launchPullT(uint32_t vb, uint32_t tw, bool h32, dim3 grid, const PullArgs& a, hipStream_t s) {
-#define GRGL_PULL(VB, TW) hipLaunchKernelGGL((pull_level_kernel<T, VB, TW, XOR>), grid, dim3(kBlock), 0, s, a)
-#define GRGL_PULL32(VB, TW) hipLaunchKernelGGL((pull_level_kernel<T, VB, TW, XOR, true>), grid, dim3(kBlock), 0, s, a)
+// variant with per-wavefront heavy rows of 32 bytes, for (vb, tw) of a 32-byte row.  The
+// level kernels are resolved to their addresses, so that one launch list serves direct
+// launches (hipLaunchKernel) and graph nodes (hipGraphAddKernelNode) alike.
+template <typename T, bool XOR> const void* pullKernelT(uint32_t vb, uint32_t tw, bool h32) {
+#define GRGL_PULL(VB, TW) return reinterpret_cast<const void*>(&pull_level_kernel<T, VB, TW, XOR>)
+#define GRGL_PULL32(VB, TW) return reinterpret_cast<const void*>(&pull_level_kernel<T, VB, TW, XOR, true>)
     if (h32) {
         if (vb == 16 && tw == 1) GRGL_PULL32(16, 1);
-        else if (vb == 16 && tw == 2) GRGL_PULL32(16, 2);
-        else if (vb == 8 && tw == 4) GRGL_PULL32(8, 4);
-        else if (vb == 8 && tw == 8) GRGL_PULL32(8, 8);
-        else if (vb == 4 && tw == 8) { if constexpr (sizeof(T) <= 4) GRGL_PULL32(4, 8); }
-        else throw std::logic_error("no 32-byte heavy-row variant for this team shape");
-        return;
+        if (vb == 16 && tw == 2) GRGL_PULL32(16, 2);
+        if (vb == 8 && tw == 4) GRGL_PULL32(8, 4);
+        if (vb == 8 && tw == 8) GRGL_PULL32(8, 8);
+        if constexpr (sizeof(T) <= 4) {
+            if (vb == 4 && tw == 8) GRGL_PULL32(4, 8);
+        }
+        throw std::logic_error("no 32-byte heavy-row variant for this team shape");
     }
 #undef GRGL_PULL32
     if (vb == 4) {
         if constexpr (sizeof(T) <= 4) {
             if (tw == 1) GRGL_PULL(4, 1);
-            else if (tw == 2) GRGL_PULL(4, 2);
-            else if (tw == 4) GRGL_PULL(4, 4);
-            else GRGL_PULL(4, 8);
+            if (tw == 2) GRGL_PULL(4, 2);
+            if (tw == 4) GRGL_PULL(4, 4);
+            GRGL_PULL(4, 8);
         }
     } else if (vb == 8) {
         if constexpr (sizeof(T) <= 8) {
             if (tw == 1) GRGL_PULL(8, 1);
-            else if (tw == 2) GRGL_PULL(8, 2);
-            else if (tw == 4) GRGL_PULL(8, 4);
-            else GRGL_PULL(8, 8);
+            if (tw == 2) GRGL_PULL(8, 2);
+            if (tw == 4) GRGL_PULL(8, 4);
+            GRGL_PULL(8, 8);
         }
     } else {
         switch (tw) {
-        case 1: GRGL_PULL(16, 1); break;
-        case 2: GRGL_PULL(16, 2); break;
-        case 4: GRGL_PULL(16, 4); break;
-        case 8: GRGL_PULL(16, 8); break;
-        case 16: GRGL_PULL(16, 16); break;
-        case 32: GRGL_PULL(16, 32); break;
-        default: GRGL_PULL(16, 64); break;
+        case 1: GRGL_PULL(16, 1);
+        case 2: GRGL_PULL(16, 2);
+        case 4: GRGL_PULL(16, 4);
+        case 8: GRGL_PULL(16, 8);
+        case 16: GRGL_PULL(16, 16);
+        case 32: GRGL_PULL(16, 32);
+        default: GRGL_PULL(16, 64);
         }
     }
 #undef GRGL_PULL
+    throw std::logic_error("no level kernel for this access width");
 }
 
-template <typename T, bool XOR> void launchTailT(uint32_t vb, uint32_t tw, const TailArgs& a, hipStream_t s) {
-#define GRGL_TAIL(VB, TW) hipLaunchKernelGGL((pull_tail_kernel<T, VB, TW, XOR>), dim3(1), dim3(kBlock), 0, s, a)
+template <typename T, bool XOR> const void* tailKernelT(uint32_t vb, uint32_t tw) {
+#define GRGL_TAIL(VB, TW) return reinterpret_cast<const void*>(&pull_tail_kernel<T, VB, TW, XOR>)
     if (vb == 4) {
         if constexpr (sizeof(T) <= 4) {
             if (tw == 1) GRGL_TAIL(4, 1);
-            else if (tw == 2) GRGL_TAIL(4, 2);
-            else if (tw == 4) GRGL_TAIL(4, 4);
-            else GRGL_TAIL(4, 8);
+            if (tw == 2) GRGL_TAIL(4, 2);
+            if (tw == 4) GRGL_TAIL(4, 4);
+            GRGL_TAIL(4, 8);
         }
     } else if (vb == 8) {
         if constexpr (sizeof(T) <= 8) {
             if (tw == 1) GRGL_TAIL(8, 1);
-            else if (tw == 2) GRGL_TAIL(8, 2);
-            else if (tw == 4) GRGL_TAIL(8, 4);
-            else GRGL_TAIL(8, 8);
+            if (tw == 2) GRGL_TAIL(8, 2);
+            if (tw == 4) GRGL_TAIL(8, 4);
+            GRGL_TAIL(8, 8);
         }
     } else {
         switch (tw) {
-        case 1: GRGL_TAIL(16, 1); break;
-        case 2: GRGL_TAIL(16, 2); break;
-        case 4: GRGL_TAIL(16, 4); break;
-        case 8: GRGL_TAIL(16, 8); break;
-        case 16: GRGL_TAIL(16, 16); break;
-        case 32: GRGL_TAIL(16, 32); break;
-        default: GRGL_TAIL(16, 64); break;
+        case 1: GRGL_TAIL(16, 1);
+        case 2: GRGL_TAIL(16, 2);
+        case 4: GRGL_TAIL(16, 4);
+        case 8: GRGL_TAIL(16, 8);
+        case 16: GRGL_TAIL(16, 16);
+        case 32: GRGL_TAIL(16, 32);
+        default: GRGL_TAIL(16, 64);
         }
     }
 #undef GRGL_TAIL
+    throw std::logic_error("no tail kernel for this access width");
 }
 
-void launchTail(TypeClass tc, uint32_t vb, uint32_t tw, const TailArgs& a, hipStream_t s) {
+const void* tailKernel(TypeClass tc, uint32_t vb, uint32_t tw) {
     switch (tc) {
-    case kF64: launchTailT<double, false>(vb, tw, a, s); break;
-    case kF32: launchTailT<float, false>(vb, tw, a, s); break;
-    case kW64: launchTailT<uint64_t, false>(vb, tw, a, s); break;
-    case kW32: launchTailT<uint32_t, false>(vb, tw, a, s); break;
-    case kW16: launchTailT<uint16_t, false>(vb, tw, a, s); break;
-    case kW8: launchTailT<uint8_t, false>(vb, tw, a, s); break;
-    case kX8: launchTailT<uint8_t, true>(vb, tw, a, s); break;
+    case kF64: return tailKernelT<double, false>(vb, tw);
+    case kF32: return tailKernelT<float, false>(vb, tw);
+    case kW64: return tailKernelT<uint64_t, false>(vb, tw);
+    case kW32: return tailKernelT<uint32_t, false>(vb, tw);
+    case kW16: return tailKernelT<uint16_t, false>(vb, tw);
+    case kW8: return tailKernelT<uint8_t, false>(vb, tw);
+    case kX8: return tailKernelT<uint8_t, true>(vb, tw);
     }
+    throw std::logic_error("bad type class");
 }
 
-void launchPull(TypeClass tc, uint32_t vb, uint32_t tw, bool h32, dim3 grid, const PullArgs& a, hipStream_t s) {
+const void* pullKernel(TypeClass tc, uint32_t vb, uint32_t tw, bool h32) {
     switch (tc) {
-    case kF64: launchPullT<double, false>(vb, tw, h32, grid, a, s); break;
-    case kF32: launchPullT<float, false>(vb, tw, h32, grid, a, s); break;
-    case kW64: launchPullT<uint64_t, false>(vb, tw, h32, grid, a, s); break;
-    case kW32: launchPullT<uint32_t, false>(vb, tw, h32, grid, a, s); break;
-    case kW16: launchPullT<uint16_t, false>(vb, tw, h32, grid, a, s); break;
-    case kW8: launchPullT<uint8_t, false>(vb, tw, h32, grid, a, s); break;
-    case kX8: launchPullT<uint8_t, true>(vb, tw, h32, grid, a, s); break;
+    case kF64: return pullKernelT<double, false>(vb, tw, h32);
+    case kF32: return pullKernelT<float, false>(vb, tw, h32);
+    case kW64: return pullKernelT<uint64_t, false>(vb, tw, h32);
+    case kW32: return pullKernelT<uint32_t, false>(vb, tw, h32);
+    case kW16: return pullKernelT<uint16_t, false>(vb, tw, h32);
+    case kW8: return pullKernelT<uint8_t, false>(vb, tw, h32);
+    case kX8: return pullKernelT<uint8_t, true>(vb, tw, h32);
     }
+    throw std::logic_error("bad type class");
 }
 
 template <typename T, bool XOR> void launchSeedT(dim3 grid, const SeedArgs& a, bool vr, bool vm, hipStream_t s) {
@@ -507,38 +519,55 @@ Tuning chooseTuning(grgl_graph* g, grgl_direction dir, TypeClass tc, uint32_t K,
 // qualifies when its light rows take at most two passes of the block's teams, it has at
 // most two heavy rows, and its rows and gathered rows come to at most 64 KB per column
 // slice set; the run needs at least two levels to save a launch.
-uint32_t tailStart(const DeviceDirPlan& plan, const RowGeometry& geo, uint32_t tw, bool orderPreserving) {
+uint32_t tailStart(const DeviceDirPlan& plan, const LaunchTable& tab, const RowGeometry& geo, uint32_t tw,
+                   bool orderPreserving) {
     const uint32_t teams = kBlock / tw;
     uint32_t l = plan.numLevels;
     while (l > plan.firstPulledLevel && plan.numLevels - l < (uint32_t)kMaxTailLevels) {
         const uint32_t k = l - 1;
         const uint64_t nodes = plan.levelStart[k + 1] - plan.levelStart[k];
-        const uint64_t heavy = orderPreserving ? 0 : plan.heavyCount[k];
-        const uint64_t bytes = (plan.levelEntries[k] + nodes) * (uint64_t)geo.rowBytes;
+        const uint64_t heavy = orderPreserving ? 0 : tab.heavyCount[k];
+        const uint64_t bytes = (tab.levelEntries[k] + nodes) * (uint64_t)geo.rowBytes;
         if (nodes - heavy > 2ULL * teams || heavy > 2 || bytes > (64ULL << 10)) break;
         l = k;
     }
     return plan.numLevels - l >= 2 ? l : plan.numLevels;
 }
 
-// Launch every pulled level of one direction (the trailing small levels as one tail launch).
-void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const RowGeometry& geo, uint32_t tw,
-                   bool xcdMap, bool readSelf, bool orderPreserving, uint32_t K, size_t elemSize, char* rows, hipStream_t stream) {
+// One level-kernel launch: kernel address, grid and arguments (a PullArgs, or the TailArgs of
+// the single-block tail launch), plus the algorithmic bytes it moves (kernel timing).
+struct LevelLaunch {
+    const void* func = nullptr;
+    dim3 grid;
+    bool tail = false;
+    PullArgs pull{};
+    TailArgs tailArgs{};
+    double bytes = 0.0;
+    void* argPtr() { return tail ? (void*)&tailArgs : (void*)&pull; }
+};
+
+// The launches of every pulled level of one direction, in order (the trailing small levels as
+// one tail launch).  With `timing`, every level gets a launch of its own and its bytes.
+std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTable& tab, TypeClass tc,
+                                       const RowGeometry& geo, uint32_t tw, bool xcdMap, bool readSelf,
+                                       bool orderPreserving, uint32_t K, size_t elemSize, char* rows, bool timing) {
+    std::vector<LevelLaunch> out;
     const uint32_t tiles = levelTiles(geo, tw, elemSize);
     const uint32_t vb = levelVb(geo, tw, elemSize);
     const uint32_t teamsPerBlock = kBlock / tw;
     // The single-block tail gains nothing from spreading rows over idle lanes: rows of one
     // access take single-lane teams there (256 rows per pass instead of 256 / tw).
     const uint32_t tailTw = geo.lanes == 1 ? 1 : tw;
-    const uint32_t tailFrom = g->timing ? plan.numLevels : tailStart(plan, geo, tailTw, orderPreserving);
+    const uint32_t tailFrom = timing ? plan.numLevels : tailStart(plan, tab, geo, tailTw, orderPreserving);
     for (uint32_t l = plan.firstPulledLevel; l < tailFrom; l++) {
-      for (uint32_t ci = plan.levelChunk[l]; ci < plan.levelChunk[l + 1]; ci++) {
-        const LevelChunk& ck = plan.chunks[ci];
+      for (uint32_t ci = tab.levelChunk[l]; ci < tab.levelChunk[l + 1]; ci++) {
+        const LevelChunk& ck = tab.chunks[ci];
         const uint32_t begin = ck.rowBegin, end = ck.rowEnd;
         if (end == begin) continue;
         const uint32_t heavy = orderPreserving ? 0 : ck.heavy;
         const uint32_t light = end - begin - heavy;
-        PullArgs a;
+        LevelLaunch L;
+        PullArgs& a = L.pull;
         a.rows = rows;
         a.ld = geo.ld;
         a.off = plan.off;
@@ -553,7 +582,7 @@ void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const
         // least a quarter of the level's gather entries (its extra registers slow the light
         // rows of the launch).
         const bool manyHeavy = heavy >= 512;
-        const bool h32 = geo.rowBytes == 32 && tw <= 8 && manyHeavy && plan.heavyEntries[l] * 4 >= plan.levelEntries[l];
+        const bool h32 = geo.rowBytes == 32 && tw <= 8 && manyHeavy && tab.heavyEntries[l] * 4 >= tab.levelEntries[l];
         a.heavyPerWave = ((geo.rowBytes <= 16 && manyHeavy) || h32) ? 1 : 0;
         a.heavyBlocks = a.heavyPerWave ? (heavy + kBlock / 64 - 1) / (kBlock / 64) : heavy;
         a.levelEnd = end;
@@ -561,101 +590,137 @@ void enqueueLevels(grgl_graph* g, const DeviceDirPlan& plan, TypeClass tc, const
         a.readSelf = readSelf ? 1 : 0;
         const uint32_t bxCount = a.heavyBlocks + (light + teamsPerBlock - 1) / teamsPerBlock;
         a.xcdTiles = (xcdMap && tiles % 8 == 0) ? tiles : 0;
-        dim3 grid = a.xcdTiles ? dim3(bxCount * tiles, 1) : dim3(bxCount, tiles);
-        hipEvent_t ev0 = nullptr, ev1 = nullptr;
-        if (g->timing) {
-            GRGL_HIP_CHECK(hipEventCreate(&ev0));
-            GRGL_HIP_CHECK(hipEventCreate(&ev1));
-            GRGL_HIP_CHECK(hipEventRecord(ev0, stream));
-        }
-        launchPull(tc, vb, tw, h32, grid, a, stream);
-        GRGL_HIP_CHECK(hipGetLastError());
-        if (g->timing) {
-            GRGL_HIP_CHECK(hipEventRecord(ev1, stream));
+        L.grid = a.xcdTiles ? dim3(bxCount * tiles, 1) : dim3(bxCount, tiles);
+        L.func = pullKernel(tc, vb, tw, h32);
+        if (timing) {
             const uint64_t nodes = end - begin;
             uint64_t ent = 0;
             for (uint32_t r = begin; r < end; r++) ent += plan.fanIn[r];
             const double rowB = (double)K * (double)elemSize;
-            const double bytes = 4.0 * (double)(ent + nodes + 1) + rowB * (double)(ent + nodes + (readSelf ? nodes : 0));
-            std::lock_guard<std::mutex> lk(g->mu);
-            g->timingEvents.push_back({ev0, ev1});
-            g->timingBytes.push_back(bytes);
+            L.bytes = 4.0 * (double)(ent + nodes + 1) + rowB * (double)(ent + nodes + (readSelf ? nodes : 0));
         }
+        out.push_back(L);
       }
     }
     if (tailFrom < plan.numLevels) {
-        TailArgs t;
+        LevelLaunch L;
+        L.tail = true;
+        L.grid = dim3(1);
+        TailArgs& t = L.tailArgs;
         t.rows = rows;
         t.ld = geo.ld;
         // Tail levels are small (at most 64 KB of rows each), so one base covers them all;
         // the tail kernel takes whole levels.
-        const uint64_t tailEntry = plan.levelEntryBegin[tailFrom];
+        const uint64_t tailEntry = tab.levelEntryBegin[tailFrom];
         t.off = plan.off;
         t.src = plan.src + tailEntry;
         t.entryBase = (uint32_t)tailEntry;
         t.rowBytes = geo.rowBytes;
         t.readSelf = readSelf ? 1 : 0;
         t.tiles = levelTiles(geo, tailTw, elemSize);
-        t.levelStart = plan.dLevelStart + tailFrom;
-        t.heavyCount = plan.dHeavyCount + tailFrom;
-        t.numLevels = plan.numLevels - tailFrom;
-        t.noHeavy = orderPreserving ? 1 : 0;
-        launchTail(tc, levelVb(geo, tailTw, elemSize), tailTw, t, stream);
-        GRGL_HIP_CHECK(hipGetLastError());
+        t.numLevels = plan.numLevels - tailFrom; // <= kMaxTailLevels (tailStart)
+        for (uint32_t i = 0; i <= t.numLevels; i++) t.levelStart[i] = plan.levelStart[tailFrom + i];
+        for (uint32_t i = 0; i < t.numLevels; i++) t.heavyCount[i] = orderPreserving ? 0 : tab.heavyCount[tailFrom + i];
+        L.func = tailKernel(tc, levelVb(geo, tailTw, elemSize), tailTw);
+        out.push_back(L);
+    }
+    return out;
+}
+
+// Launches on `stream`; with timing, each launch is bracketed by events kept on the graph.
+void launchDirect(grgl_graph* g, std::vector<LevelLaunch>& launches, hipStream_t stream, bool timing) {
+    for (LevelLaunch& L : launches) {
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        if (timing) {
+            GRGL_HIP_CHECK(hipEventCreate(&ev0));
+            GRGL_HIP_CHECK(hipEventCreate(&ev1));
+            GRGL_HIP_CHECK(hipEventRecord(ev0, stream));
+        }
+        void* args[1] = {L.argPtr()};
+        GRGL_HIP_CHECK(hipLaunchKernel(L.func, L.grid, dim3(kBlock), args, 0, stream));
+        if (timing) {
+            GRGL_HIP_CHECK(hipEventRecord(ev1, stream));
+            std::lock_guard<std::mutex> lk(g->mu);
+            g->timingEvents.push_back({ev0, ev1});
+            g->timingBytes.push_back(L.bytes);
+        }
     }
 }
 
-// Level sequence, either launched directly or replayed from a captured hipGraph.  A
-// captured sequence bakes in the row buffer address, the row geometry and the team
-// width, so the cache key holds all of them; different workspaces get different graphs.
+// The launches as a chain of kernel nodes of a new hipGraph, instantiated.  Built node by node
+// (no stream capture), so what other threads do on the device meanwhile cannot affect it.
+void buildLevelGraph(std::vector<LevelLaunch>& launches, CapturedLevels& out) {
+    GRGL_HIP_CHECK(hipGraphCreate(&out.graph, 0));
+    hipGraphNode_t prev = nullptr;
+    for (LevelLaunch& L : launches) {
+        hipKernelNodeParams p{};
+        void* args[1] = {L.argPtr()};
+        p.func = const_cast<void*>(L.func);
+        p.gridDim = L.grid;
+        p.blockDim = dim3(kBlock);
+        p.sharedMemBytes = 0;
+        p.kernelParams = args;
+        p.extra = nullptr;
+        hipGraphNode_t node = nullptr;
+        GRGL_HIP_CHECK(hipGraphAddKernelNode(&node, out.graph, prev ? &prev : nullptr, prev ? 1 : 0, &p));
+        prev = node;
+    }
+    GRGL_HIP_CHECK(hipGraphInstantiate(&out.exec, out.graph, nullptr, nullptr, 0));
+}
+
+// Level sequence, either launched directly or replayed from a hipGraph of it.  A graph bakes
+// in the row buffer address, the row geometry, the team width and the launch table, so the
+// cache key holds all of them; different workspaces get different graphs.
 void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometry& geo, const Tuning& t,
                bool readSelf, bool orderPreserving, uint32_t K, size_t es, char* rows, hipStream_t stream) {
-    DeviceDirPlan& plan = dir == GRGL_DIRECTION_UP ? g->up : g->down;
-    if (!t.useGraph || stream == nullptr || g->timing) {
-        enqueueLevels(g, plan, tc, geo, t.teamLanes, t.xcdMap == 1, readSelf, orderPreserving, K, es, rows, stream);
+    const DeviceDirPlan& plan = dir == GRGL_DIRECTION_UP ? g->up : g->down;
+    const std::shared_ptr<const LaunchTable> tab = launchTable(g, dir);
+    const bool timing = g->timing;
+    bool direct = !t.useGraph || timing;
+    if (!direct && stream != nullptr) {
+        // A stream the caller is capturing gets direct launches (they become part of the
+        // caller's graph, which then does not depend on this library's cache).
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        GRGL_HIP_CHECK(hipStreamIsCapturing(stream, &cs));
+        direct = cs != hipStreamCaptureStatusNone;
+    }
+    if (direct) {
+        std::vector<LevelLaunch> launches = levelLaunches(plan, *tab, tc, geo, t.teamLanes, t.xcdMap == 1, readSelf,
+                                                          orderPreserving, K, es, rows, timing);
+        launchDirect(g, launches, stream, timing);
         return;
     }
     const auto key = std::make_tuple((int)dir, (int)tc, geo.vb, t.teamLanes, geo.ld, geo.rowBytes,
                                      (int)readSelf * 2 + (int)orderPreserving + 4 * (int)(t.xcdMap == 1) +
-                                         8 * (int)plan.heavyFanin,
-                                     (uintptr_t)rows, plan.chunkEntries);
-    hipGraphExec_t exec = nullptr;
+                                         8 * (int)tab->heavyFanin,
+                                     (uintptr_t)rows, tab->chunkEntries);
+    std::shared_ptr<CapturedLevels> cap;
     {
         std::lock_guard<std::mutex> lk(g->mu);
         auto it = g->graphs.find(key);
-        if (it != g->graphs.end()) exec = it->second.exec;
+        if (it != g->graphs.end()) cap = it->second;
     }
-    if (!exec) {
-        CapturedLevels cap;
-        GRGL_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        try {
-            enqueueLevels(g, plan, tc, geo, t.teamLanes, t.xcdMap == 1, readSelf, orderPreserving, K, es, rows, stream);
-        } catch (...) {
-            hipGraph_t broken = nullptr;
-            (void)hipStreamEndCapture(stream, &broken);
-            if (broken) (void)hipGraphDestroy(broken);
-            throw;
-        }
-        GRGL_HIP_CHECK(hipStreamEndCapture(stream, &cap.graph));
-        const hipError_t ie = hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0);
-        if (ie != hipSuccess) {
-            (void)hipGraphDestroy(cap.graph);
-            GRGL_HIP_CHECK(ie);
-        }
-        exec = cap.exec;
+    if (!cap) {
+        auto fresh = std::make_shared<CapturedLevels>();
+        GRGL_HIP_CHECK(hipEventCreateWithFlags(&fresh->lastLaunch, hipEventDisableTiming));
+        std::vector<LevelLaunch> launches = levelLaunches(plan, *tab, tc, geo, t.teamLanes, t.xcdMap == 1, readSelf,
+                                                          orderPreserving, K, es, rows, false);
+        buildLevelGraph(launches, *fresh);
         std::lock_guard<std::mutex> lk(g->mu);
-        if (g->graphs.size() >= 64) {
-            // Bounded cache: drop everything not in use (graphs of idle workspaces).
-            (void)hipDeviceSynchronize();
-            for (auto& kv : g->graphs) {
-                (void)hipGraphExecDestroy(kv.second.exec);
-                (void)hipGraphDestroy(kv.second.graph);
+        if (g->graphs.size() >= kMaxCapturedGraphs) {
+            // Bounded cache: drop the entries that no traversal holds and whose last launch
+            // has completed (a non-blocking event query; nothing waits here).
+            for (auto it = g->graphs.begin(); it != g->graphs.end();) {
+                const bool idle = it->second.use_count() == 1 && hipEventQuery(it->second->lastLaunch) == hipSuccess;
+                it = idle ? g->graphs.erase(it) : std::next(it);
             }
-            g->graphs.clear();
         }
-        g->graphs[key] = cap;
+        // Two threads may build the same key: the first insertion wins, and the other
+        // thread's graph (never launched) is destroyed when `fresh` goes out of scope.
+        cap = g->graphs.try_emplace(key, fresh).first->second;
     }
-    GRGL_HIP_CHECK(hipGraphLaunch(exec, stream));
+    GRGL_HIP_CHECK(hipGraphLaunch(cap->exec, stream));
+    GRGL_HIP_CHECK(hipEventRecord(cap->lastLaunch, stream));
 }
 
 } // namespace
@@ -793,39 +858,48 @@ void enqueueTraversal(grgl_graph* g, grgl_dtype dt, grgl_direction dir, const st
 }
 
 Workspace* acquireWorkspace(grgl_graph* g, size_t bytes, hipStream_t stream) {
-    std::lock_guard<std::mutex> lk(g->mu);
+    std::unique_ptr<Workspace> victim; // freed outside the lock: hipFree synchronises the device
     Workspace* pick = nullptr;
-    for (auto& w : g->workspaces) {
-        if (w->busy || w->bytes < bytes) continue;
-        if (!w->used || w->stream == stream || hipEventQuery(w->done) == hipSuccess) {
-            pick = w.get();
-            break;
+    {
+        std::lock_guard<std::mutex> lk(g->mu);
+        for (auto& w : g->workspaces) {
+            if (w->busy || w->bytes < bytes) continue;
+            if (!w->used || w->stream == stream || hipEventQuery(w->done) == hipSuccess) {
+                pick = w.get();
+                break;
+            }
         }
-    }
-    if (!pick) {
-        // Grow: drop an idle, finished workspace that is too small, then allocate.
+        if (pick) {
+            pick->busy = true;
+            return pick;
+        }
+        // Grow: take out an idle, finished workspace that is too small, then allocate.
         for (auto it = g->workspaces.begin(); it != g->workspaces.end(); ++it) {
             Workspace& w = **it;
             if (!w.busy && w.bytes < bytes && (!w.used || hipEventQuery(w.done) == hipSuccess)) {
-                (void)hipFree(w.ptr);
-                if (w.done) (void)hipEventDestroy(w.done);
+                victim = std::move(*it);
                 g->workspaces.erase(it);
                 break;
             }
         }
-        std::unique_ptr<Workspace> w(new Workspace());
-        const size_t sz = std::max<size_t>(bytes, 1 << 20);
-        GRGL_HIP_CHECK(hipMalloc(&w->ptr, sz));
-        w->bytes = sz;
-        const hipError_t ev = hipEventCreateWithFlags(&w->done, hipEventDisableTiming);
-        if (ev != hipSuccess) {
-            (void)hipFree(w->ptr);
-            GRGL_HIP_CHECK(ev);
-        }
-        pick = w.get();
-        g->workspaces.push_back(std::move(w));
     }
-    pick->busy = true;
+    if (victim) {
+        (void)hipFree(victim->ptr);
+        if (victim->done) (void)hipEventDestroy(victim->done);
+    }
+    std::unique_ptr<Workspace> w(new Workspace());
+    const size_t sz = std::max<size_t>(bytes, 1 << 20);
+    GRGL_HIP_CHECK(hipMalloc(&w->ptr, sz));
+    w->bytes = sz;
+    const hipError_t ev = hipEventCreateWithFlags(&w->done, hipEventDisableTiming);
+    if (ev != hipSuccess) {
+        (void)hipFree(w->ptr);
+        GRGL_HIP_CHECK(ev);
+    }
+    w->busy = true;
+    pick = w.get();
+    std::lock_guard<std::mutex> lk(g->mu);
+    g->workspaces.push_back(std::move(w));
     return pick;
 }
 
@@ -894,9 +968,7 @@ void uploadPlan(DeviceDirPlan& d, const DirPlan& p, uint64_t& bytes) {
     d.outRow = uploadVec(p.outRow, bytes);
     d.missRow = uploadVec(p.missRow, bytes);
     d.coalByRow = uploadVec(p.coalByRow, bytes);
-    d.dLevelStart = uploadVec(p.levelStart, bytes);
-    d.computeHeavy();
-    bytes += d.numLevels * sizeof(uint32_t);
+    d.table = d.buildTable(64, kChunkEntries);
 }
 
 void freePlan(DeviceDirPlan& d) {
@@ -907,8 +979,6 @@ void freePlan(DeviceDirPlan& d) {
     freeDev(d.outRow);
     freeDev(d.missRow);
     freeDev(d.coalByRow);
-    freeDev(d.dLevelStart);
-    freeDev(d.dHeavyCount);
     d = DeviceDirPlan();
 }
 
@@ -1020,10 +1090,7 @@ GRGL_EXPORT grgl_status grgl_hip_graph_destroy(grgl_graph* g) {
     if (!g) return GRGL_OK;
     (void)hipSetDevice(g->device);
     (void)hipDeviceSynchronize();
-    for (auto& kv : g->graphs) {
-        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
-        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
-    }
+    g->graphs.clear(); // CapturedLevels destroys its graph, exec and event
     for (auto& w : g->workspaces) {
         (void)hipFree(w->ptr);
         if (w->done) (void)hipEventDestroy(w->done);
@@ -1202,11 +1269,19 @@ void matmulRows(grgl_graph* g, grgl_dtype dtype, const void* input, size_t input
 // Input rows per traversal: 65536, or fewer when the row buffer (plus staging of host
 // buffers) would take more than half of the free device memory.  Rows are independent, so a
 // larger input is computed block by block with identical results.
-uint32_t rowsPerTraversal(grgl_graph* g, grgl_direction dir, size_t es, bool host, size_t inCols, size_t outCols) {
+uint32_t rowsPerTraversal(grgl_graph* g, grgl_direction dir, size_t es, bool host, size_t inCols, size_t outCols,
+                          grgl_init_mode initMode, bool hasMiss) {
     uint32_t kb = 65536;
+    // Host buffers are staged in the same workspace as the row buffer: input, output, the
+    // init vector or matrix and the miss matrix (as matmulRows lays them out).
     auto need = [&](uint32_t k) {
         size_t b = rowsBufferBytes(g, dir, k, es);
-        if (host) b += (size_t)k * (inCols + outCols) * es;
+        if (host) {
+            b += (size_t)k * (inCols + outCols) * es;
+            if (initMode == GRGL_INIT_VECTOR) b += (size_t)k * es;
+            else if (initMode == GRGL_INIT_MATRIX) b += (size_t)k * g->V * es;
+            if (hasMiss) b += (size_t)k * g->M * es;
+        }
         return b;
     };
     if (need(kb) <= (1ULL << 30)) return kb; // small graphs: no device query
@@ -1238,7 +1313,8 @@ GRGL_EXPORT grgl_status grgl_hip_matmul(grgl_graph* g, grgl_dtype dtype, const v
     const size_t es = dtypeSize(dtype);
     const size_t outCols = output_size / input_rows;
     const uint32_t kb = (uint32_t)std::min<size_t>(
-        input_rows, rowsPerTraversal(g, direction, es, (flags & GRGL_FLAG_HOST_PTRS) != 0, input_cols, outCols));
+        input_rows, rowsPerTraversal(g, direction, es, (flags & GRGL_FLAG_HOST_PTRS) != 0, input_cols, outCols,
+                                     init_mode, miss != nullptr));
     // Row blocks: input, output, init and miss are row-major with one row per input row.
     for (size_t k0 = 0; k0 < input_rows; k0 += kb) {
         const uint32_t rows = (uint32_t)std::min<size_t>(kb, input_rows - k0);
@@ -1344,11 +1420,13 @@ GRGL_EXPORT grgl_status grgl_hip_set_tuning(grgl_graph* g, grgl_direction direct
     t.xcdMap = tuning->xcd_map;
     g->tuning[std::make_tuple((int)direction, (int)tc, rows)] = t;
     if (tuning->heavy_fanin || tuning->max_launch_entries) {
+        // A new launch table replaces the direction's current one; traversals already
+        // running keep the snapshot they took.
         DeviceDirPlan& plan = direction == GRGL_DIRECTION_UP ? g->up : g->down;
-        if (tuning->heavy_fanin) plan.heavyFanin = tuning->heavy_fanin;
-        if (tuning->max_launch_entries) plan.chunkEntries = std::min<uint64_t>(tuning->max_launch_entries, kChunkEntries);
-        GRGL_HIP_CHECK(hipSetDevice(g->device)); // computeHeavy uploads the heavy counts
-        plan.computeHeavy();
+        const uint32_t heavy = tuning->heavy_fanin ? tuning->heavy_fanin : plan.table->heavyFanin;
+        const uint64_t chunk = tuning->max_launch_entries ? std::min<uint64_t>(tuning->max_launch_entries, kChunkEntries)
+                                                          : plan.table->chunkEntries;
+        if (heavy != plan.table->heavyFanin || chunk != plan.table->chunkEntries) plan.table = plan.buildTable(heavy, chunk);
     }
     GRGL_API_END
 }
@@ -1364,10 +1442,10 @@ GRGL_EXPORT grgl_status grgl_hip_get_tuning(grgl_graph* g, grgl_direction direct
     std::lock_guard<std::mutex> lk(g->mu);
     const DeviceDirPlan& plan = direction == GRGL_DIRECTION_UP ? g->up : g->down;
     out->team_lanes = t.teamLanes;
-    out->heavy_fanin = plan.heavyFanin;
+    out->heavy_fanin = plan.table->heavyFanin;
     out->use_graph = t.useGraph;
     out->xcd_map = t.xcdMap;
-    out->max_launch_entries = (uint32_t)plan.chunkEntries;
+    out->max_launch_entries = (uint32_t)plan.table->chunkEntries;
     GRGL_API_END
 }
 
@@ -1409,8 +1487,10 @@ GRGL_EXPORT grgl_status grgl_hip_tuning_save(grgl_graph* g, const char* path) {
     os << "graph " << fp << "\n";
     {
         std::lock_guard<std::mutex> lk(g->mu);
-        os << "direction " << (int)GRGL_DIRECTION_UP << " " << g->up.heavyFanin << " " << g->up.chunkEntries << "\n";
-        os << "direction " << (int)GRGL_DIRECTION_DOWN << " " << g->down.heavyFanin << " " << g->down.chunkEntries << "\n";
+        os << "direction " << (int)GRGL_DIRECTION_UP << " " << g->up.table->heavyFanin << " "
+           << g->up.table->chunkEntries << "\n";
+        os << "direction " << (int)GRGL_DIRECTION_DOWN << " " << g->down.table->heavyFanin << " "
+           << g->down.table->chunkEntries << "\n";
         for (const auto& kv : g->tuning) {
             const Tuning& t = kv.second;
             os << "entry " << std::get<0>(kv.first) << " " << std::get<1>(kv.first) << " " << std::get<2>(kv.first) << " "
@@ -1462,15 +1542,11 @@ GRGL_EXPORT grgl_status grgl_hip_tuning_load(grgl_graph* g, const char* path) {
             throw MisuseError("malformed tuning file");
         }
     }
-    GRGL_HIP_CHECK(hipSetDevice(g->device)); // computeHeavy uploads the heavy counts
     std::lock_guard<std::mutex> lk(g->mu);
     for (const Dir& d : dirs) {
         DeviceDirPlan& plan = d.dir == GRGL_DIRECTION_UP ? g->up : g->down;
-        if (plan.heavyFanin != (uint32_t)d.heavy || plan.chunkEntries != d.chunk) {
-            plan.heavyFanin = (uint32_t)d.heavy;
-            plan.chunkEntries = d.chunk;
-            plan.computeHeavy();
-        }
+        if (plan.table->heavyFanin != (uint32_t)d.heavy || plan.table->chunkEntries != d.chunk)
+            plan.table = plan.buildTable((uint32_t)d.heavy, d.chunk);
     }
     for (const auto& e : entries) g->tuning[e.first] = e.second;
     GRGL_API_END
@@ -1591,10 +1667,11 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
     }
     if (chosen) {
         chosen->team_lanes = best.teamLanes;
-        chosen->heavy_fanin = (direction == GRGL_DIRECTION_UP ? g->up : g->down).heavyFanin;
+        const std::shared_ptr<const LaunchTable> tab = launchTable(g, direction);
+        chosen->heavy_fanin = tab->heavyFanin;
         chosen->use_graph = best.useGraph;
         chosen->xcd_map = best.xcdMap;
-        chosen->max_launch_entries = (uint32_t)(direction == GRGL_DIRECTION_UP ? g->up : g->down).chunkEntries;
+        chosen->max_launch_entries = (uint32_t)tab->chunkEntries;
     }
     if (best_ms) *best_ms = bestMs;
     GRGL_API_END

@@ -397,13 +397,14 @@ struct TailArgs {
     const uint32_t* off;
     const uint32_t* src;
     uint32_t entryBase;         // as PullArgs (src points at the first level's first entry)
-    const uint32_t* levelStart; // numLevels + 1 row boundaries (device)
-    const uint32_t* heavyCount; // numLevels heavy-row counts (device)
     uint32_t rowBytes;
     uint32_t readSelf;
     uint32_t tiles;       // column slices of TW * VB bytes
     uint32_t numLevels;
-    uint32_t noHeavy;     // 1: every row is light (order-preserving traversal)
+    // Level boundaries and heavy-row counts travel in the kernel arguments (read with
+    // uniform indices: scalar loads), so the kernel reads no table that retuning replaces.
+    uint32_t levelStart[kMaxTailLevels + 1];
+    uint32_t heavyCount[kMaxTailLevels]; // 0 for every level of an order-preserving traversal
 };
 
 template <typename T, int VB, int TW, bool XOR>
@@ -419,7 +420,7 @@ __global__ __launch_bounds__(kBlock) void pull_tail_kernel(TailArgs t) {
         a.src = t.src;
         a.entryBase = t.entryBase;
         a.levelBegin = t.levelStart[l];
-        a.heavyCount = t.noHeavy ? 0 : t.heavyCount[l];
+        a.heavyCount = t.heavyCount[l];
         a.heavyBlocks = a.heavyCount;
         a.heavyPerWave = 0;
         a.levelEnd = t.levelStart[l + 1];

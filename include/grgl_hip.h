@@ -151,14 +151,18 @@ grgl_status grgl_hip_matmul_batch(grgl_graph* graph, grgl_dtype dtype, grgl_dire
 /* Tuning of the level kernels for (graph, direction, dtype, rows).  0 = library default.
  * Team width, hipGraph replay and block order only change speed.  heavy_fanin changes which
  * gather lists are split 64 ways (float rounding of those rows) for every later call in that
- * direction; set it before traversals start, not while others run on the graph. */
+ * direction.  heavy_fanin and max_launch_entries (and grgl_hip_tuning_load) replace the
+ * direction's launch table; a traversal already enqueued or being enqueued on another thread
+ * keeps the table it started with, so retuning is safe while traversals run (only calls that
+ * start afterwards see the new heavy threshold). */
 typedef struct grgl_tuning {
     uint32_t team_lanes;     /* lanes that share one node's row slice: a power of two up to the
                                 row's 16-byte slices rounded up, or up to 8 for short rows (which
                                 then take 8- or 4-byte accesses and idle lanes); larger values are
                                 clamped */
     uint32_t heavy_fanin;    /* gather lists longer than this are split into 64 partials */
-    uint32_t use_graph;      /* 1 = replay the level sequence as a captured hipGraph */
+    uint32_t use_graph;      /* 1 = replay the level sequence as a hipGraph (built once per row buffer and
+                                shape from explicit kernel nodes, cached per graph) */
     uint32_t xcd_map;        /* block order of the level kernels when the row splits into a multiple
                                 of 8 column slices: 0 = default (plain 2-D grid), 1 = XCD-aware (XCD i
                                 takes slices i, i+8, ...), 2 = plain 2-D grid.  The plain grid runs the

@@ -3,10 +3,16 @@
 Bars (stated here, enforced below):
   * integer and bool dtypes: bit-exact;
   * floats with order_preserving=True: bit-exact (same summation order as the CPU path);
-  * floats by default (high fan-in lists split 64 ways): FP64 <= 1e-12 and FP32 <= 1e-6,
-    normwise per output row (max |gpu - ref| / max |ref|) against the FP64 oracle — for FP32
-    the bar is max(1e-6, 4 x the error of the CPU path's own FP32 result), since the
-    reference CPU FP32 result carries summation rounding of the same size.
+  * floats by default (high fan-in lists split 64 ways):
+      - FP64: <= 1e-12 normwise per output row (max |gpu - ref| / max |ref|) against the oracle;
+      - FP32, any graph: every element within the a priori rounding bound of a correct FP32
+        evaluation (oracle/fp_bounds.py: sum over the nodes reaching the output of
+        (fan-in - 1) * 2^-24 * sum |list terms|; bound_ratio <= 1), so no term is dropped,
+        repeated or misplaced;
+      - FP32 on the coalescent benchmark graphs with N(0,1) inputs (1,000 x 10,000;
+        100,000 x 500,000; 200,000 x 1,000,000): <= 1e-6 normwise per row against FP64.
+        Arbitrary test DAGs carry no such bar: there the reference's own FP32 result is up to
+        2.3e-6 from FP64 (high fan-in lists of nearly cancelling terms).
 """
 
 import numpy as np
@@ -15,6 +21,7 @@ import pytest
 import grgl_amd
 import oracle
 from grgl_amd import capi
+from oracle import fp_bounds
 from helpers import (bits_equal, depth3_bin_tree, direct_map_grg, make_input, normwise_rel_err, random_grg,
                      sample8_grg, vcf_fixture)
 
@@ -36,15 +43,22 @@ def _odir(d):
     return oracle.UP if d == UP else oracle.DOWN
 
 
-def _check_float(gpu_out, x, og, direction, **kw):
-    ref64 = og.matmul(x.astype(np.float64), _odir(direction), **kw)
-    err = normwise_rel_err(gpu_out, ref64)
+FP32_BAR = 1e-6  # normwise, against FP64, on the benchmark graphs
+
+
+def _check_float(gpu_out, x, og, direction, bar=None):
+    """FP64: <= 1e-12 normwise.  FP32: within the rounding bound, and <= `bar` normwise when
+    given.  Returns the FP32 error report (None for FP64)."""
     if x.dtype == np.float64:
+        ref64 = og.matmul(x, _odir(direction))
+        err = normwise_rel_err(gpu_out, ref64)
         assert err <= 1e-12, err
-    else:
-        cpu32 = og.matmul(x, _odir(direction), **kw)
-        bar = max(1e-6, 4 * normwise_rel_err(cpu32, ref64))
-        assert err <= bar, (err, bar)
+        return None
+    rep = fp_bounds.fp32_report(og, x, _odir(direction), gpu_out, with_reference=False)
+    assert rep["bound_ratio"] <= 1.0, rep
+    if bar is not None:
+        assert rep["normwise"] <= bar, rep
+    return rep
 
 
 # --- golden vectors on the device --------------------------------------------------------
@@ -120,7 +134,7 @@ def test_parity_all_dtypes(gpu, name, dtype):
             if np.dtype(dtype).kind in "biu":
                 assert bits_equal(out, og.matmul(x, _odir(direction))), (direction, K)
             else:
-                _check_float(out, x, og, direction)
+                _check_float(out, x, og, direction, bar=FP32_BAR if name == "tree_cfg1" else None)
                 exact = grgl_amd.matmul(g, x, direction, order_preserving=True)
                 assert bits_equal(exact, og.matmul(x, _odir(direction))), (direction, K)
 
@@ -305,7 +319,9 @@ def test_config2_scale_parity(gpu):
             ref64 = og.matmul(x64, _odir(direction))
             assert normwise_rel_err(out64, ref64) <= 1e-12
             x32 = x64.astype(np.float32)
-            _check_float(grgl_amd.matmul(g, x32, direction), x32, og, direction)
+            rep = _check_float(grgl_amd.matmul(g, x32, direction), x32, og, direction, bar=FP32_BAR)
+            print("fp32 100k x 500k K=%d %s: normwise %.3e, bound ratio %.3f" %
+                  (K, "UP" if direction == UP else "DOWN", rep["normwise"], rep["bound_ratio"]))
     ones = np.ones((1, g.num_samples), np.int32)
     assert bits_equal(grgl_amd.matmul(g, ones, UP), og.matmul(ones, oracle.UP))
 

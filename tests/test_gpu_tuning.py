@@ -176,3 +176,65 @@ def test_tuning_cache_file(gpu, tmp_path):
         fresh.load_tuning(str(bad))
     for h in (cg, fresh, other):
         h.close()
+
+
+def test_graph_cache_threads_past_its_bound(gpu):
+    """Captured level sequences from several threads at once, over more keys than the cache
+    keeps (64), while another thread swaps the DOWN launch table (max_launch_entries, which
+    never changes results): every replay equals the direct launches, bit for bit.  Entries are
+    evicted only when no traversal holds them and their last launch has completed."""
+    import concurrent.futures as cf
+
+    import torch
+
+    arrays = _graphs()["tree"]
+    cg = capi.Graph(arrays)
+    rng = np.random.default_rng(21)
+    Ks = list(range(1, 91))
+    cases = []
+    for i, K in enumerate(Ks):
+        dtype = np.float32 if i % 2 == 0 else np.int32
+        direction = capi.UP if i % 3 else capi.DOWN
+        cin = cg.num_samples if direction == capi.UP else cg.num_mutations
+        cout = cg.num_mutations if direction == capi.UP else cg.num_samples
+        cases.append((K, dtype, direction, cin, cout, torch.from_numpy(make_input(rng, dtype, (K, cin))).cuda()))
+    stream = torch.cuda.current_stream().cuda_stream
+    refs = []
+    for K, dtype, direction, cin, cout, x in cases:
+        refs.append(_run(cg, capi.NUMPY_TO_DTYPE[np.dtype(dtype)], x, K, direction, cin, cout, stream).cpu().numpy())
+    for K, dtype, direction, _, _, _ in cases:
+        cg.set_tuning(direction, capi.NUMPY_TO_DTYPE[np.dtype(dtype)], K, use_graph=1)
+    stop = [False]
+
+    def retune():
+        n = 0
+        while not stop[0]:
+            cg.set_tuning(capi.DOWN, capi.I8, 1, max_launch_entries=(5000 if n % 2 else 1 << 30))
+            n += 1
+        return n
+
+    def worker(t):
+        s = torch.cuda.Stream()
+        bad = []
+        with torch.cuda.stream(s):
+            for rep in range(3):
+                for i in range(t, len(cases), 4):
+                    K, dtype, direction, cin, cout, x = cases[i]
+                    y = torch.zeros((K, cout), dtype=x.dtype, device="cuda")
+                    cg.matmul_ptr(capi.NUMPY_TO_DTYPE[np.dtype(dtype)], x.data_ptr(), cin, K, direction,
+                                  y.data_ptr(), y.numel(), stream=s.cuda_stream)
+                    s.synchronize()
+                    if not bits_equal(y.cpu().numpy(), refs[i]):
+                        bad.append((i, rep))
+        return bad
+
+    with cf.ThreadPoolExecutor(5) as ex:
+        tuner = ex.submit(retune)
+        try:
+            futs = [ex.submit(worker, t) for t in range(4)]
+            bad = [b for f in futs for b in f.result()]
+        finally:
+            stop[0] = True  # the retuning thread ends whatever the workers did
+        assert tuner.result() > 0
+    assert bad == []
+    cg.close()
