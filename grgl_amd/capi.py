@@ -31,6 +31,8 @@ INIT_ZERO, INIT_VECTOR, INIT_MATRIX, INIT_XTX = 0, 1, 2, 3
 FLAG_HOST_PTRS = 0x1
 FLAG_ACCUMULATE = 0x2
 FLAG_ORDER_PRESERVING = 0x4
+# Size of the device array holding the row buffer of grgl_hip_matmul_noalloc (GRGL_NOALLOC_SCRATCH_BYTES).
+NOALLOC_SCRATCH_BYTES = 128 << 20
 
 INVALID_NODE = 0x7FFFFFFF
 COAL_COUNT_NOT_SET = 0xFFFFFFFF
@@ -55,6 +57,7 @@ EXPORTED_SYMBOLS = [
     "grgl_hip_graph_destroy",
     "grgl_hip_graph_info",
     "grgl_hip_matmul",
+    "grgl_hip_matmul_noalloc",
     "grgl_hip_matmul_batch",
     "grgl_hip_set_tuning",
     "grgl_hip_autotune",
@@ -182,6 +185,9 @@ def lib():
     L.grgl_hip_graph_info.argtypes = [vp, ctypes.POINTER(GraphInfo)]
     L.grgl_hip_matmul.restype = ci
     L.grgl_hip_matmul.argtypes = [vp, ci, vp, sz, sz, ci, vp, sz, ci, ci, vp, ci, vp, u32, vp]
+    if hasattr(L, "grgl_hip_matmul_noalloc"):
+        L.grgl_hip_matmul_noalloc.restype = ci
+        L.grgl_hip_matmul_noalloc.argtypes = [vp, ci, vp, sz, sz, ci, vp, sz, ci, ci, vp, ci, vp, u32, vp]
     L.grgl_hip_matmul_batch.restype = ci
     L.grgl_hip_matmul_batch.argtypes = [vp, ci, ci, ctypes.POINTER(Request), u32, u32, vp]
     L.grgl_hip_set_tuning.restype = ci
@@ -299,6 +305,15 @@ class Graph:
                    by_individual=False, init_ptr=None, init_mode=INIT_ZERO, miss_ptr=None, flags=0, stream=None):
         check(lib().grgl_hip_matmul(self.handle, dtype, in_ptr, in_cols, rows, direction, out_ptr, out_size,
                                     int(emit_all), int(by_individual), init_ptr, init_mode, miss_ptr, flags, stream))
+
+    def matmul_noalloc_ptr(self, dtype, in_ptr, in_cols, rows, direction, out_ptr, out_size, emit_all=False,
+                           by_individual=False, init_ptr=None, init_mode=INIT_ZERO, miss_ptr=None, flags=0,
+                           stream=None):
+        """grgl_hip_matmul_noalloc: device pointers, no device allocation (row buffer in the
+        library's fixed scratch array; overlapping calls on one GPU race)."""
+        check(lib().grgl_hip_matmul_noalloc(self.handle, dtype, in_ptr, in_cols, rows, direction, out_ptr, out_size,
+                                            int(emit_all), int(by_individual), init_ptr, init_mode, miss_ptr, flags,
+                                            stream))
 
     def matmul_host(self, inp, direction, emit_all=False, by_individual=False, init=None, miss=None, flags=0):
         """Host numpy in, numpy out (zero-initialised, accumulated: pygrgl semantics)."""

@@ -194,6 +194,9 @@ struct TravOptions {
     void* miss = nullptr;        // device
     bool accumulate = false;
     bool orderPreserving = false;
+    // Direct kernel launches only: no hipGraph, no timing events (the allocation-free entry
+    // point, whose callers may not allocate).
+    bool launchOnly = false;
 };
 
 // Size in bytes of the row buffer for a traversal of Ktotal stacked rows.
@@ -211,6 +214,14 @@ hipStream_t acquireInternalStream(grgl_graph* g);
 void releaseInternalStream(grgl_graph* g, hipStream_t s);
 
 void setLastError(const std::string& msg);
+
+// All argument checks of grgl_hip_matmul (internal_mult.h:130-158, _grgl.cpp:100-185).
+void validateMatmulArgs(const grgl_graph* g, grgl_dtype dtype, const void* input, size_t input_cols, size_t input_rows,
+                        grgl_direction direction, const void* output, size_t output_size, int emit_all_nodes,
+                        int by_individual, const void* init, grgl_init_mode init_mode, const void* miss);
+
+// Status code of an exception caught at the C ABI; sets the thread-local message.
+grgl_status statusOf(const std::exception& e);
 
 // Validation shared by grgl_hip_matmul and the server (internal_mult.h:130-158).
 void validateShapes(const grgl_graph* g, grgl_direction dir, size_t inCols, size_t rows, size_t outSize,

@@ -672,11 +672,12 @@ void buildLevelGraph(std::vector<LevelLaunch>& launches, CapturedLevels& out) {
 // in the row buffer address, the row geometry, the team width and the launch table, so the
 // cache key holds all of them; different workspaces get different graphs.
 void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometry& geo, const Tuning& t,
-               bool readSelf, bool orderPreserving, uint32_t K, size_t es, char* rows, hipStream_t stream) {
+               bool readSelf, bool orderPreserving, uint32_t K, size_t es, char* rows, hipStream_t stream,
+               bool launchOnly = false) {
     const DeviceDirPlan& plan = dir == GRGL_DIRECTION_UP ? g->up : g->down;
     const std::shared_ptr<const LaunchTable> tab = launchTable(g, dir);
-    const bool timing = g->timing;
-    bool direct = !t.useGraph || timing;
+    const bool timing = g->timing && !launchOnly;
+    bool direct = !t.useGraph || timing || launchOnly;
     if (!direct && stream != nullptr) {
         // A stream the caller is capturing gets direct launches (they become part of the
         // caller's graph, which then does not depend on this library's cache).
@@ -821,7 +822,7 @@ void enqueueTraversal(grgl_graph* g, grgl_dtype dt, grgl_direction dir, const st
     const Tuning tuning = chooseTuning(g, dir, tc, Ktotal, geo);
     // ORDER_PRESERVING: every list is summed by one team front to back, i.e. in the
     // reference's order.
-    runLevels(g, dir, tc, geo, tuning, hasInit, opt.orderPreserving, Ktotal, es, rows, stream);
+    runLevels(g, dir, tc, geo, tuning, hasInit, opt.orderPreserving, Ktotal, es, rows, stream, opt.launchOnly);
     // 4. Read-out.  Several requests: one stacked launch for the main outputs.
     const bool stackOut = !stacked.empty() && !(dir == GRGL_DIRECTION_UP && opt.miss && plan.missRow && !opt.emitAll);
     if (stackOut) {
@@ -934,7 +935,29 @@ using namespace grgl_amd;
 
 namespace {
 
-grgl_status translate(const std::exception& e) {
+grgl_status translate(const std::exception& e) { return grgl_amd::statusOf(e); }
+
+} // namespace
+
+namespace grgl_amd {
+
+void validateMatmulArgs(const grgl_graph* g, grgl_dtype dtype, const void* input, size_t input_cols, size_t input_rows,
+                        grgl_direction direction, const void* output, size_t output_size, int emit_all_nodes,
+                        int by_individual, const void* init, grgl_init_mode init_mode, const void* miss) {
+    if (!g) throw MisuseError("null graph");
+    const TypeClass tc = typeClassOf(dtype);
+    validateShapes(g, direction, input_cols, input_rows, output_size, emit_all_nodes != 0, by_individual != 0);
+    if (!input || !output) throw MisuseError("null input or output");
+    if (tc == kX8 && init_mode != GRGL_INIT_ZERO)
+        throw MisuseError("Node initialization and bit vector type cannot be mixed");
+    if (init_mode == GRGL_INIT_XTX && !g->hasCoal)
+        throw MisuseError("Coalescent counts not available for this GRG (init=XTX won't work)");
+    if ((init_mode == GRGL_INIT_VECTOR || init_mode == GRGL_INIT_MATRIX) && !init)
+        throw MisuseError("init mode requires an init buffer");
+    if (miss && emit_all_nodes) throw MisuseError("The \"miss\" parameter cannot be mixed with the \"emit_all_nodes\" parameter");
+}
+
+grgl_status statusOf(const std::exception& e) {
     setLastError(e.what());
     if (dynamic_cast<const MisuseError*>(&e)) return GRGL_ERR_MISUSE;
     if (dynamic_cast<const NoMemError*>(&e)) return GRGL_ERR_NOMEM;
@@ -943,6 +966,10 @@ grgl_status translate(const std::exception& e) {
     if (dynamic_cast<const std::bad_alloc*>(&e)) return GRGL_ERR_NOMEM;
     return GRGL_ERR_MISUSE;
 }
+
+} // namespace grgl_amd
+
+namespace {
 
 #define GRGL_API_BEGIN try {
 #define GRGL_API_END                                                                                   \
@@ -1298,17 +1325,8 @@ GRGL_EXPORT grgl_status grgl_hip_matmul(grgl_graph* g, grgl_dtype dtype, const v
                                         int emit_all_nodes, int by_individual, const void* init,
                                         grgl_init_mode init_mode, void* miss, uint32_t flags, void* stream) {
     GRGL_API_BEGIN
-    if (!g) throw MisuseError("null graph");
-    const TypeClass tc = typeClassOf(dtype);
-    validateShapes(g, direction, input_cols, input_rows, output_size, emit_all_nodes != 0, by_individual != 0);
-    if (!input || !output) throw MisuseError("null input or output");
-    if (tc == kX8 && init_mode != GRGL_INIT_ZERO)
-        throw MisuseError("Node initialization and bit vector type cannot be mixed");
-    if (init_mode == GRGL_INIT_XTX && !g->hasCoal)
-        throw MisuseError("Coalescent counts not available for this GRG (init=XTX won't work)");
-    if ((init_mode == GRGL_INIT_VECTOR || init_mode == GRGL_INIT_MATRIX) && !init)
-        throw MisuseError("init mode requires an init buffer");
-    if (miss && emit_all_nodes) throw MisuseError("The \"miss\" parameter cannot be mixed with the \"emit_all_nodes\" parameter");
+    validateMatmulArgs(g, dtype, input, input_cols, input_rows, direction, output, output_size, emit_all_nodes,
+                       by_individual, init, init_mode, miss);
     GRGL_HIP_CHECK(hipSetDevice(g->device));
     const size_t es = dtypeSize(dtype);
     const size_t outCols = output_size / input_rows;

@@ -131,6 +131,28 @@ grgl_status grgl_hip_matmul(grgl_graph* graph, grgl_dtype dtype, const void* inp
                             void* miss, uint32_t flags, void* stream);
 
 /*
+ * grgl_hip_matmul without any device allocation, for callers that cannot allocate: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  No counterpart in
+ * the reference.  Same arguments, checks and results (bit for bit) as grgl_hip_matmul with
+ * device pointers; GRGL_FLAG_HOST_PTRS is rejected (GRGL_ERR_MISUSE).  The traversal's row
+ * buffer lives in a device array of GRGL_NOALLOC_SCRATCH_BYTES that the library holds on every
+ * GPU it runs on until the process ends; inputs of more rows than fit are run as several
+ * traversals of row blocks, and a graph whose single-row buffer exceeds the array is rejected
+ * with GRGL_ERR_MISUSE.  The row buffer is (num_nodes [+ num_mutations, or 2 x num_mutations
+ * with missing data, DOWN]) rows of K x dtype size bytes rounded up to 4, 8, 16 bytes and then
+ * to a power of two under 64 bytes, to 64 bytes up to 256, to 256 bytes above.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (from two threads, or
+ * on two streams with no order between them) race and corrupt each other's results.  The call
+ * enqueues kernels only (no hipGraph, no events; kernel timing does not see it) and returns
+ * without a host sync.
+ */
+#define GRGL_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+grgl_status grgl_hip_matmul_noalloc(grgl_graph* graph, grgl_dtype dtype, const void* input, size_t input_cols,
+                                    size_t input_rows, grgl_direction direction, void* output, size_t output_size,
+                                    int emit_all_nodes, int by_individual, const void* init, grgl_init_mode init_mode,
+                                    void* miss, uint32_t flags, void* stream);
+
+/*
  * Several requests on one graph, one direction and one dtype carried by ONE traversal:
  * their rows are stacked (each request starting on a 16-byte boundary of the row; stacked
  * rows <= 65536) and share every edge read.  Each request keeps its own input/output
