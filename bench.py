@@ -18,7 +18,12 @@ Also reported:
               device's measured streaming read / copy rates (grgl_hip_stream_ceiling);
   cpu_baseline the reference's own CPU matmul (oracle/_ref) on a bounded sample of the
               same workload: all host cores (one independent call per thread) as value,
-              one thread beside it.
+              one thread beside it;
+  fp32_error  per result (graph, K, direction): FP32 error against the FP64 oracle, the ratio
+              to the a priori rounding bound, and the reference's own FP32 error beside it
+              (checked after the timed region);
+  serving     burst / Poisson / small-K streams, served vs one request per traversal, with a
+              checked sample of served results compared with the same requests alone.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -208,9 +213,19 @@ def serve_benchmark(num_requests=2000, load=0.6, num_streams=2):
     srv = capi.Server(num_streams=num_streams, max_batch_rows=512, hbm_budget_bytes=48 << 30)
     gids = [srv.add_graph(cg) for cg in graphs]
 
-    def served_submit(r):
+    # A checked sample of the stream (every 25th request) writes to outputs of its own, so its
+    # served results can be compared with the same requests run alone after the timing.
+    private = {}
+    for i in range(0, num_requests, 25):
+        gi, K, d, cdt = reqs[i]
+        x, y, cin, cout = bufs[(gi, d, cdt)]
+        private[i] = torch.zeros((K, cout), device="cuda", dtype=y.dtype)
+
+    def served_submit(r, i=None):
         gi, K, d, cdt = r
         x, y, cin, cout = bufs[(gi, d, cdt)]
+        if i in private:
+            y = private[i]
         return srv.submit_device(gids[gi], cdt, d, K, x.data_ptr(), y.data_ptr())
 
     for r in reqs[:64]:
@@ -226,12 +241,21 @@ def serve_benchmark(num_requests=2000, load=0.6, num_streams=2):
     plain_burst = time.perf_counter() - t
     # Burst: served.
     t = time.perf_counter()
-    ids = [served_submit(r) for r in reqs]
+    ids = [served_submit(r, i) for i, r in enumerate(reqs)]
     for rid in ids:
         srv.wait(rid)
     served_burst = time.perf_counter() - t
     st_burst = srv.stats(reset=True)
     batches1 = st_burst["batches"]
+    # The checked sample against the same requests alone (bit for bit).
+    checked_equal = True
+    for i, y_served in private.items():
+        gi, K, d, cdt = reqs[i]
+        x, _, cin, cout = bufs[(gi, d, cdt)]
+        alone = torch.zeros_like(y_served)
+        graphs[gi].matmul_ptr(cdt, x.data_ptr(), cin, K, d, alone.data_ptr(), alone.numel(), stream=sp)
+        torch.cuda.synchronize()
+        checked_equal = checked_equal and bool(torch.equal(alone.view(torch.int32), y_served.view(torch.int32)))
     plain_rate = num_requests / plain_burst
     rate = load * plain_rate
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=num_requests))
@@ -301,6 +325,8 @@ def serve_benchmark(num_requests=2000, load=0.6, num_streams=2):
             "served_batches": batches1 - batches0,
             "served_p50_ms": round(st_burst["latency_p50_ms"], 3),
             "served_p99_ms": round(st_burst["latency_p99_ms"], 3),
+            "checked_requests": len(private),
+            "checked_equal_alone": checked_equal,
         },
         "poisson": {
             "offered_req_per_s": round(rate, 1),
@@ -420,6 +446,59 @@ def construction_benchmark():
             "seconds": round(dt, 3), "mutations_per_s": round(len(sets) / dt, 1),
             "nodes": g.num_nodes, "edges": g.num_edges, "carrier_entries": entries,
             "edges_per_carrier": round(g.num_edges / entries, 4)}
+
+
+def fp32_errors(arrays3, f32_results):
+    """FP32 error of the traversal results, stated per result (a checker leg, run after the
+    timed region): against the FP64 oracle (normwise per row, max over rows), against the a
+    priori rounding bound of a correct FP32 evaluation (bound_ratio <= 1), and the same
+    normwise error of the reference's own FP32 result (oracle/_ref) beside it.
+      1,000 x 10,000 (1 tree) and 100,000 x 500,000 (20 trees): one request per traversal,
+        K = 1 and 16, UP and DOWN, N(0,1) inputs;
+      200,000 x 1,000,000 (50 trees): the K = 1 and K = 16 requests of the timed 511-RHS
+        batched traversal (`f32_results`: direction -> [(K, x, y)])."""
+    import numpy as np
+    import torch
+
+    import grgl_amd
+    import oracle
+    from grgl_amd import capi
+    from oracle import fp_bounds
+
+    sp = torch.cuda.current_stream().cuda_stream
+    out = []
+
+    def record(name, path, K, d, x, y, og):
+        rep = fp_bounds.fp32_report(og, x, oracle.UP if d == capi.UP else oracle.DOWN, y)
+        rep.update({"graph": name, "path": path, "K": K, "direction": "UP" if d == capi.UP else "DOWN"})
+        for k in ("normwise", "ref_fp32_normwise"):
+            rep[k] = float("%.4g" % rep[k])
+        rep["bound_ratio"] = round(rep["bound_ratio"], 4)
+        out.append(rep)
+
+    rng = np.random.default_rng(SEED + 7)
+    for n, m, t in ((1000, 10_000, 1), (100_000, 500_000, 20)):
+        a = grgl_amd.grg_from_coalescent(n, m, t, seed=SEED).arrays()
+        cg = capi.Graph(a)
+        og = oracle.OracleGraph(a)
+        for d in (capi.UP, capi.DOWN):
+            cin, cout = (n, m) if d == capi.UP else (m, n)
+            for K in (1, 16):
+                x = rng.standard_normal((K, cin)).astype(np.float32)
+                xd = torch.from_numpy(x).cuda()
+                yd = torch.zeros((K, cout), device="cuda")
+                cg.matmul_ptr(capi.F32, xd.data_ptr(), cin, K, d, yd.data_ptr(), yd.numel(), stream=sp)
+                torch.cuda.synchronize()
+                record(f"{n} x {m}", "plain", K, d, x, yd.cpu().numpy(), og)
+        cg.close()
+    og3 = oracle.OracleGraph(arrays3)
+    for d, reqs in f32_results.items():
+        for K, x, y in reqs:
+            if K in (1, 16):
+                record("200000 x 1000000", "batched 511 RHS", K, d, x.cpu().numpy(), y.cpu().numpy(), og3)
+    return {"bar": "normwise <= 1e-6 against FP64 on these graphs; bound_ratio <= 1",
+            "max_normwise": max(r["normwise"] for r in out), "max_bound_ratio": max(r["bound_ratio"] for r in out),
+            "results": out}
 
 
 def _cpu_ms(og, kind, x, direction, reps):
@@ -575,6 +654,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving (mixed request stream) scenario")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--no-pca", action="store_true", help="skip the standardized PCA scenario")
+    ap.add_argument("--no-fp32-error", action="store_true",
+                    help="skip the FP32 error statement (oracle comparisons after the timed region)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the option, .grg file and construction measurements")
     ap.add_argument("--latency-trace", action="store_true",
@@ -719,6 +800,12 @@ def main():
     torch.cuda.synchronize()
     batched_f32_ms = (time.perf_counter() - t) / reps * 1e3
     log(f"plain per-K f32 UP+DOWN total {plain_ms_total:.3f} ms vs batched {batched_f32_ms:.3f} ms")
+    fp32_err = None
+    if not args.no_fp32_error:
+        log("FP32 error against the FP64 oracle ...")
+        fp32_err = fp32_errors(arrays, {d: reqs for cdt, d, _, _, reqs in f32_work})
+        log(f"fp32 error: max normwise {fp32_err['max_normwise']:.3g}, max bound ratio "
+            f"{fp32_err['max_bound_ratio']:.3g}")
 
     achieved = k_bytes / (k_ms / 1e3) / 1e9 if k_ms > 0 else 0.0
     avg_launch_us = k_ms / max(1, k_launches) * 1e3
@@ -798,6 +885,7 @@ def main():
         "plain_i32": plain_i32,
         "batched_vs_plain_f32": round(plain_ms_total / batched_f32_ms, 3) if plain_ms_total else None,
         "roofline": roofline,
+        "fp32_error": fp32_err,
         "plain_100k_x_500k": at_scale,
         "serving": serving,
         "pca": pca,

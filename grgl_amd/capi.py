@@ -62,6 +62,7 @@ EXPORTED_SYMBOLS = [
     "grgl_hip_set_tuning",
     "grgl_hip_autotune",
     "grgl_hip_get_tuning",
+    "grgl_hip_chain_control",
     "grgl_hip_tuning_save",
     "grgl_hip_tuning_load",
     "grgl_hip_kernel_timing",
@@ -121,7 +122,7 @@ class Request(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("team_lanes", ctypes.c_uint32), ("heavy_fanin", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
-                ("xcd_map", ctypes.c_uint32), ("max_launch_entries", ctypes.c_uint32)]
+                ("xcd_map", ctypes.c_uint32), ("max_launch_entries", ctypes.c_uint32), ("chain", ctypes.c_uint32)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -200,6 +201,8 @@ def lib():
     if hasattr(L, "grgl_hip_autotune"):
         L.grgl_hip_autotune.restype = ci
         L.grgl_hip_autotune.argtypes = [vp, ci, ci, u32, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_float)]
+        L.grgl_hip_chain_control.restype = ci
+        L.grgl_hip_chain_control.argtypes = [vp, ci, u32, ctypes.POINTER(u32), ci]
         L.grgl_hip_get_tuning.restype = ci
         L.grgl_hip_get_tuning.argtypes = [vp, ci, ci, u32, ctypes.POINTER(Tuning)]
         L.grgl_hip_tuning_save.restype = ci
@@ -347,8 +350,8 @@ class Graph:
         check(lib().grgl_hip_matmul_batch(self.handle, dtype, direction, arr, len(reqs), flags, stream))
 
     def set_tuning(self, direction, dtype, rows, team_lanes=0, heavy_fanin=0, use_graph=0, xcd_map=0,
-                   max_launch_entries=0):
-        t = Tuning(team_lanes, heavy_fanin, use_graph, xcd_map, max_launch_entries)
+                   max_launch_entries=0, chain=0):
+        t = Tuning(team_lanes, heavy_fanin, use_graph, xcd_map, max_launch_entries, chain)
         check(lib().grgl_hip_set_tuning(self.handle, direction, dtype, rows, ctypes.byref(t)))
 
     def autotune(self, direction, dtype, rows):
@@ -356,13 +359,20 @@ class Graph:
         ms = ctypes.c_float()
         check(lib().grgl_hip_autotune(self.handle, direction, dtype, rows, ctypes.byref(t), ctypes.byref(ms)))
         return {"team_lanes": t.team_lanes, "heavy_fanin": t.heavy_fanin, "use_graph": t.use_graph,
-                "xcd_map": t.xcd_map, "max_launch_entries": t.max_launch_entries}, ms.value
+                "xcd_map": t.xcd_map, "max_launch_entries": t.max_launch_entries, "chain": t.chain}, ms.value
 
     def get_tuning(self, direction, dtype, rows):
         t = Tuning()
         check(lib().grgl_hip_get_tuning(self.handle, direction, dtype, rows, ctypes.byref(t)))
         return {"team_lanes": t.team_lanes, "heavy_fanin": t.heavy_fanin, "use_graph": t.use_graph,
-                "xcd_map": t.xcd_map, "max_launch_entries": t.max_launch_entries}
+                "xcd_map": t.xcd_map, "max_launch_entries": t.max_launch_entries, "chain": t.chain}
+
+    def chain_control(self, abort=-1, wait_us=0, reset=False):
+        """grgl_hip_chain_control: set (1) / clear (0) / keep (-1) the abort word, bound chained
+        waits (wait_us > 0), and return 1 if a chained traversal on the graph gave up (then reset)."""
+        n = ctypes.c_uint32()
+        check(lib().grgl_hip_chain_control(self.handle, abort, wait_us, ctypes.byref(n), int(reset)))
+        return n.value
 
     def save_tuning(self, path):
         check(lib().grgl_hip_tuning_save(self.handle, os.fsencode(path)))

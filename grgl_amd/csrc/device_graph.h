@@ -83,6 +83,9 @@ struct LevelChunk {
     uint64_t entryBegin;  // first gather entry of rowBegin
 };
 constexpr uint64_t kChunkEntries = 1ULL << 30;
+// Device words a chained traversal keeps its ticket and completion counters in (zero whenever
+// no chained traversal runs on them).
+constexpr size_t kChainStateBytes = 4096;
 // Captured level sequences kept per graph before idle ones are dropped.
 constexpr size_t kMaxCapturedGraphs = 64;
 
@@ -119,7 +122,9 @@ struct DeviceDirPlan {
 };
 
 struct Workspace {
-    char* ptr = nullptr;
+    char* raw = nullptr;        // the allocation: chain state area, then `bytes` at ptr
+    char* ptr = nullptr;        // row buffer (and host-buffer staging)
+    uint32_t* chainState = nullptr; // kChainStateBytes, zero whenever no traversal runs
     size_t bytes = 0;
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
@@ -131,6 +136,13 @@ struct Tuning {
     uint32_t teamLanes = 0; // 0 = default
     uint32_t useGraph = 0;
     uint32_t xcdMap = 0;    // 0 = default (off), 1 = XCD-aware block order, 2 = off
+    uint32_t chain = 0;     // 0 = default, 1 = level-by-level launches, 2 = chained when eligible
+};
+
+// Host-mapped control words of a graph's chained traversals (page-locked, device-visible).
+struct ChainControl {
+    uint32_t abortWord; // nonzero: chained traversals give up at their next wait
+    uint32_t gaveUp;    // 1: a chained traversal gave up (its outputs are undefined)
 };
 
 // A level sequence as an instantiated hipGraph.  Held by shared_ptr: a traversal keeps its reference from the
@@ -159,6 +171,10 @@ struct grgl_graph {
     bool hasMiss = false, hasCoal = false;
     grgl_amd::DeviceDirPlan up, down;
     uint64_t deviceBytes = 0;
+    grgl_amd::ChainControl* chainCtl = nullptr;  // host pointer (hipHostMalloc, mapped)
+    grgl_amd::ChainControl* chainCtlDev = nullptr; // its device address
+    std::atomic<uint32_t> chainWaitUs{200000};  // longest wait of a chained traversal, us
+    uint64_t wallClockKHz = 100000;              // rate of the device clock the waits read
     std::mutex mu;
     std::vector<std::unique_ptr<grgl_amd::Workspace>> workspaces; // stable addresses
     std::vector<hipStream_t> freeStreams;
@@ -197,6 +213,9 @@ struct TravOptions {
     // Direct kernel launches only: no hipGraph, no timing events (the allocation-free entry
     // point, whose callers may not allocate).
     bool launchOnly = false;
+    // kChainStateBytes of zeroed device memory owned by the caller's buffer, or null: chained
+    // (single-launch) level sequences are possible only with it.
+    uint32_t* chainState = nullptr;
 };
 
 // Size in bytes of the row buffer for a traversal of Ktotal stacked rows.

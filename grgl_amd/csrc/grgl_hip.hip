@@ -296,6 +296,68 @@ const void* pullKernel(TypeClass tc, uint32_t vb, uint32_t tw, bool h32) {
     throw std::logic_error("bad type class");
 }
 
+// Chained kernels exist for the narrow-team shapes (TW <= 4, or accesses under 16 bytes) of
+// 4- and 8-byte types; H32 for 32-byte rows.  Null: no chained kernel for this shape.
+template <typename T> const void* chainKernelT(uint32_t vb, uint32_t tw, bool h32) {
+#define GRGL_CHAIN(VB, TW) return reinterpret_cast<const void*>(&pull_chain_kernel<T, VB, TW, false, false>)
+#define GRGL_CHAIN32(VB, TW) return reinterpret_cast<const void*>(&pull_chain_kernel<T, VB, TW, false, true>)
+    if (h32) {
+        if (vb == 16 && tw == 1) GRGL_CHAIN32(16, 1);
+        if (vb == 16 && tw == 2) GRGL_CHAIN32(16, 2);
+        if (vb == 8 && tw == 4) GRGL_CHAIN32(8, 4);
+        if (vb == 8 && tw == 8) GRGL_CHAIN32(8, 8);
+        if constexpr (sizeof(T) <= 4) {
+            if (vb == 4 && tw == 8) GRGL_CHAIN32(4, 8);
+        }
+        return nullptr;
+    }
+    if (vb == 4) {
+        if constexpr (sizeof(T) <= 4) {
+            if (tw == 1) GRGL_CHAIN(4, 1);
+            if (tw == 2) GRGL_CHAIN(4, 2);
+            if (tw == 4) GRGL_CHAIN(4, 4);
+            if (tw == 8) GRGL_CHAIN(4, 8);
+        }
+    } else if (vb == 8) {
+        if (tw == 1) GRGL_CHAIN(8, 1);
+        if (tw == 2) GRGL_CHAIN(8, 2);
+        if (tw == 4) GRGL_CHAIN(8, 4);
+        if (tw == 8) GRGL_CHAIN(8, 8);
+    } else if (vb == 16) {
+        if (tw == 1) GRGL_CHAIN(16, 1);
+        if (tw == 2) GRGL_CHAIN(16, 2);
+        if (tw == 4) GRGL_CHAIN(16, 4);
+    }
+#undef GRGL_CHAIN
+#undef GRGL_CHAIN32
+    return nullptr;
+}
+
+const void* chainKernel(TypeClass tc, uint32_t vb, uint32_t tw, bool h32) {
+    switch (tc) {
+    case kF64: return chainKernelT<double>(vb, tw, h32);
+    case kF32: return chainKernelT<float>(vb, tw, h32);
+    case kW64: return chainKernelT<uint64_t>(vb, tw, h32);
+    case kW32: return chainKernelT<uint32_t>(vb, tw, h32);
+    default: return nullptr;
+    }
+}
+
+// Resident blocks of `func` on the whole device (occupancy per compute unit x units), cached.
+uint32_t residentBlocks(int device, const void* func) {
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, uint32_t> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find({device, func});
+    if (it != cache.end()) return it->second;
+    int perCu = 0, cus = 0;
+    GRGL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, func, kBlock, 0));
+    GRGL_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    const uint32_t n = (uint32_t)std::max(1, perCu) * (uint32_t)std::max(1, cus);
+    cache[{device, func}] = n;
+    return n;
+}
+
 template <typename T, bool XOR> void launchSeedT(dim3 grid, const SeedArgs& a, bool vr, bool vm, hipStream_t s) {
     constexpr int V = 16 / (int)sizeof(T);
     if (vr && vm) hipLaunchKernelGGL((seed_rows_kernel<T, XOR, V, V>), grid, dim3(kBlock), 0, s, a);
@@ -512,6 +574,7 @@ Tuning chooseTuning(grgl_graph* g, grgl_direction dir, TypeClass tc, uint32_t K,
     // 200k x 1M graph FP32 DOWN K = 1 goes from 0.35 to 0.22 ms, K = 8 from 0.36 to 0.25 ms).
     if (tw == 0) tw = geo.lanes >= 3 ? std::min<uint32_t>(16, pow2ceil(geo.lanes)) : (geo.rowBytes <= 8 ? 4 : 8);
     t.teamLanes = std::min<uint32_t>(maxTeamLanes(geo), pow2ceil(tw));
+    if (t.chain == 0) t.chain = 1; // level-by-level launches unless chosen
     return t;
 }
 
@@ -668,15 +731,76 @@ void buildLevelGraph(std::vector<LevelLaunch>& launches, CapturedLevels& out) {
     GRGL_HIP_CHECK(hipGraphInstantiate(&out.exec, out.graph, nullptr, nullptr, 0));
 }
 
+// The level sequence as ONE chained launch (pull_chain_kernel), when the shape allows it: rows
+// of at most 64 bytes in one column slice on narrow teams, at most kMaxChainLevels pulled
+// levels each launched whole, fewer than 2^32 gather entries, a 4- or 8-byte type, and zeroed
+// chain state from the caller's buffer.  Returns false (nothing launched) otherwise.
+bool launchChained(grgl_graph* g, const DeviceDirPlan& plan, const LaunchTable& tab, TypeClass tc,
+                   const RowGeometry& geo, uint32_t tw, bool readSelf, bool orderPreserving, size_t elemSize,
+                   char* rows, uint32_t* chainState, hipStream_t stream) {
+    if (!chainState || !g->chainCtlDev || geo.rowBytes > 64) return false;
+    const uint32_t vb = levelVb(geo, tw, elemSize);
+    if (levelTiles(geo, tw, elemSize) != 1 || !(tw <= 4 || vb < 16)) return false;
+    const uint32_t first = plan.firstPulledLevel;
+    const uint32_t n = plan.numLevels > first ? plan.numLevels - first : 0;
+    if (n == 0 || n > (uint32_t)kMaxChainLevels || plan.numEntries >= (1ULL << 32)) return false;
+    for (uint32_t l = first; l < plan.numLevels; l++)
+        if (tab.levelChunk[l + 1] - tab.levelChunk[l] != 1) return false;
+    const void* func = chainKernel(tc, vb, tw, geo.rowBytes == 32);
+    if (!func) return false;
+    ChainArgs c{};
+    c.rows = rows;
+    c.ld = geo.ld;
+    c.off = plan.off;
+    c.src = plan.src;
+    c.rowBytes = geo.rowBytes;
+    c.readSelf = readSelf ? 1 : 0;
+    c.numLevels = n;
+    c.state = chainState;
+    c.gaveUp = &g->chainCtlDev->gaveUp;
+    c.abortWord = &g->chainCtlDev->abortWord;
+    c.waitTicks = (uint64_t)g->chainWaitUs.load() * g->wallClockKHz / 1000;
+    const uint32_t teamsPerBlock = kBlock / tw;
+    uint32_t task = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t l = first + i;
+        const LevelChunk& ck = tab.chunks[tab.levelChunk[l]];
+        const uint32_t heavy = orderPreserving ? 0 : ck.heavy;
+        const uint32_t light = ck.rowEnd - ck.rowBegin - heavy;
+        // One heavy row per wavefront where the level kernels use it (levelLaunches).
+        const bool manyHeavy = heavy >= 512;
+        const bool h32 = geo.rowBytes == 32 && tw <= 8 && manyHeavy && tab.heavyEntries[l] * 4 >= tab.levelEntries[l];
+        const bool perWave = (geo.rowBytes <= 16 && manyHeavy) || h32;
+        const uint32_t heavyBlocks = perWave ? (heavy + kBlock / 64 - 1) / (kBlock / 64) : heavy;
+        const uint32_t tasks = heavyBlocks + (light + teamsPerBlock - 1) / teamsPerBlock;
+        if (tasks == 0) return false; // every level must hold work (a complete level implies the ones below)
+        c.taskStart[i] = task;
+        c.levelBegin[i] = ck.rowBegin;
+        c.heavyCount[i] = heavy;
+        c.heavyBlocks[i] = heavyBlocks | (perWave ? 0x80000000U : 0U);
+        task += tasks;
+    }
+    c.taskStart[n] = task;
+    c.levelBegin[n] = plan.levelStart[plan.numLevels];
+    c.totalTasks = task;
+    const uint32_t grid = std::min<uint32_t>(task, residentBlocks(g->device, func));
+    void* args[1] = {&c};
+    GRGL_HIP_CHECK(hipLaunchKernel(func, dim3(grid), dim3(kBlock), args, 0, stream));
+    return true;
+}
+
 // Level sequence, either launched directly or replayed from a hipGraph of it.  A graph bakes
 // in the row buffer address, the row geometry, the team width and the launch table, so the
 // cache key holds all of them; different workspaces get different graphs.
 void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometry& geo, const Tuning& t,
                bool readSelf, bool orderPreserving, uint32_t K, size_t es, char* rows, hipStream_t stream,
-               bool launchOnly = false) {
+               bool launchOnly = false, uint32_t* chainState = nullptr) {
     const DeviceDirPlan& plan = dir == GRGL_DIRECTION_UP ? g->up : g->down;
     const std::shared_ptr<const LaunchTable> tab = launchTable(g, dir);
     const bool timing = g->timing && !launchOnly;
+    if (t.chain == 2 && !timing && !launchOnly &&
+        launchChained(g, plan, *tab, tc, geo, t.teamLanes, readSelf, orderPreserving, es, rows, chainState, stream))
+        return;
     bool direct = !t.useGraph || timing || launchOnly;
     if (!direct && stream != nullptr) {
         // A stream the caller is capturing gets direct launches (they become part of the
@@ -726,8 +850,16 @@ void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometr
 
 } // namespace
 
+void checkChainHealth(const grgl_graph* g) {
+    if (g->chainCtl && __atomic_load_n(&g->chainCtl->gaveUp, __ATOMIC_ACQUIRE) != 0)
+        throw DeviceError("a chained traversal on this graph gave up waiting (abort word set, or a wait longer than " +
+                          std::to_string(g->chainWaitUs.load()) + " us): its outputs are undefined; "
+                          "grgl_hip_chain_control(graph, ..., reset = 1) clears this state");
+}
+
 void enqueueTraversal(grgl_graph* g, grgl_dtype dt, grgl_direction dir, const std::vector<TravRequest>& reqs,
                       uint32_t Ktotal, const TravOptions& opt, char* rows, hipStream_t stream) {
+    checkChainHealth(g);
     const TypeClass tc = typeClassOf(dt);
     const size_t es = dtypeSize(dt);
     const RowGeometry geo = rowGeometry(Ktotal, es);
@@ -822,7 +954,8 @@ void enqueueTraversal(grgl_graph* g, grgl_dtype dt, grgl_direction dir, const st
     const Tuning tuning = chooseTuning(g, dir, tc, Ktotal, geo);
     // ORDER_PRESERVING: every list is summed by one team front to back, i.e. in the
     // reference's order.
-    runLevels(g, dir, tc, geo, tuning, hasInit, opt.orderPreserving, Ktotal, es, rows, stream, opt.launchOnly);
+    runLevels(g, dir, tc, geo, tuning, hasInit, opt.orderPreserving, Ktotal, es, rows, stream, opt.launchOnly,
+              opt.chainState);
     // 4. Read-out.  Several requests: one stacked launch for the main outputs.
     const bool stackOut = !stacked.empty() && !(dir == GRGL_DIRECTION_UP && opt.miss && plan.missRow && !opt.emitAll);
     if (stackOut) {
@@ -885,16 +1018,19 @@ Workspace* acquireWorkspace(grgl_graph* g, size_t bytes, hipStream_t stream) {
         }
     }
     if (victim) {
-        (void)hipFree(victim->ptr);
+        (void)hipFree(victim->raw);
         if (victim->done) (void)hipEventDestroy(victim->done);
     }
     std::unique_ptr<Workspace> w(new Workspace());
     const size_t sz = std::max<size_t>(bytes, 1 << 20);
-    GRGL_HIP_CHECK(hipMalloc(&w->ptr, sz));
+    GRGL_HIP_CHECK(hipMalloc(&w->raw, sz + kChainStateBytes));
+    w->chainState = reinterpret_cast<uint32_t*>(w->raw);
+    w->ptr = w->raw + kChainStateBytes;
     w->bytes = sz;
-    const hipError_t ev = hipEventCreateWithFlags(&w->done, hipEventDisableTiming);
+    hipError_t ev = hipMemset(w->raw, 0, kChainStateBytes);
+    if (ev == hipSuccess) ev = hipEventCreateWithFlags(&w->done, hipEventDisableTiming);
     if (ev != hipSuccess) {
-        (void)hipFree(w->ptr);
+        (void)hipFree(w->raw);
         GRGL_HIP_CHECK(ev);
     }
     w->busy = true;
@@ -1103,9 +1239,20 @@ GRGL_EXPORT grgl_status grgl_hip_graph_create(const grgl_graph_desc* desc, int d
     try {
         uploadPlan(g->up, up, g->deviceBytes);
         uploadPlan(g->down, down, g->deviceBytes);
+        void* ctl = nullptr;
+        GRGL_HIP_CHECK(hipHostMalloc(&ctl, sizeof(ChainControl), hipHostMallocMapped));
+        g->chainCtl = static_cast<ChainControl*>(ctl);
+        std::memset(g->chainCtl, 0, sizeof(ChainControl));
+        void* dctl = nullptr;
+        GRGL_HIP_CHECK(hipHostGetDevicePointer(&dctl, ctl, 0));
+        g->chainCtlDev = static_cast<ChainControl*>(dctl);
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+            g->wallClockKHz = (uint64_t)khz;
     } catch (...) {
         freePlan(g->up);
         freePlan(g->down);
+        if (g->chainCtl) (void)hipHostFree(g->chainCtl);
         throw;
     }
     *out = g.release();
@@ -1119,7 +1266,7 @@ GRGL_EXPORT grgl_status grgl_hip_graph_destroy(grgl_graph* g) {
     (void)hipDeviceSynchronize();
     g->graphs.clear(); // CapturedLevels destroys its graph, exec and event
     for (auto& w : g->workspaces) {
-        (void)hipFree(w->ptr);
+        (void)hipFree(w->raw);
         if (w->done) (void)hipEventDestroy(w->done);
     }
     for (auto s : g->freeStreams) (void)hipStreamDestroy(s);
@@ -1129,6 +1276,7 @@ GRGL_EXPORT grgl_status grgl_hip_graph_destroy(grgl_graph* g) {
     }
     freePlan(g->up);
     freePlan(g->down);
+    if (g->chainCtl) (void)hipHostFree(g->chainCtl);
     delete g;
     GRGL_API_END
 }
@@ -1276,6 +1424,7 @@ void matmulRows(grgl_graph* g, grgl_dtype dtype, const void* input, size_t input
         opt.miss = dMiss;
         opt.accumulate = accumulate;
         opt.orderPreserving = (flags & GRGL_FLAG_ORDER_PRESERVING) != 0;
+        opt.chainState = ws->chainState;
         std::vector<TravRequest> reqs{{K, dIn, dOut, 0}};
         enqueueTraversal(g, dtype, direction, reqs, K, opt, base, s);
         if (host) {
@@ -1283,6 +1432,7 @@ void matmulRows(grgl_graph* g, grgl_dtype dtype, const void* input, size_t input
             if (missBytes && direction == GRGL_DIRECTION_UP)
                 copyHostSplit(g, miss, dMiss, missBytes, hipMemcpyDeviceToHost, s);
             GRGL_HIP_CHECK(hipStreamSynchronize(s));
+            checkChainHealth(g);
         }
     } catch (...) {
         releaseWorkspace(g, ws, s);
@@ -1405,6 +1555,7 @@ GRGL_EXPORT grgl_status grgl_hip_matmul_batch(grgl_graph* g, grgl_dtype dtype, g
         TravOptions opt;
         opt.accumulate = accumulate;
         opt.orderPreserving = (flags & GRGL_FLAG_ORDER_PRESERVING) != 0;
+        opt.chainState = ws->chainState;
         enqueueTraversal(g, dtype, direction, reqs, K, opt, base, s);
         if (host) {
             for (uint32_t i = 0; i < num_requests; i++) {
@@ -1412,6 +1563,7 @@ GRGL_EXPORT grgl_status grgl_hip_matmul_batch(grgl_graph* g, grgl_dtype dtype, g
                                               hipMemcpyDeviceToHost, s));
             }
             GRGL_HIP_CHECK(hipStreamSynchronize(s));
+            checkChainHealth(g);
         }
     } catch (...) {
         releaseWorkspace(g, ws, s);
@@ -1436,6 +1588,8 @@ GRGL_EXPORT grgl_status grgl_hip_set_tuning(grgl_graph* g, grgl_direction direct
     t.useGraph = tuning->use_graph;
     if (tuning->xcd_map > 2) throw MisuseError("xcd_map must be 0, 1 or 2");
     t.xcdMap = tuning->xcd_map;
+    if (tuning->chain > 2) throw MisuseError("chain must be 0, 1 or 2");
+    t.chain = tuning->chain;
     g->tuning[std::make_tuple((int)direction, (int)tc, rows)] = t;
     if (tuning->heavy_fanin || tuning->max_launch_entries) {
         // A new launch table replaces the direction's current one; traversals already
@@ -1446,6 +1600,18 @@ GRGL_EXPORT grgl_status grgl_hip_set_tuning(grgl_graph* g, grgl_direction direct
                                                           : plan.table->chunkEntries;
         if (heavy != plan.table->heavyFanin || chunk != plan.table->chunkEntries) plan.table = plan.buildTable(heavy, chunk);
     }
+    GRGL_API_END
+}
+
+GRGL_EXPORT grgl_status grgl_hip_chain_control(grgl_graph* g, int abort, uint32_t wait_us, uint32_t* gave_up,
+                                               int reset) {
+    GRGL_API_BEGIN
+    if (!g) throw MisuseError("null graph");
+    if (abort < -1 || abort > 1) throw MisuseError("abort must be -1, 0 or 1");
+    if (abort >= 0) __atomic_store_n(&g->chainCtl->abortWord, (uint32_t)abort, __ATOMIC_RELEASE);
+    if (wait_us) g->chainWaitUs = wait_us;
+    if (gave_up) *gave_up = __atomic_load_n(&g->chainCtl->gaveUp, __ATOMIC_ACQUIRE);
+    if (reset) __atomic_store_n(&g->chainCtl->gaveUp, 0U, __ATOMIC_RELEASE);
     GRGL_API_END
 }
 
@@ -1463,6 +1629,7 @@ GRGL_EXPORT grgl_status grgl_hip_get_tuning(grgl_graph* g, grgl_direction direct
     out->heavy_fanin = plan.table->heavyFanin;
     out->use_graph = t.useGraph;
     out->xcd_map = t.xcdMap;
+    out->chain = t.chain;
     out->max_launch_entries = (uint32_t)plan.table->chunkEntries;
     GRGL_API_END
 }
@@ -1499,7 +1666,7 @@ GRGL_EXPORT grgl_status grgl_hip_tuning_save(grgl_graph* g, const char* path) {
     GRGL_API_BEGIN
     if (!g || !path) throw MisuseError("null argument");
     std::ostringstream os;
-    os << kTuningMagic << " 1\n";
+    os << kTuningMagic << " 2\n";
     char fp[32];
     std::snprintf(fp, sizeof(fp), "%016llx", (unsigned long long)tuningFingerprint(g));
     os << "graph " << fp << "\n";
@@ -1512,7 +1679,7 @@ GRGL_EXPORT grgl_status grgl_hip_tuning_save(grgl_graph* g, const char* path) {
         for (const auto& kv : g->tuning) {
             const Tuning& t = kv.second;
             os << "entry " << std::get<0>(kv.first) << " " << std::get<1>(kv.first) << " " << std::get<2>(kv.first) << " "
-               << t.teamLanes << " " << t.useGraph << " " << t.xcdMap << "\n";
+               << t.teamLanes << " " << t.useGraph << " " << t.xcdMap << " " << t.chain << "\n";
         }
     }
     std::ofstream f(path, std::ios::trunc);
@@ -1529,7 +1696,9 @@ GRGL_EXPORT grgl_status grgl_hip_tuning_load(grgl_graph* g, const char* path) {
     if (!f) throw MisuseError(std::string("cannot read tuning file ") + path);
     std::string magic, word, fp;
     int version = 0;
-    if (!(f >> magic >> version) || magic != kTuningMagic || version != 1) throw MisuseError("not a tuning file");
+    // Version 1 files carry no chain column (their entries keep the default).
+    if (!(f >> magic >> version) || magic != kTuningMagic || (version != 1 && version != 2))
+        throw MisuseError("not a tuning file");
     if (!(f >> word >> fp) || word != "graph") throw MisuseError("malformed tuning file");
     char want[32];
     std::snprintf(want, sizeof(want), "%016llx", (unsigned long long)tuningFingerprint(g));
@@ -1547,14 +1716,16 @@ GRGL_EXPORT grgl_status grgl_hip_tuning_load(grgl_graph* g, const char* path) {
             dirs.push_back(d);
         } else if (word == "entry") {
             int dir = 0, tc = 0;
-            uint64_t rows = 0, tw = 0, ug = 0, xcd = 0;
-            if (!(f >> dir >> tc >> rows >> tw >> ug >> xcd) || (dir != GRGL_DIRECTION_UP && dir != GRGL_DIRECTION_DOWN) ||
-                tc < 0 || tc > (int)kX8 || rows == 0 || rows > 65536 || tw > 64 || ug > 1 || xcd > 2)
+            uint64_t rows = 0, tw = 0, ug = 0, xcd = 0, chain = 0;
+            if (!(f >> dir >> tc >> rows >> tw >> ug >> xcd) || (version >= 2 && !(f >> chain)) ||
+                (dir != GRGL_DIRECTION_UP && dir != GRGL_DIRECTION_DOWN) || tc < 0 || tc > (int)kX8 || rows == 0 ||
+                rows > 65536 || tw > 64 || ug > 1 || xcd > 2 || chain > 2)
                 throw MisuseError("malformed tuning file (entry)");
             Tuning t;
             t.teamLanes = (uint32_t)tw;
             t.useGraph = (uint32_t)ug;
             t.xcdMap = (uint32_t)xcd;
+            t.chain = (uint32_t)chain;
             entries.push_back({std::make_tuple(dir, tc, (uint32_t)rows), t});
         } else {
             throw MisuseError("malformed tuning file");
@@ -1641,31 +1812,47 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
         GRGL_HIP_CHECK(hipEventCreate(&e0));
         GRGL_HIP_CHECK(hipEventCreate(&e1));
         GRGL_HIP_CHECK(hipMemsetAsync(ws->ptr, 0, rowsBufferBytes(g, direction, rows, es), s));
-        for (uint32_t trialIdx = 0; trialIdx < 2 * tws.size() * 2; trialIdx++) {
-            const uint32_t useGraph = trialIdx % 2, xcd = 1 + (trialIdx / 2) % 2;
-            const uint32_t tw = tws[trialIdx / 4];
-            if (xcd == 2 && levelTiles(geo, tw, es) % 8 != 0) continue; // same launch as xcd = 1
-            {
-                Tuning t;
-                t.teamLanes = tw;
-                t.useGraph = useGraph;
-                t.xcdMap = xcd;
-                runLevels(g, direction, tc, geo, t, false, false, rows, es, ws->ptr, s); // warm-up / capture
-                float trial[3];
-                for (float& ms : trial) {
-                    GRGL_HIP_CHECK(hipEventRecord(e0, s));
-                    runLevels(g, direction, tc, geo, t, false, false, rows, es, ws->ptr, s);
-                    GRGL_HIP_CHECK(hipEventRecord(e1, s));
-                    GRGL_HIP_CHECK(hipEventSynchronize(e1));
-                    GRGL_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-                }
-                std::sort(trial, trial + 3);
-                if (trial[1] < bestMs) {
-                    bestMs = trial[1];
-                    best = t;
+        // Candidates: every team width with direct or graph launches and both block orders
+        // (the XCD-aware order only where it differs), and, for rows of at most 64 bytes, the
+        // chained single launch at every team width.
+        std::vector<Tuning> cands;
+        for (uint32_t tw : tws) {
+            for (uint32_t useGraph = 0; useGraph < 2; useGraph++) {
+                for (uint32_t xcd = 1; xcd <= 2; xcd++) {
+                    if (xcd == 2 && levelTiles(geo, tw, es) % 8 != 0) continue; // same launch as xcd = 1
+                    Tuning t;
+                    t.teamLanes = tw;
+                    t.useGraph = useGraph;
+                    t.xcdMap = xcd;
+                    t.chain = 1;
+                    cands.push_back(t);
                 }
             }
+            if (geo.rowBytes <= 64) {
+                Tuning t;
+                t.teamLanes = tw;
+                t.xcdMap = 1;
+                t.chain = 2;
+                cands.push_back(t);
+            }
         }
+        for (const Tuning& t : cands) {
+            runLevels(g, direction, tc, geo, t, false, false, rows, es, ws->ptr, s, false, ws->chainState); // warm-up
+            float trial[3];
+            for (float& ms : trial) {
+                GRGL_HIP_CHECK(hipEventRecord(e0, s));
+                runLevels(g, direction, tc, geo, t, false, false, rows, es, ws->ptr, s, false, ws->chainState);
+                GRGL_HIP_CHECK(hipEventRecord(e1, s));
+                GRGL_HIP_CHECK(hipEventSynchronize(e1));
+                GRGL_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            }
+            std::sort(trial, trial + 3);
+            if (trial[1] < bestMs) {
+                bestMs = trial[1];
+                best = t;
+            }
+        }
+        checkChainHealth(g);
     } catch (...) {
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
@@ -1689,6 +1876,7 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
         chosen->heavy_fanin = tab->heavyFanin;
         chosen->use_graph = best.useGraph;
         chosen->xcd_map = best.xcdMap;
+        chosen->chain = best.chain;
         chosen->max_launch_entries = (uint32_t)tab->chunkEntries;
     }
     if (best_ms) *best_ms = bestMs;

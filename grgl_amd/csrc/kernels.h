@@ -26,6 +26,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "device_graph.h"
+
 namespace grgl_amd {
 
 constexpr uint32_t kInvalidRow = 0x7fffffffU;
@@ -314,34 +316,48 @@ struct StagedIdx {
     __device__ uint32_t operator()(uint32_t i) const { return lds[i - base]; }
 };
 
-// H32: the variant launched for levels with heavy rows when rows are 32 bytes; its
-// per-wavefront heavy path holds 32 bytes per lane, whose registers would lower the
-// occupancy of the levels without heavy rows, so they use the plain variant.
-template <typename T, int VB, int TW, bool XOR, bool H32 = false>
-__global__ __launch_bounds__(kBlock) void pull_level_kernel(PullArgs a) {
+// Waiting hook of pull_level_block.  The level kernel needs none: kernel boundaries order the
+// levels.  The chained kernel waits there for the previous level: poll() runs before a block
+// barrier (thread 0 polls a completion counter), acquire() after it (false: the wait gave up,
+// and the block does no work).
+struct NoWait {
+    static constexpr bool kWaits = false;
+    __device__ void poll() const {}
+    __device__ bool acquire() const { return true; }
+};
+
+// One block's share of a level: heavy rows (a block each, or 4 per block one per wavefront)
+// or NT light rows.  The gather-list structure (offsets, indices) is read before the wait,
+// the rows after it.  Returns false when the wait gave up (nothing written).
+template <typename T, int VB, int TW, bool XOR, bool H32, typename Wait>
+__device__ __forceinline__ bool pull_level_block(const PullArgs& a, uint32_t bx, uint32_t by, const Wait& wait) {
     const uint32_t lane = threadIdx.x % TW;
     const uint32_t team = threadIdx.x / TW;
-    uint32_t bx, by;
-    pull_block_coords(a, bx, by);
     const uint64_t colByte = ((uint64_t)by * TW + lane) * VB;
     const bool active = colByte < a.rowBytes;
     constexpr uint32_t NT = kBlock / TW;
     if (bx < a.heavyBlocks) {
+        if constexpr (Wait::kWaits) {
+            wait.poll();
+            __syncthreads();
+            if (!wait.acquire()) return false;
+        }
         if (a.heavyPerWave) {
             // Short rows: one wavefront per heavy row, each lane one partial holding the
             // whole row (the single-lane-team arithmetic: same partials, same tree), so a
             // block takes 4 heavy rows and no lane idles.
             const uint32_t h = bx * (kBlock / 64) + threadIdx.x / 64;
-            if (h >= a.heavyCount) return;
-            const uint32_t row = a.levelBegin + h, part = threadIdx.x % 64;
-            if (a.rowBytes == 4) {
-                if constexpr (sizeof(T) <= 4) pull_heavy<T, 4, 1, XOR>(a, row, 0, true, part, 0);
-            } else if (a.rowBytes == 8) {
-                if constexpr (sizeof(T) <= 8) pull_heavy<T, 8, 1, XOR>(a, row, 0, true, part, 0);
-            } else if (a.rowBytes == 16) {
-                pull_heavy<T, 16, 1, XOR>(a, row, 0, true, part, 0);
-            } else if constexpr (H32) {
-                pull_heavy<T, 32, 1, XOR>(a, row, 0, true, part, 0);
+            if (h < a.heavyCount) {
+                const uint32_t row = a.levelBegin + h, part = threadIdx.x % 64;
+                if (a.rowBytes == 4) {
+                    if constexpr (sizeof(T) <= 4) pull_heavy<T, 4, 1, XOR>(a, row, 0, true, part, 0);
+                } else if (a.rowBytes == 8) {
+                    if constexpr (sizeof(T) <= 8) pull_heavy<T, 8, 1, XOR>(a, row, 0, true, part, 0);
+                } else if (a.rowBytes == 16) {
+                    pull_heavy<T, 16, 1, XOR>(a, row, 0, true, part, 0);
+                } else if constexpr (H32) {
+                    pull_heavy<T, 32, 1, XOR>(a, row, 0, true, part, 0);
+                }
             }
         } else {
             pull_heavy<T, VB, TW, XOR>(a, a.levelBegin + bx, colByte, active, team, lane);
@@ -375,15 +391,38 @@ __global__ __launch_bounds__(kBlock) void pull_level_kernel(PullArgs a) {
                 const uint32_t t = threadIdx.x + (uint32_t)u * kBlock;
                 if (t < n) staged[t] = v[u];
             }
+            wait.poll();
             __syncthreads();
+            if (!wait.acquire()) return false;
             if (row < r1) pull_light_range<T, VB, TW, XOR, UNR>(a, row, rb, re, colByte, active, StagedIdx{staged, e0});
-        } else if (row < r1) {
-            pull_light_range<T, VB, TW, XOR, UNR>(a, row, rb, re, colByte, active, GlobalIdx{a.src});
+        } else {
+            if constexpr (Wait::kWaits) {
+                wait.poll();
+                __syncthreads();
+                if (!wait.acquire()) return false;
+            }
+            if (row < r1) pull_light_range<T, VB, TW, XOR, UNR>(a, row, rb, re, colByte, active, GlobalIdx{a.src});
         }
     } else {
+        if constexpr (Wait::kWaits) {
+            wait.poll();
+            __syncthreads();
+            if (!wait.acquire()) return false;
+        }
         const uint32_t row = a.levelBegin + a.heavyCount + (bx - a.heavyBlocks) * NT + team;
         if (row < a.levelEnd) pull_light<T, VB, TW, XOR, 4>(a, row, colByte, active, GlobalIdx{a.src});
     }
+    return true;
+}
+
+// H32: the variant launched for levels with heavy rows when rows are 32 bytes; its
+// per-wavefront heavy path holds 32 bytes per lane, whose registers would lower the
+// occupancy of the levels without heavy rows, so they use the plain variant.
+template <typename T, int VB, int TW, bool XOR, bool H32 = false>
+__global__ __launch_bounds__(kBlock) void pull_level_kernel(PullArgs a) {
+    uint32_t bx, by;
+    pull_block_coords(a, bx, by);
+    (void)pull_level_block<T, VB, TW, XOR, H32>(a, bx, by, NoWait{});
 }
 
 // The trailing levels of a traversal that are too small to fill the chip (at most two
@@ -435,6 +474,147 @@ __global__ __launch_bounds__(kBlock) void pull_tail_kernel(TailArgs t) {
                 pull_light<T, VB, TW, XOR, 4>(a, row, colByte, active, GlobalIdx{a.src});
         }
         __syncthreads(); // the next level reads rows written by other waves of this block
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Chained traversal: every pulled level of a direction in ONE launch, with no grid barrier.
+// The work is cut into the same per-block tasks as the level kernels (a level's heavy blocks,
+// then its light blocks, levels in order) and numbered in that order.  Blocks claim task
+// numbers from a ticket counter; a task of level l first reads its gather-list structure, then
+// waits until every task of level l - 1 has completed (a per-level completion counter), then
+// gathers and stores its rows and counts itself complete.
+//   Progress: tickets are claimed in order and a block finishes its task before claiming the
+//   next, so the lowest unfinished task is always held by a running block whose predecessors
+//   are complete; no wait depends on a block that has not started, whatever the grid size or
+//   residency.
+//   Ordering: rows are stored, the block synchronises, and thread 0 increments the counter
+//   with agent-scope release; the waiting block's thread 0 reads the counter, the block
+//   synchronises, and every wavefront performs an agent-scope acquire before loading rows.
+//   Each level's tasks waited for the level below, so a complete level l - 1 implies complete
+//   levels below it (happens-before is transitive).
+//   Bounded: a wait gives up after `waitTicks` of the constant 100 MHz clock, when another
+//   block gave up (a flag in `state`), or when the host-mapped abort word is nonzero (read at
+//   block start and on every 64th poll); a block that gives up stops claiming tasks, so the
+//   kernel always ends.  The last block to leave sets the host-mapped `gaveUp` word if any
+//   block gave up (a store: no atomics on host memory) and clears `state` (ticket, exits, flag,
+//   counters) for the next traversal.
+// Each row goes through pull_level_block, the level kernels' code, so results are bit for bit
+// those of the level-by-level launches.
+// ---------------------------------------------------------------------------------------
+constexpr int kMaxChainLevels = 120;
+constexpr uint32_t kChainHeader = 4; // state words: ticket, exits, gave-up flag, unused
+static_assert((kChainHeader + kMaxChainLevels) * 4 <= kChainStateBytes, "chain state area");
+
+struct ChainArgs {
+    char* rows;
+    uint64_t ld;
+    const uint32_t* off;
+    const uint32_t* src;        // all gather entries (fewer than 2^32)
+    uint32_t rowBytes;
+    uint32_t readSelf;
+    uint32_t numLevels;         // pulled levels
+    uint32_t totalTasks;
+    uint32_t* state;            // device words, zero at launch: ticket, exits, flag, counters
+    uint32_t* gaveUp;           // host-mapped: set to 1 when a traversal gave up
+    const uint32_t* abortWord;  // host-mapped: nonzero = give up
+    uint64_t waitTicks;         // longest wait, in ticks of wall_clock64()
+    uint32_t taskStart[kMaxChainLevels + 1];  // first task of each level; [numLevels] = total
+    uint32_t levelBegin[kMaxChainLevels + 1]; // first row of each level; [numLevels] = end
+    uint32_t heavyCount[kMaxChainLevels];
+    uint32_t heavyBlocks[kMaxChainLevels];    // bit 31: one heavy row per wavefront
+};
+
+struct ChainWait {
+    static constexpr bool kWaits = true;
+    const ChainArgs* c;
+    uint32_t level;
+    uint32_t* flag; // LDS: 1 = go, 0 = gave up
+    __device__ bool giveUp() const {
+        return __hip_atomic_load(c->state + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+               __hip_atomic_load(const_cast<uint32_t*>(c->abortWord), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0;
+    }
+    __device__ void poll() const {
+        if (threadIdx.x != 0) return;
+        uint32_t ok = 1;
+        if (level > 0) {
+            const uint32_t need = c->taskStart[level] - c->taskStart[level - 1];
+            uint32_t* ctr = c->state + kChainHeader + level - 1;
+            if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                const uint64_t t0 = wall_clock64();
+                for (uint32_t spin = 0;; spin++) {
+                    if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need) break;
+                    if ((spin & 63) == 63 && (giveUp() || wall_clock64() - t0 > c->waitTicks)) {
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+        }
+        *flag = ok;
+    }
+    __device__ bool acquire() const {
+        if (*flag == 0) return false;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        return true;
+    }
+};
+
+template <typename T, int VB, int TW, bool XOR, bool H32>
+__global__ __launch_bounds__(kBlock) void pull_chain_kernel(ChainArgs c) {
+    __shared__ uint32_t sTask, sOk;
+    bool stop = false;
+    if (threadIdx.x == 0 &&
+        __hip_atomic_load(const_cast<uint32_t*>(c.abortWord), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0)
+        __hip_atomic_store(c.state + 2, 1U, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (!stop) {
+        if (threadIdx.x == 0) {
+            uint32_t t = c.totalTasks;
+            if (__hip_atomic_load(c.state + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                t = __hip_atomic_fetch_add(c.state, 1U, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            sTask = t;
+        }
+        __syncthreads();
+        const uint32_t task = sTask;
+        if (task >= c.totalTasks) break; // block-uniform
+        uint32_t lo = 0, hi = c.numLevels - 1; // last level whose first task is <= task
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) / 2;
+            if (c.taskStart[mid] <= task) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t l = lo;
+        PullArgs a;
+        a.rows = c.rows;
+        a.ld = c.ld;
+        a.off = c.off;
+        a.src = c.src;
+        a.entryBase = 0;
+        a.levelBegin = c.levelBegin[l];
+        a.heavyCount = c.heavyCount[l];
+        a.heavyBlocks = c.heavyBlocks[l] & 0x7fffffffU;
+        a.heavyPerWave = c.heavyBlocks[l] >> 31;
+        a.levelEnd = c.levelBegin[l + 1];
+        a.rowBytes = c.rowBytes;
+        a.readSelf = c.readSelf;
+        a.xcdTiles = 0;
+        const bool done = pull_level_block<T, VB, TW, XOR, H32>(a, task - c.taskStart[l], 0, ChainWait{&c, l, &sOk});
+        __syncthreads(); // the task's rows are stored; sTask and the LDS are free again
+        if (threadIdx.x == 0) {
+            if (done) __hip_atomic_fetch_add(c.state + kChainHeader + l, 1U, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            else __hip_atomic_store(c.state + 2, 1U, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        stop = !done;
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t out = __hip_atomic_fetch_add(c.state + 1, 1U, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (out == gridDim.x - 1) {
+            if (__hip_atomic_load(c.state + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+                __hip_atomic_store(c.gaveUp, 1U, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            for (uint32_t i = 0; i < kChainHeader + c.numLevels; i++)
+                __hip_atomic_store(c.state + i, 0U, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 

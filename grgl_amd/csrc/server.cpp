@@ -51,7 +51,9 @@ struct ServerRequest {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    char* dev = nullptr;
+    char* raw = nullptr;            // device allocation: chain state area, then `dev`
+    char* dev = nullptr;            // row buffer and staging of host requests
+    uint32_t* chainState = nullptr; // kChainStateBytes, zero whenever no traversal runs
     size_t devBytes = 0;
     char* pinned = nullptr;
     size_t pinnedBytes = 0;
@@ -100,11 +102,15 @@ namespace {
 
 void growDevice(Slot& s, size_t bytes) {
     if (s.devBytes >= bytes) return;
-    if (s.dev) GRGL_HIP_CHECK(hipFree(s.dev));
-    s.dev = nullptr;
+    if (s.raw) GRGL_HIP_CHECK(hipFree(s.raw));
+    s.raw = s.dev = nullptr;
+    s.chainState = nullptr;
     s.devBytes = 0;
     const size_t sz = std::max<size_t>(bytes + bytes / 4, 8u << 20);
-    GRGL_HIP_CHECK(hipMalloc(&s.dev, sz));
+    GRGL_HIP_CHECK(hipMalloc(&s.raw, sz + kChainStateBytes));
+    GRGL_HIP_CHECK(hipMemset(s.raw, 0, kChainStateBytes));
+    s.chainState = reinterpret_cast<uint32_t*>(s.raw);
+    s.dev = s.raw + kChainStateBytes;
     s.devBytes = sz;
 }
 
@@ -266,6 +272,7 @@ bool Server::launchBatch(Slot& s, std::unique_lock<std::mutex>& lk) {
             GRGL_HIP_CHECK(hipMemcpyAsync(devIn, pinIn, (size_t)K * inCols * es, hipMemcpyHostToDevice, s.stream));
         TravOptions opt;
         opt.orderPreserving = (cfg.flags & GRGL_FLAG_ORDER_PRESERVING) != 0;
+        opt.chainState = s.chainState;
         enqueueTraversal(g, head->dtype, head->dir, treqs, K, opt, s.dev, s.stream);
         if (anyHost)
             GRGL_HIP_CHECK(hipMemcpyAsync(s.pinned + alignUp(K * inCols * es), devOut, (size_t)K * outCols * es,
@@ -536,7 +543,7 @@ GRGL_EXPORT grgl_status grgl_hip_server_destroy(grgl_server* server) {
     (void)hipSetDevice(srv->cfg.device);
     for (auto& s : srv->slots) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
-        if (s.dev) (void)hipFree(s.dev);
+        if (s.raw) (void)hipFree(s.raw);
         if (s.pinned) (void)hipHostFree(s.pinned);
         if (s.done) (void)hipEventDestroy(s.done);
         if (s.stream) (void)hipStreamDestroy(s.stream);

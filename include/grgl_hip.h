@@ -194,10 +194,24 @@ typedef struct grgl_tuning {
                                     2^30, also the upper limit); larger levels are split into several
                                     launches.  Never changes results; like heavy_fanin it applies to
                                     every later call in that direction. */
+    uint32_t chain;          /* 0 = default, 1 = one launch per level, 2 = the whole level sequence as
+                                ONE chained launch where the shape allows it (rows of at most 64 bytes
+                                of a 4- or 8-byte type, at most 120 levels): blocks take the levels'
+                                work in order from a ticket counter and wait on per-level completion
+                                counters, with no grid barrier; bit-identical results.  Its waits are
+                                bounded (grgl_hip_chain_control). */
 } grgl_tuning;
 
 grgl_status grgl_hip_set_tuning(grgl_graph* graph, grgl_direction direction, grgl_dtype dtype, uint32_t rows,
                                 const grgl_tuning* tuning);
+/* Chained traversals (grgl_tuning::chain = 2) on `graph`: every wait of one is bounded by
+ * wait_us (default 200000 us; 0 leaves it unchanged) and gives up when the abort word is set
+ * (abort = 1 sets it, 0 clears it, -1 leaves it; blocks read it when they start and while they
+ * wait).  A traversal that gave up stops and leaves its outputs undefined, and the graph is
+ * marked: *gave_up (if not NULL) receives 1 when marked, 0 otherwise; reset = 1 clears the mark.
+ * While marked, every matmul call on the graph fails with GRGL_ERR_DEVICE (a call with host
+ * buffers also checks after its own traversal).  No counterpart in the reference. */
+grgl_status grgl_hip_chain_control(grgl_graph* graph, int abort, uint32_t wait_us, uint32_t* gave_up, int reset);
 /* Time candidate tunings on the device and keep the fastest; writes it to *chosen. */
 grgl_status grgl_hip_autotune(grgl_graph* graph, grgl_direction direction, grgl_dtype dtype, uint32_t rows,
                               grgl_tuning* chosen, float* best_ms);
