@@ -300,6 +300,9 @@ HostGrg MutableGraph::freeze() const {
     g.mutAlleles = mutAlleles_;
     g.mutAltIndex = mutAltIndex_;
     if (anyCoal_) g.coal = coal_;
+    // Adding mutations to a mutable graph leaves them unordered (grg.cpp:104-106): the .grg
+    // writer sorts them.
+    g.mutationsOrdered = g.mutNode.empty();
     validateHostGrg(g);
     return g;
 }

@@ -276,11 +276,129 @@ HostGrg readGrgFile(const std::string& path) {
         }
     }
     if (h.flags & kFlagMissing) g.mutMiss = r.array<uint32_t>(M);
+    g.mutationsOrdered = true;
     validateHostGrg(g);
     return g;
 }
 
-void writeGrgFile(const HostGrg& g, const std::string& path) {
+// Simplify-on-write (src/serialize.cpp:387-467, RenumberAndWriteVisitor, applied by
+// simplifyAndSerialize to every node of an ordered graph in ID order, grg_helpers.h:58-64):
+//   * samples keep their IDs;
+//   * a non-sample node's children are its down edges, where a child that was dropped is
+//     replaced by that child's own (expanded) children, recursively, adding the dropped child's
+//     coalescence count to the node's (serialize.cpp:149-177);
+//   * the node is dropped ("extraneous") when it then has at most one child, or at most one
+//     parent (up edges in the input graph), or exactly two children and two parents — unless
+//     it has children and carries a mutation or is a missingness node (:436-446);
+//   * kept nodes are renumbered in visiting order, children sorted by new ID; mutations (and
+//     missingness nodes) of dropped nodes become unmapped.
+HostGrg simplifyGrg(const HostGrg& g) {
+    validateHostGrg(g);
+    const uint32_t V = g.numNodes(), N = g.numSamples;
+    std::vector<uint32_t> numParents(V, 0);
+    for (uint64_t i = 0; i < g.numEdges(); i++) numParents[g.downChild[i]]++;
+    std::vector<uint8_t> special(V, 0); // carries a mutation or is a missingness node
+    for (uint32_t n : g.mutNode)
+        if (n != kInvalidNode) special[n] = 1;
+    for (uint32_t n : g.mutMiss)
+        if (n != kInvalidNode) special[n] = 1;
+    const bool coals = g.hasCoals();
+    std::vector<uint32_t> newId(V, kInvalidNode);
+    for (uint32_t s = 0; s < N; s++) newId[s] = s;
+    HostGrg out = g;
+    out.downOff.assign(1, 0);
+    out.downChild.clear();
+    out.coal.clear();
+    if (coals) out.coal.assign(g.coal.begin(), g.coal.begin() + N);
+    for (uint32_t s = 0; s < N; s++) out.downOff.push_back(0);
+    uint32_t counter = N;
+    std::vector<uint32_t> children, stack;
+    for (uint32_t n = N; n < V; n++) {
+        children.clear();
+        uint32_t nodeCoals = coals ? g.coal[n] : kCoalNotSet;
+        // Depth-first expansion of dropped children, in down-edge order.
+        stack.clear();
+        for (uint64_t i = g.downOff[n + 1]; i-- > g.downOff[n];) stack.push_back(g.downChild[i]);
+        while (!stack.empty()) {
+            const uint32_t c = stack.back();
+            stack.pop_back();
+            if (newId[c] != kInvalidNode) {
+                children.push_back(newId[c]);
+                continue;
+            }
+            if (nodeCoals != kCoalNotSet && coals && g.coal[c] != kCoalNotSet) nodeCoals += g.coal[c];
+            for (uint64_t i = g.downOff[c + 1]; i-- > g.downOff[c];) stack.push_back(g.downChild[i]);
+        }
+        const uint32_t parents = numParents[n];
+        const bool extraneous = children.size() <= 1 || parents <= 1 || (children.size() == 2 && parents == 2);
+        const bool keepMutNode = !children.empty() && special[n];
+        if (extraneous && !keepMutNode) continue;
+        newId[n] = counter++;
+        std::sort(children.begin(), children.end());
+        out.downChild.insert(out.downChild.end(), children.begin(), children.end());
+        out.downOff.push_back(out.downChild.size());
+        if (coals) out.coal.push_back(nodeCoals);
+    }
+    for (uint32_t& m : out.mutNode)
+        if (m != kInvalidNode) m = newId[m];
+    for (uint32_t& m : out.mutMiss)
+        if (m != kInvalidNode) m = newId[m];
+    return out;
+}
+
+namespace {
+
+bool mutationLess(const HostGrg& g, uint32_t a, uint32_t b) {
+    const double pa = g.mutPos.empty() ? (double)a : g.mutPos[a];
+    const double pb = g.mutPos.empty() ? (double)b : g.mutPos[b];
+    if (pa != pb) return pa < pb;
+    if (g.mutAlleles.empty()) return false;
+    const uint16_t ia = g.mutAltIndex.empty() ? 1 : g.mutAltIndex[a];
+    const uint16_t ib = g.mutAltIndex.empty() ? 1 : g.mutAltIndex[b];
+    const std::string refA = g.mutAlleles[a].substr(0, ia), refB = g.mutAlleles[b].substr(0, ib);
+    if (refA != refB) return refA < refB;
+    return g.mutAlleles[a].substr(ia) < g.mutAlleles[b].substr(ib);
+}
+
+} // namespace
+
+// Mutations in (position, REF, ALT) order (the reference writes a graph whose mutations are
+// not marked ordered in that order, grg.h:528-545 with grg.cpp:189-249; files it reads are
+// marked ordered, serialize.cpp:757).
+HostGrg sortMutationsForWrite(const HostGrg& g) {
+    const uint32_t M = g.numMutations();
+    std::vector<uint32_t> order(M);
+    for (uint32_t m = 0; m < M; m++) order[m] = m;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return mutationLess(g, a, b); });
+    HostGrg out = g;
+    auto permute = [&](auto& v) {
+        if (v.empty()) return;
+        auto copy = v;
+        for (uint32_t m = 0; m < M; m++) v[m] = copy[order[m]];
+    };
+    permute(out.mutNode);
+    permute(out.mutMiss);
+    permute(out.mutPos);
+    permute(out.mutAlleles);
+    permute(out.mutAltIndex);
+    permute(out.mutTime);
+    out.mutationsOrdered = true;
+    return out;
+}
+
+void writeGrgFile(const HostGrg& input, const std::string& path, bool allowSimplify) {
+    HostGrg sorted;
+    const HostGrg* src = &input;
+    if (!input.mutationsOrdered) {
+        sorted = sortMutationsForWrite(input);
+        src = &sorted;
+    }
+    HostGrg simplified;
+    if (allowSimplify) {
+        simplified = simplifyGrg(*src);
+        src = &simplified;
+    }
+    const HostGrg& g = *src;
     validateHostGrg(g);
     const uint32_t V = g.numNodes(), M = g.numMutations();
     std::ofstream out(path, std::ios::binary | std::ios::trunc);

@@ -31,6 +31,10 @@ struct HostGrg {
     std::vector<std::string> individualIds;
     uint64_t rangeStart = 0, rangeEnd = 0;
     bool phased = true;
+    // Mutation IDs are in (position, REF, ALT) order as far as the graph's owner knows: files
+    // read and immutable graphs built from arrays are; graphs grown by map_mutations are not,
+    // and are written with their mutations sorted.
+    bool mutationsOrdered = true;
     bool hasCoals() const { return !coal.empty(); }
     uint32_t numNodes() const { return downOff.empty() ? 0 : (uint32_t)(downOff.size() - 1); }
     uint64_t numEdges() const { return downOff.empty() ? 0 : downOff.back(); }
@@ -67,8 +71,11 @@ GraphStats computeGraphStats(const HostGrg& g);
 
 // .grg files (format v5.x of the reference, src/serialize.cpp:518-669 and :697-890).
 // Reading accepts versions 5.0-5.3 (u32 node index up to 5.1, u64 after); writing produces
-// version 5.3 without simplification (the graph is stored as it is).
+// version 5.3, with the reference's simplify-on-write when allowSimplify, and mutations
+// sorted by (position, REF, ALT) when the graph's are not marked ordered.
 HostGrg readGrgFile(const std::string& path);
-void writeGrgFile(const HostGrg& g, const std::string& path);
+void writeGrgFile(const HostGrg& g, const std::string& path, bool allowSimplify);
+// The graph as the reference's writer simplifies it (src/serialize.cpp:387-467).
+HostGrg simplifyGrg(const HostGrg& g);
 
 } // namespace grgl_amd

@@ -710,12 +710,14 @@ PYBIND11_MODULE(_grgl_amd, m) {
     m.def(
         "save_grg",
         [](PyGRGPtr& g, const std::string& filename, bool allowSimplify) {
-            (void)allowSimplify; // written as is: nodes and edges are not simplified
             py::gil_scoped_release release;
-            writeGrgFile(g->H(), filename);
+            writeGrgFile(g->H(), filename, allowSimplify);
         },
         py::arg("grg"), py::arg("filename"), py::arg("allow_simplify") = true,
-        "Write the graph as a .grg file, format 5.3, without simplification.");
+        "Write the graph as a .grg file, format 5.3 (src/python/_grgl.cpp save_grg): with "
+        "allow_simplify, nodes that only make the graph larger are removed as the reference's writer "
+        "does (src/serialize.cpp:387-467); mutations of a graph built by map_mutations are written "
+        "sorted by position and alleles.");
 
     m.def("grg_from_arrays", &fromArrays, py::arg("num_samples"), py::arg("ploidy"), py::arg("down_offsets"),
           py::arg("down_children"), py::arg("mutation_nodes"), py::arg("mutation_miss_nodes") = py::none(),

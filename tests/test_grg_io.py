@@ -8,7 +8,7 @@ import pytest
 
 import grgl_amd
 import oracle
-from helpers import bits_equal, make_input, random_grg
+from helpers import INVALID, csr_from_lists, dense_x, depth3_bin_tree, bits_equal, make_input, random_grg
 
 needs_ref = pytest.mark.skipif(not oracle.ref_available(), reason="reference library oracle/_ref not built here")
 
@@ -42,7 +42,7 @@ def _rich_graph():
 def test_round_trip_everything(tmp_path):
     a, g, refs, alts = _rich_graph()
     fn = str(tmp_path / "rich.grg")
-    grgl_amd.save_grg(g, fn)
+    grgl_amd.save_grg(g, fn, allow_simplify=False)
     h = grgl_amd.load_immutable_grg(fn)
     _arrays_equal(g.arrays(), h.arrays())
     for m in (0, 1, 2, 3, 199):
@@ -83,12 +83,18 @@ def test_edge_section_bytes_match_reference_encoder(tmp_path):
         g = grgl_amd.grg_from_arrays(a["num_samples"], a["ploidy"], a["down_offsets"], a["down_children"],
                                      a["mutation_nodes"])
         fn = str(tmp_path / "e.grg")
-        grgl_amd.save_grg(g, fn)
+        grgl_amd.save_grg(g, fn, allow_simplify=False)
         _, hdr, edges, idx = _sections(fn)
         ref_edges, ref_idx = oracle.OracleGraph(a).ref_edge_section()
         assert np.array_equal(edges, ref_edges)
         assert np.array_equal(idx, ref_idx)
         assert hdr[0] == 0xE9366C64DDC8C5B0 and (hdr[1], hdr[2]) == (5, 3)
+        # Simplified on write: the section holds the simplified graph, encoded the same way.
+        grgl_amd.save_grg(g, fn)
+        _, hdr, edges, idx = _sections(fn)
+        ref_edges, ref_idx = oracle.OracleGraph(grgl_amd.load_immutable_grg(fn).arrays()).ref_edge_section()
+        assert np.array_equal(edges, ref_edges) and np.array_equal(idx, ref_idx)
+        assert hdr[7] <= len(a["down_offsets"]) - 1 and hdr[8] <= len(a["down_children"])
 
 
 def test_reads_version_5_1_small_index(tmp_path):
@@ -236,3 +242,127 @@ def test_mutation_records_match_reference_layout(tmp_path):
                 break
         assert tail[pos_t:pos_t + n].decode() == st
         pos_t += n
+
+
+# --- simplify-on-write (src/serialize.cpp:387-467) ------------------------------------------
+
+def _save_load(g, tmp_path, name, **kw):
+    fn = str(tmp_path / name)
+    grgl_amd.save_grg(g, fn, **kw)
+    return grgl_amd.load_immutable_grg(fn)
+
+
+def test_simplify_rule_hand_cases(tmp_path):
+    """The reference's keep/drop rule on small graphs, node by node.
+    depth3 tree (test/unit/common_grgs.h:13-26) with mutations on 4 and 6: node 4 (2 children,
+    1 parent) carries a mutation and stays; node 5 (2 children, 1 parent, no mutation) is
+    dropped and its children move to 6; node 6 (no parents) carries a mutation and stays."""
+    a = depth3_bin_tree([4, 6], coal=[0, 0, 0, 0, 1, 2, 0])
+    g = grgl_amd.grg_from_arrays(a["num_samples"], a["ploidy"], a["down_offsets"], a["down_children"],
+                                 a["mutation_nodes"], coal_counts=a["coal_counts"])
+    h = _save_load(g, tmp_path, "d3.grg").arrays()
+    assert h["down_offsets"].tolist() == [0, 0, 0, 0, 0, 2, 5]
+    assert h["down_children"].tolist() == [0, 1, 2, 3, 4]
+    assert h["mutation_nodes"].tolist() == [4, 5]
+    assert h["coal_counts"][4:].tolist() == [1, 0 + 2]  # 6 takes the dropped node 5's count
+    # Two children and two parents: dropped; a node with one child is dropped too; a childless
+    # node's mutation becomes unmapped.
+    kids = [[], [], [], [], [0, 1], [4, 2], [4, 3], [2], [], [7, 6]]
+    a = csr_from_lists(4, kids, [5, 6, 8, 9], ploidy=2)
+    g = grgl_amd.grg_from_arrays(a["num_samples"], a["ploidy"], a["down_offsets"], a["down_children"],
+                                 a["mutation_nodes"])
+    h = _save_load(g, tmp_path, "x.grg").arrays()
+    # 4: two children, two parents -> dropped.  5 -> (0,1,2), a root with a mutation: kept as 4.
+    # 6 -> (0,1,3), one parent, a mutation: kept as 5.  7: one child -> dropped.  8: no
+    # children -> dropped, its mutation unmapped.  9 -> 2 (through 7) and 6: kept as 6.
+    assert h["down_offsets"].tolist() == [0, 0, 0, 0, 0, 3, 6, 8]
+    assert h["down_children"].tolist() == [0, 1, 2, 0, 1, 3, 2, 5]
+    assert h["mutation_nodes"].tolist() == [4, 5, INVALID, 6]
+
+
+def test_simplify_keeps_genotypes_and_unique_paths(tmp_path):
+    """On random DAGs and mapped graphs the simplified file multiplies exactly like the input
+    (int64, both directions), with no more nodes or edges."""
+    graphs = [random_grg(s, num_samples=80, num_internal=400, max_children=6, num_mutations=500, unmapped_frac=0.1)
+              for s in (1, 2, 3)]
+    graphs.append(grgl_amd.grg_from_coalescent(300, 3000, 2, seed=7).arrays())
+    for a in graphs:
+        g = grgl_amd.grg_from_arrays(a["num_samples"], a["ploidy"], a["down_offsets"], a["down_children"],
+                                     a["mutation_nodes"])
+        h = _save_load(g, tmp_path, "r.grg")
+        assert h.num_nodes <= g.num_nodes and h.num_edges <= g.num_edges
+        oa, ob = oracle.OracleGraph(a), oracle.OracleGraph(h.arrays())
+        rng = np.random.default_rng(0)
+        x = rng.integers(-50, 50, (3, g.num_samples)).astype(np.int64)
+        w = rng.integers(-50, 50, (3, g.num_mutations)).astype(np.int64)
+        assert np.array_equal(oa.matmul(x, oracle.UP), ob.matmul(x, oracle.UP))
+        assert np.array_equal(oa.matmul(w, oracle.DOWN), ob.matmul(w, oracle.DOWN))
+    # the coalescent graphs are already in simplified form: the file is the graph
+    a = graphs[-1]
+    g = grgl_amd.grg_from_arrays(a["num_samples"], a["ploidy"], a["down_offsets"], a["down_children"],
+                                 a["mutation_nodes"])
+    _arrays_equal(g.arrays(), _save_load(g, tmp_path, "t.grg").arrays())
+
+
+def _brute_coals(off, ch, N):
+    """Per node: individuals (haplotypes 2i, 2i+1) whose two haplotypes arrive through
+    different children."""
+    V = len(off) - 1
+    below = [{v} if v < N else set() for v in range(V)]
+    for v in range(N, V):
+        for c in ch[off[v]:off[v + 1]]:
+            below[v] |= below[c]
+    out = [0] * V
+    for v in range(N, V):
+        owner = {}
+        for c in ch[off[v]:off[v + 1]]:
+            for smp in below[c]:
+                owner[smp] = c
+        out[v] = sum(1 for i in range(N // 2) if 2 * i in owner and 2 * i + 1 in owner
+                     and owner[2 * i] != owner[2 * i + 1])
+    return out
+
+
+def test_simplified_coalescence_counts_brute_force(tmp_path):
+    """Coalescence counts of dropped nodes move to the nodes that take their children; on the
+    simplified graph they equal a brute-force count (individuals whose two haplotypes arrive
+    through different children).  Random binary trees, mutations on 40% of the nodes."""
+    for seed in range(4):
+        rng = np.random.default_rng(seed)
+        N = 40
+        kids = [[] for _ in range(N)]
+        active = list(range(N))
+        while len(active) > 1:
+            i, j = sorted(rng.choice(len(active), size=2, replace=False).tolist(), reverse=True)
+            a_, b_ = active.pop(i), active.pop(j)
+            kids.append(sorted([a_, b_]))
+            active.append(len(kids) - 1)
+        V = len(kids)
+        muts = [v for v in range(N, V) if rng.random() < 0.4] + [V - 1]
+        a = csr_from_lists(N, kids, muts, ploidy=2)
+        coal = np.array(_brute_coals(a["down_offsets"], a["down_children"], N), np.uint32)
+        g = grgl_amd.grg_from_arrays(N, 2, a["down_offsets"], a["down_children"], a["mutation_nodes"],
+                                     coal_counts=coal)
+        h = _save_load(g, tmp_path, "c.grg").arrays()
+        assert len(h["down_offsets"]) - 1 < V
+        want = _brute_coals(h["down_offsets"], h["down_children"], N)
+        assert h["coal_counts"][N:].tolist() == want[N:], seed
+        assert np.array_equal(dense_x(h), dense_x(a))
+
+
+def test_mapped_graph_mutations_written_in_position_order(tmp_path):
+    """A graph grown by map_mutations has unordered mutation IDs; the file lists them by
+    position (then REF, ALT), as the reference's writer does (grg.h:528-545), with their nodes."""
+    N = 20
+    rng = np.random.default_rng(4)
+    pos = [500.0, 10.0, 300.0, 10.0, 42.0]
+    refs = ["A", "C", "G", "A", "T"]
+    sets = [sorted(rng.choice(N, size=5, replace=False).tolist()) for _ in pos]
+    g = grgl_amd.MutableGRG(N, 2)
+    grgl_amd.map_mutations(g, [grgl_amd.Mutation(p, "G", r) for p, r in zip(pos, refs)], sets)
+    h = _save_load(g, tmp_path, "m.grg")
+    order = sorted(range(len(pos)), key=lambda m: (pos[m], refs[m], "G"))
+    assert [h.get_mutation_by_id(i).position for i in range(len(pos))] == [int(pos[m]) for m in order]
+    X = dense_x(h.arrays())
+    for i, m in enumerate(order):
+        assert sorted(np.nonzero(X[:, i])[0].tolist()) == sets[m]

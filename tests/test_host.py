@@ -247,3 +247,27 @@ def test_graph_create_rejects_missing_arrays():
         assert L.grgl_hip_graph_create(ctypes.byref(d), 0, ctypes.byref(out)) == capi.ERR_MISUSE
         assert what in L.grgl_hip_last_error().decode()
         assert not out.value
+
+
+def test_import_does_not_load_torch():
+    """`import grgl_amd` loads neither torch nor anything that needs it; with GRGL_AMD_NO_TORCH=1
+    the numpy surface (generator, arrays, .grg files) works the same."""
+    import subprocess
+    import sys
+
+    code = ("import sys, os, tempfile\n"
+            "import grgl_amd\n"
+            "assert 'torch' not in sys.modules, 'torch imported'\n"
+            "g = grgl_amd.grg_from_coalescent(200, 1000, 1, seed=3)\n"
+            "a = g.arrays()\n"
+            "d = tempfile.mkdtemp()\n"
+            "grgl_amd.save_grg(g, os.path.join(d, 'x.grg'))\n"
+            "h = grgl_amd.load_immutable_grg(os.path.join(d, 'x.grg'))\n"
+            "assert (h.num_nodes, h.num_edges) == (g.num_nodes, g.num_edges)\n"
+            "assert 'torch' not in sys.modules\n"
+            "print('ok')\n")
+    for env_extra in ({}, {"GRGL_AMD_NO_TORCH": "1"}):
+        env = dict(os.environ, **env_extra)
+        proc = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, stdout=subprocess.PIPE,
+                              stderr=subprocess.STDOUT, text=True, timeout=120)
+        assert proc.returncode == 0 and proc.stdout.strip().endswith("ok"), proc.stdout
