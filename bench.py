@@ -729,9 +729,12 @@ def main():
 
     # The library stacks the ladder largest-first on 16-byte boundaries: 511 rows exactly.
     Kpad = Ktot
+    tuning_chosen = {}
     if not args.no_autotune:
         for cdt, d, _, _, _ in work:
             best, ms = cg.autotune(d, cdt, Kpad)
+            tuning_chosen[f"{'f32' if cdt == capi.F32 else 'i32'}_{'UP' if d == capi.UP else 'DOWN'}_K{Kpad}"] = dict(
+                best, levels_ms=round(ms, 4))
             log(f"autotune dtype={cdt} dir={d} rows={Kpad}: {best} levels {ms:.3f} ms")
 
     def step(timing_f32=False):
@@ -769,7 +772,8 @@ def main():
         name = "UP" if d == capi.UP else "DOWN"
         for K, x, y in reqs:
             if not args.no_autotune:
-                cg.autotune(d, cdt, K)
+                best, ms = cg.autotune(d, cdt, K)
+                tuning_chosen[f"{'f32' if cdt == capi.F32 else 'i32'}_{name}_K{K}"] = dict(best, levels_ms=round(ms, 4))
             reps = 5
             cg.matmul_ptr(cdt, x.data_ptr(), cin, K, d, y.data_ptr(), y.numel(), stream=sp)
             torch.cuda.synchronize()
@@ -885,6 +889,9 @@ def main():
         "plain_i32": plain_i32,
         "batched_vs_plain_f32": round(plain_ms_total / batched_f32_ms, 3) if plain_ms_total else None,
         "roofline": roofline,
+        # autotuned level-kernel tile per (dtype, direction, stacked rows): team width, block
+        # threads, loads in flight, hipGraph replay, block order, chained launch
+        "tuning": tuning_chosen,
         "fp32_error": fp32_err,
         "plain_100k_x_500k": at_scale,
         "serving": serving,

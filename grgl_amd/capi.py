@@ -122,7 +122,8 @@ class Request(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("team_lanes", ctypes.c_uint32), ("heavy_fanin", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
-                ("xcd_map", ctypes.c_uint32), ("max_launch_entries", ctypes.c_uint32), ("chain", ctypes.c_uint32)]
+                ("xcd_map", ctypes.c_uint32), ("max_launch_entries", ctypes.c_uint32), ("chain", ctypes.c_uint32),
+                ("block_threads", ctypes.c_uint32), ("loads_in_flight", ctypes.c_uint32)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -350,8 +351,9 @@ class Graph:
         check(lib().grgl_hip_matmul_batch(self.handle, dtype, direction, arr, len(reqs), flags, stream))
 
     def set_tuning(self, direction, dtype, rows, team_lanes=0, heavy_fanin=0, use_graph=0, xcd_map=0,
-                   max_launch_entries=0, chain=0):
-        t = Tuning(team_lanes, heavy_fanin, use_graph, xcd_map, max_launch_entries, chain)
+                   max_launch_entries=0, chain=0, block_threads=0, loads_in_flight=0):
+        t = Tuning(team_lanes, heavy_fanin, use_graph, xcd_map, max_launch_entries, chain, block_threads,
+                   loads_in_flight)
         check(lib().grgl_hip_set_tuning(self.handle, direction, dtype, rows, ctypes.byref(t)))
 
     def autotune(self, direction, dtype, rows):
@@ -359,13 +361,15 @@ class Graph:
         ms = ctypes.c_float()
         check(lib().grgl_hip_autotune(self.handle, direction, dtype, rows, ctypes.byref(t), ctypes.byref(ms)))
         return {"team_lanes": t.team_lanes, "heavy_fanin": t.heavy_fanin, "use_graph": t.use_graph,
-                "xcd_map": t.xcd_map, "max_launch_entries": t.max_launch_entries, "chain": t.chain}, ms.value
+                "xcd_map": t.xcd_map, "max_launch_entries": t.max_launch_entries, "chain": t.chain,
+                "block_threads": t.block_threads, "loads_in_flight": t.loads_in_flight}, ms.value
 
     def get_tuning(self, direction, dtype, rows):
         t = Tuning()
         check(lib().grgl_hip_get_tuning(self.handle, direction, dtype, rows, ctypes.byref(t)))
         return {"team_lanes": t.team_lanes, "heavy_fanin": t.heavy_fanin, "use_graph": t.use_graph,
-                "xcd_map": t.xcd_map, "max_launch_entries": t.max_launch_entries, "chain": t.chain}
+                "xcd_map": t.xcd_map, "max_launch_entries": t.max_launch_entries, "chain": t.chain,
+                "block_threads": t.block_threads, "loads_in_flight": t.loads_in_flight}
 
     def chain_control(self, abort=-1, wait_us=0, reset=False):
         """grgl_hip_chain_control: set (1) / clear (0) / keep (-1) the abort word, bound chained

@@ -137,6 +137,8 @@ struct Tuning {
     uint32_t useGraph = 0;
     uint32_t xcdMap = 0;    // 0 = default (off), 1 = XCD-aware block order, 2 = off
     uint32_t chain = 0;     // 0 = default, 1 = level-by-level launches, 2 = chained when eligible
+    uint32_t blockThreads = 0;  // wide-team level kernels: 0 = default (256), 256 or 512 threads
+    uint32_t loadsInFlight = 0; // wide-team level kernels: 0 = default (4), 4 or 8 row loads per batch
 };
 
 // Host-mapped control words of a graph's chained traversals (page-locked, device-visible).

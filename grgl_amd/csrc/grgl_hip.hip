@@ -196,7 +196,22 @@ namespace {
 // variant with per-wavefront heavy rows of 32 bytes, for (vb, tw) of a 32-byte row.  The
 // level kernels are resolved to their addresses, so that one launch list serves direct
 // launches (hipLaunchKernel) and graph nodes (hipGraphAddKernelNode) alike.
-template <typename T, bool XOR> const void* pullKernelT(uint32_t vb, uint32_t tw, bool h32) {
+// Wide-team shapes (16-byte accesses, teams of 8-32 lanes) of the 4- and 8-byte types also come
+// with 512-thread blocks and 8 row loads in flight per batch (tuning dimensions).
+template <typename T, bool XOR> constexpr bool kTileTunable = !XOR && sizeof(T) >= 4;
+inline bool tileTunable(uint32_t vb, uint32_t tw) { return vb == 16 && tw >= 8 && tw <= 32; }
+
+template <typename T, bool XOR, int TW> const void* wideKernel(uint32_t blk, uint32_t unr) {
+    if constexpr (kTileTunable<T, XOR>) {
+        if (blk == 512 && unr == 8) return reinterpret_cast<const void*>(&pull_level_kernel<T, 16, TW, XOR, false, 512, 8>);
+        if (blk == 512) return reinterpret_cast<const void*>(&pull_level_kernel<T, 16, TW, XOR, false, 512, 4>);
+        if (unr == 8) return reinterpret_cast<const void*>(&pull_level_kernel<T, 16, TW, XOR, false, kBlock, 8>);
+    }
+    return reinterpret_cast<const void*>(&pull_level_kernel<T, 16, TW, XOR>);
+}
+
+template <typename T, bool XOR>
+const void* pullKernelT(uint32_t vb, uint32_t tw, bool h32, uint32_t blk, uint32_t unr) {
 #define GRGL_PULL(VB, TW) return reinterpret_cast<const void*>(&pull_level_kernel<T, VB, TW, XOR>)
 #define GRGL_PULL32(VB, TW) return reinterpret_cast<const void*>(&pull_level_kernel<T, VB, TW, XOR, true>)
     if (h32) {
@@ -229,9 +244,9 @@ template <typename T, bool XOR> const void* pullKernelT(uint32_t vb, uint32_t tw
         case 1: GRGL_PULL(16, 1);
         case 2: GRGL_PULL(16, 2);
         case 4: GRGL_PULL(16, 4);
-        case 8: GRGL_PULL(16, 8);
-        case 16: GRGL_PULL(16, 16);
-        case 32: GRGL_PULL(16, 32);
+        case 8: return wideKernel<T, XOR, 8>(blk, unr);
+        case 16: return wideKernel<T, XOR, 16>(blk, unr);
+        case 32: return wideKernel<T, XOR, 32>(blk, unr);
         default: GRGL_PULL(16, 64);
         }
     }
@@ -283,15 +298,26 @@ const void* tailKernel(TypeClass tc, uint32_t vb, uint32_t tw) {
     throw std::logic_error("bad type class");
 }
 
-const void* pullKernel(TypeClass tc, uint32_t vb, uint32_t tw, bool h32) {
+// Block size and loads in flight the level kernel of (type, vb, tw) runs with, for the tuned
+// request (0 = default: 256 threads, 4 loads): only wide-team shapes of 4- and 8-byte types
+// take other values.
+void tileShape(TypeClass tc, uint32_t vb, uint32_t tw, const Tuning& t, uint32_t& blk, uint32_t& unr) {
+    blk = kBlock;
+    unr = 4;
+    if (!tileTunable(vb, tw) || tc == kW16 || tc == kW8 || tc == kX8) return;
+    if (t.blockThreads == 512) blk = 512;
+    if (t.loadsInFlight == 8) unr = 8;
+}
+
+const void* pullKernel(TypeClass tc, uint32_t vb, uint32_t tw, bool h32, uint32_t blk, uint32_t unr) {
     switch (tc) {
-    case kF64: return pullKernelT<double, false>(vb, tw, h32);
-    case kF32: return pullKernelT<float, false>(vb, tw, h32);
-    case kW64: return pullKernelT<uint64_t, false>(vb, tw, h32);
-    case kW32: return pullKernelT<uint32_t, false>(vb, tw, h32);
-    case kW16: return pullKernelT<uint16_t, false>(vb, tw, h32);
-    case kW8: return pullKernelT<uint8_t, false>(vb, tw, h32);
-    case kX8: return pullKernelT<uint8_t, true>(vb, tw, h32);
+    case kF64: return pullKernelT<double, false>(vb, tw, h32, blk, unr);
+    case kF32: return pullKernelT<float, false>(vb, tw, h32, blk, unr);
+    case kW64: return pullKernelT<uint64_t, false>(vb, tw, h32, blk, unr);
+    case kW32: return pullKernelT<uint32_t, false>(vb, tw, h32, blk, unr);
+    case kW16: return pullKernelT<uint16_t, false>(vb, tw, h32, blk, unr);
+    case kW8: return pullKernelT<uint8_t, false>(vb, tw, h32, blk, unr);
+    case kX8: return pullKernelT<uint8_t, true>(vb, tw, h32, blk, unr);
     }
     throw std::logic_error("bad type class");
 }
@@ -575,6 +601,10 @@ Tuning chooseTuning(grgl_graph* g, grgl_direction dir, TypeClass tc, uint32_t K,
     if (tw == 0) tw = geo.lanes >= 3 ? std::min<uint32_t>(16, pow2ceil(geo.lanes)) : (geo.rowBytes <= 8 ? 4 : 8);
     t.teamLanes = std::min<uint32_t>(maxTeamLanes(geo), pow2ceil(tw));
     if (t.chain == 0) t.chain = 1; // level-by-level launches unless chosen
+    // Rows of 128 B - 1 KB (FP32 K = 32..256): 8 row loads in flight per batch; the autotuner
+    // picks it for these on the 200k x 1M graph (DOWN K = 32: 0.47 -> 0.41 ms, K = 64: 0.75 ->
+    // 0.70 ms).  Wider rows keep 4 (the 511-RHS traversal measures no gain).
+    if (t.loadsInFlight == 0 && geo.rowBytes >= 128 && geo.rowBytes <= 1024) t.loadsInFlight = 8;
     return t;
 }
 
@@ -602,6 +632,7 @@ uint32_t tailStart(const DeviceDirPlan& plan, const LaunchTable& tab, const RowG
 struct LevelLaunch {
     const void* func = nullptr;
     dim3 grid;
+    uint32_t block = kBlock;
     bool tail = false;
     PullArgs pull{};
     TailArgs tailArgs{};
@@ -612,12 +643,16 @@ struct LevelLaunch {
 // The launches of every pulled level of one direction, in order (the trailing small levels as
 // one tail launch).  With `timing`, every level gets a launch of its own and its bytes.
 std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTable& tab, TypeClass tc,
-                                       const RowGeometry& geo, uint32_t tw, bool xcdMap, bool readSelf,
+                                       const RowGeometry& geo, const Tuning& tuning, bool readSelf,
                                        bool orderPreserving, uint32_t K, size_t elemSize, char* rows, bool timing) {
     std::vector<LevelLaunch> out;
+    const uint32_t tw = tuning.teamLanes;
+    const bool xcdMap = tuning.xcdMap == 1;
     const uint32_t tiles = levelTiles(geo, tw, elemSize);
     const uint32_t vb = levelVb(geo, tw, elemSize);
-    const uint32_t teamsPerBlock = kBlock / tw;
+    uint32_t blk = kBlock, unr = 4;
+    tileShape(tc, vb, tw, tuning, blk, unr);
+    const uint32_t teamsPerBlock = blk / tw;
     // The single-block tail gains nothing from spreading rows over idle lanes: rows of one
     // access take single-lane teams there (256 rows per pass instead of 256 / tw).
     const uint32_t tailTw = geo.lanes == 1 ? 1 : tw;
@@ -647,14 +682,15 @@ std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTa
         const bool manyHeavy = heavy >= 512;
         const bool h32 = geo.rowBytes == 32 && tw <= 8 && manyHeavy && tab.heavyEntries[l] * 4 >= tab.levelEntries[l];
         a.heavyPerWave = ((geo.rowBytes <= 16 && manyHeavy) || h32) ? 1 : 0;
-        a.heavyBlocks = a.heavyPerWave ? (heavy + kBlock / 64 - 1) / (kBlock / 64) : heavy;
+        a.heavyBlocks = a.heavyPerWave ? (heavy + blk / 64 - 1) / (blk / 64) : heavy;
         a.levelEnd = end;
         a.rowBytes = geo.rowBytes;
         a.readSelf = readSelf ? 1 : 0;
         const uint32_t bxCount = a.heavyBlocks + (light + teamsPerBlock - 1) / teamsPerBlock;
         a.xcdTiles = (xcdMap && tiles % 8 == 0) ? tiles : 0;
         L.grid = a.xcdTiles ? dim3(bxCount * tiles, 1) : dim3(bxCount, tiles);
-        L.func = pullKernel(tc, vb, tw, h32);
+        L.func = pullKernel(tc, vb, tw, h32, blk, unr);
+        L.block = blk;
         if (timing) {
             const uint64_t nodes = end - begin;
             uint64_t ent = 0;
@@ -700,7 +736,7 @@ void launchDirect(grgl_graph* g, std::vector<LevelLaunch>& launches, hipStream_t
             GRGL_HIP_CHECK(hipEventRecord(ev0, stream));
         }
         void* args[1] = {L.argPtr()};
-        GRGL_HIP_CHECK(hipLaunchKernel(L.func, L.grid, dim3(kBlock), args, 0, stream));
+        GRGL_HIP_CHECK(hipLaunchKernel(L.func, L.grid, dim3(L.block), args, 0, stream));
         if (timing) {
             GRGL_HIP_CHECK(hipEventRecord(ev1, stream));
             std::lock_guard<std::mutex> lk(g->mu);
@@ -720,7 +756,7 @@ void buildLevelGraph(std::vector<LevelLaunch>& launches, CapturedLevels& out) {
         void* args[1] = {L.argPtr()};
         p.func = const_cast<void*>(L.func);
         p.gridDim = L.grid;
-        p.blockDim = dim3(kBlock);
+        p.blockDim = dim3(L.block);
         p.sharedMemBytes = 0;
         p.kernelParams = args;
         p.extra = nullptr;
@@ -810,14 +846,15 @@ void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometr
         direct = cs != hipStreamCaptureStatusNone;
     }
     if (direct) {
-        std::vector<LevelLaunch> launches = levelLaunches(plan, *tab, tc, geo, t.teamLanes, t.xcdMap == 1, readSelf,
-                                                          orderPreserving, K, es, rows, timing);
+        std::vector<LevelLaunch> launches =
+            levelLaunches(plan, *tab, tc, geo, t, readSelf, orderPreserving, K, es, rows, timing);
         launchDirect(g, launches, stream, timing);
         return;
     }
     const auto key = std::make_tuple((int)dir, (int)tc, geo.vb, t.teamLanes, geo.ld, geo.rowBytes,
                                      (int)readSelf * 2 + (int)orderPreserving + 4 * (int)(t.xcdMap == 1) +
-                                         8 * (int)tab->heavyFanin,
+                                         8 * (int)(t.blockThreads == 512) + 16 * (int)(t.loadsInFlight == 8) +
+                                         32 * (int)tab->heavyFanin,
                                      (uintptr_t)rows, tab->chunkEntries);
     std::shared_ptr<CapturedLevels> cap;
     {
@@ -828,8 +865,8 @@ void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometr
     if (!cap) {
         auto fresh = std::make_shared<CapturedLevels>();
         GRGL_HIP_CHECK(hipEventCreateWithFlags(&fresh->lastLaunch, hipEventDisableTiming));
-        std::vector<LevelLaunch> launches = levelLaunches(plan, *tab, tc, geo, t.teamLanes, t.xcdMap == 1, readSelf,
-                                                          orderPreserving, K, es, rows, false);
+        std::vector<LevelLaunch> launches =
+            levelLaunches(plan, *tab, tc, geo, t, readSelf, orderPreserving, K, es, rows, false);
         buildLevelGraph(launches, *fresh);
         std::lock_guard<std::mutex> lk(g->mu);
         if (g->graphs.size() >= kMaxCapturedGraphs) {
@@ -1590,6 +1627,12 @@ GRGL_EXPORT grgl_status grgl_hip_set_tuning(grgl_graph* g, grgl_direction direct
     t.xcdMap = tuning->xcd_map;
     if (tuning->chain > 2) throw MisuseError("chain must be 0, 1 or 2");
     t.chain = tuning->chain;
+    if (tuning->block_threads != 0 && tuning->block_threads != 256 && tuning->block_threads != 512)
+        throw MisuseError("block_threads must be 0, 256 or 512");
+    if (tuning->loads_in_flight != 0 && tuning->loads_in_flight != 4 && tuning->loads_in_flight != 8)
+        throw MisuseError("loads_in_flight must be 0, 4 or 8");
+    t.blockThreads = tuning->block_threads;
+    t.loadsInFlight = tuning->loads_in_flight;
     g->tuning[std::make_tuple((int)direction, (int)tc, rows)] = t;
     if (tuning->heavy_fanin || tuning->max_launch_entries) {
         // A new launch table replaces the direction's current one; traversals already
@@ -1630,6 +1673,8 @@ GRGL_EXPORT grgl_status grgl_hip_get_tuning(grgl_graph* g, grgl_direction direct
     out->use_graph = t.useGraph;
     out->xcd_map = t.xcdMap;
     out->chain = t.chain;
+    const RowGeometry geo = rowGeometry(rows, dtypeSize(dtype));
+    tileShape(tc, levelVb(geo, t.teamLanes, dtypeSize(dtype)), t.teamLanes, t, out->block_threads, out->loads_in_flight);
     out->max_launch_entries = (uint32_t)plan.table->chunkEntries;
     GRGL_API_END
 }
@@ -1666,7 +1711,7 @@ GRGL_EXPORT grgl_status grgl_hip_tuning_save(grgl_graph* g, const char* path) {
     GRGL_API_BEGIN
     if (!g || !path) throw MisuseError("null argument");
     std::ostringstream os;
-    os << kTuningMagic << " 2\n";
+    os << kTuningMagic << " 3\n";
     char fp[32];
     std::snprintf(fp, sizeof(fp), "%016llx", (unsigned long long)tuningFingerprint(g));
     os << "graph " << fp << "\n";
@@ -1679,7 +1724,8 @@ GRGL_EXPORT grgl_status grgl_hip_tuning_save(grgl_graph* g, const char* path) {
         for (const auto& kv : g->tuning) {
             const Tuning& t = kv.second;
             os << "entry " << std::get<0>(kv.first) << " " << std::get<1>(kv.first) << " " << std::get<2>(kv.first) << " "
-               << t.teamLanes << " " << t.useGraph << " " << t.xcdMap << " " << t.chain << "\n";
+               << t.teamLanes << " " << t.useGraph << " " << t.xcdMap << " " << t.chain << " " << t.blockThreads
+               << " " << t.loadsInFlight << "\n";
         }
     }
     std::ofstream f(path, std::ios::trunc);
@@ -1696,8 +1742,9 @@ GRGL_EXPORT grgl_status grgl_hip_tuning_load(grgl_graph* g, const char* path) {
     if (!f) throw MisuseError(std::string("cannot read tuning file ") + path);
     std::string magic, word, fp;
     int version = 0;
-    // Version 1 files carry no chain column (their entries keep the default).
-    if (!(f >> magic >> version) || magic != kTuningMagic || (version != 1 && version != 2))
+    // Version 1 files carry no chain column, versions 1 and 2 no block size and loads in
+    // flight (their entries keep the defaults).
+    if (!(f >> magic >> version) || magic != kTuningMagic || version < 1 || version > 3)
         throw MisuseError("not a tuning file");
     if (!(f >> word >> fp) || word != "graph") throw MisuseError("malformed tuning file");
     char want[32];
@@ -1716,16 +1763,19 @@ GRGL_EXPORT grgl_status grgl_hip_tuning_load(grgl_graph* g, const char* path) {
             dirs.push_back(d);
         } else if (word == "entry") {
             int dir = 0, tc = 0;
-            uint64_t rows = 0, tw = 0, ug = 0, xcd = 0, chain = 0;
+            uint64_t rows = 0, tw = 0, ug = 0, xcd = 0, chain = 0, blk = 0, unr = 0;
             if (!(f >> dir >> tc >> rows >> tw >> ug >> xcd) || (version >= 2 && !(f >> chain)) ||
-                (dir != GRGL_DIRECTION_UP && dir != GRGL_DIRECTION_DOWN) || tc < 0 || tc > (int)kX8 || rows == 0 ||
-                rows > 65536 || tw > 64 || ug > 1 || xcd > 2 || chain > 2)
+                (version >= 3 && !(f >> blk >> unr)) || (dir != GRGL_DIRECTION_UP && dir != GRGL_DIRECTION_DOWN) ||
+                tc < 0 || tc > (int)kX8 || rows == 0 || rows > 65536 || tw > 64 || ug > 1 || xcd > 2 || chain > 2 ||
+                (blk != 0 && blk != 256 && blk != 512) || (unr != 0 && unr != 4 && unr != 8))
                 throw MisuseError("malformed tuning file (entry)");
             Tuning t;
             t.teamLanes = (uint32_t)tw;
             t.useGraph = (uint32_t)ug;
             t.xcdMap = (uint32_t)xcd;
             t.chain = (uint32_t)chain;
+            t.blockThreads = (uint32_t)blk;
+            t.loadsInFlight = (uint32_t)unr;
             entries.push_back({std::make_tuple(dir, tc, (uint32_t)rows), t});
         } else {
             throw MisuseError("malformed tuning file");
@@ -1835,6 +1885,18 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
                 t.chain = 2;
                 cands.push_back(t);
             }
+            // Wide-team shapes: 512-thread blocks and 8 loads in flight, direct launches.
+            if (tileTunable(levelVb(geo, tw, es), tw) && tc != kW16 && tc != kW8 && tc != kX8) {
+                for (uint32_t shape = 1; shape < 4; shape++) {
+                    Tuning t;
+                    t.teamLanes = tw;
+                    t.xcdMap = 1;
+                    t.chain = 1;
+                    t.blockThreads = (shape & 1) ? 512 : 256;
+                    t.loadsInFlight = (shape & 2) ? 8 : 4;
+                    cands.push_back(t);
+                }
+            }
         }
         for (const Tuning& t : cands) {
             runLevels(g, direction, tc, geo, t, false, false, rows, es, ws->ptr, s, false, ws->chainState); // warm-up
@@ -1877,6 +1939,8 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
         chosen->use_graph = best.useGraph;
         chosen->xcd_map = best.xcdMap;
         chosen->chain = best.chain;
+        tileShape(tc, levelVb(geo, best.teamLanes, es), best.teamLanes, best, chosen->block_threads,
+                  chosen->loads_in_flight);
         chosen->max_launch_entries = (uint32_t)tab->chunkEntries;
     }
     if (best_ms) *best_ms = bestMs;

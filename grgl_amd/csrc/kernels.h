@@ -193,11 +193,11 @@ __device__ __forceinline__ void pull_light(const PullArgs& a, uint32_t row, uint
 }
 
 // Heavy rows: the whole block on one row, 64 interleaved partials, fixed reduction tree.
-template <typename T, int VB, int TW, bool XOR>
+template <typename T, int VB, int TW, bool XOR, int BLK = kBlock>
 __device__ __forceinline__ void pull_heavy(const PullArgs& a, uint32_t row, uint64_t colByte, bool active,
                                            uint32_t team, uint32_t lane) {
     using P = Pack<T, VB>;
-    constexpr int NT = kBlock / TW;                          // teams in the block
+    constexpr int NT = BLK / TW;                             // teams in the block
     constexpr int PPT = NT >= kHeavyPartials ? 1 : kHeavyPartials / NT; // partials per team
     constexpr int ACTIVE_TEAMS = NT >= kHeavyPartials ? kHeavyPartials : NT;
     __shared__ char lds[ACTIVE_TEAMS * TW * VB];
@@ -329,13 +329,15 @@ struct NoWait {
 // One block's share of a level: heavy rows (a block each, or 4 per block one per wavefront)
 // or NT light rows.  The gather-list structure (offsets, indices) is read before the wait,
 // the rows after it.  Returns false when the wait gave up (nothing written).
-template <typename T, int VB, int TW, bool XOR, bool H32, typename Wait>
+// BLK: threads per block; UNRW: row loads in flight per batch on the wide-team path (TW >= 8
+// with 16-byte accesses).  Neither changes the arithmetic of any row.
+template <typename T, int VB, int TW, bool XOR, bool H32, int BLK, int UNRW, typename Wait>
 __device__ __forceinline__ bool pull_level_block(const PullArgs& a, uint32_t bx, uint32_t by, const Wait& wait) {
     const uint32_t lane = threadIdx.x % TW;
     const uint32_t team = threadIdx.x / TW;
     const uint64_t colByte = ((uint64_t)by * TW + lane) * VB;
     const bool active = colByte < a.rowBytes;
-    constexpr uint32_t NT = kBlock / TW;
+    constexpr uint32_t NT = BLK / TW;
     if (bx < a.heavyBlocks) {
         if constexpr (Wait::kWaits) {
             wait.poll();
@@ -346,7 +348,7 @@ __device__ __forceinline__ bool pull_level_block(const PullArgs& a, uint32_t bx,
             // Short rows: one wavefront per heavy row, each lane one partial holding the
             // whole row (the single-lane-team arithmetic: same partials, same tree), so a
             // block takes 4 heavy rows and no lane idles.
-            const uint32_t h = bx * (kBlock / 64) + threadIdx.x / 64;
+            const uint32_t h = bx * (BLK / 64) + threadIdx.x / 64;
             if (h < a.heavyCount) {
                 const uint32_t row = a.levelBegin + h, part = threadIdx.x % 64;
                 if (a.rowBytes == 4) {
@@ -360,7 +362,7 @@ __device__ __forceinline__ bool pull_level_block(const PullArgs& a, uint32_t bx,
                 }
             }
         } else {
-            pull_heavy<T, VB, TW, XOR>(a, a.levelBegin + bx, colByte, active, team, lane);
+            pull_heavy<T, VB, TW, XOR, BLK>(a, a.levelBegin + bx, colByte, active, team, lane);
         }
     } else if constexpr (TW <= 4 || VB < 16) {
         constexpr int UNR = TW == 1 ? (VB <= 8 ? 32 : 16) : (VB == 16 ? 8 : 16);
@@ -379,16 +381,16 @@ __device__ __forceinline__ bool pull_level_block(const PullArgs& a, uint32_t bx,
         if (n <= kStageEntries) { // block-uniform
             // Every index load of a thread is issued before its first LDS store, so the
             // copy is one round trip whatever n is (a load-store loop costs one per pass).
-            constexpr int SU = (int)(kStageEntries / kBlock);
+            constexpr int SU = (int)(kStageEntries / BLK);
             uint32_t v[SU];
 #pragma unroll
             for (int u = 0; u < SU; u++) {
-                const uint32_t t = threadIdx.x + (uint32_t)u * kBlock;
+                const uint32_t t = threadIdx.x + (uint32_t)u * BLK;
                 if (t < n) v[u] = a.src[e0 + t];
             }
 #pragma unroll
             for (int u = 0; u < SU; u++) {
-                const uint32_t t = threadIdx.x + (uint32_t)u * kBlock;
+                const uint32_t t = threadIdx.x + (uint32_t)u * BLK;
                 if (t < n) staged[t] = v[u];
             }
             wait.poll();
@@ -410,7 +412,7 @@ __device__ __forceinline__ bool pull_level_block(const PullArgs& a, uint32_t bx,
             if (!wait.acquire()) return false;
         }
         const uint32_t row = a.levelBegin + a.heavyCount + (bx - a.heavyBlocks) * NT + team;
-        if (row < a.levelEnd) pull_light<T, VB, TW, XOR, 4>(a, row, colByte, active, GlobalIdx{a.src});
+        if (row < a.levelEnd) pull_light<T, VB, TW, XOR, UNRW>(a, row, colByte, active, GlobalIdx{a.src});
     }
     return true;
 }
@@ -418,11 +420,11 @@ __device__ __forceinline__ bool pull_level_block(const PullArgs& a, uint32_t bx,
 // H32: the variant launched for levels with heavy rows when rows are 32 bytes; its
 // per-wavefront heavy path holds 32 bytes per lane, whose registers would lower the
 // occupancy of the levels without heavy rows, so they use the plain variant.
-template <typename T, int VB, int TW, bool XOR, bool H32 = false>
-__global__ __launch_bounds__(kBlock) void pull_level_kernel(PullArgs a) {
+template <typename T, int VB, int TW, bool XOR, bool H32 = false, int BLK = kBlock, int UNRW = 4>
+__global__ __launch_bounds__(BLK) void pull_level_kernel(PullArgs a) {
     uint32_t bx, by;
     pull_block_coords(a, bx, by);
-    (void)pull_level_block<T, VB, TW, XOR, H32>(a, bx, by, NoWait{});
+    (void)pull_level_block<T, VB, TW, XOR, H32, BLK, UNRW>(a, bx, by, NoWait{});
 }
 
 // The trailing levels of a traversal that are too small to fill the chip (at most two
@@ -599,7 +601,8 @@ __global__ __launch_bounds__(kBlock) void pull_chain_kernel(ChainArgs c) {
         a.rowBytes = c.rowBytes;
         a.readSelf = c.readSelf;
         a.xcdTiles = 0;
-        const bool done = pull_level_block<T, VB, TW, XOR, H32>(a, task - c.taskStart[l], 0, ChainWait{&c, l, &sOk});
+        const bool done =
+            pull_level_block<T, VB, TW, XOR, H32, kBlock, 4>(a, task - c.taskStart[l], 0, ChainWait{&c, l, &sOk});
         __syncthreads(); // the task's rows are stored; sTask and the LDS are free again
         if (threadIdx.x == 0) {
             if (done) __hip_atomic_fetch_add(c.state + kChainHeader + l, 1U, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);

@@ -200,6 +200,11 @@ typedef struct grgl_tuning {
                                 work in order from a ticket counter and wait on per-level completion
                                 counters, with no grid barrier; bit-identical results.  Its waits are
                                 bounded (grgl_hip_chain_control). */
+    uint32_t block_threads;  /* threads per block of the level kernels with wide teams (16-byte accesses,
+                                8-32 lanes per row) of 4- and 8-byte types: 0 = default (256), 256 or
+                                512; other shapes always use 256.  Never changes results. */
+    uint32_t loads_in_flight; /* row loads issued before the first add in each batch of such kernels:
+                                0 = default (4), 4 or 8.  Never changes results. */
 } grgl_tuning;
 
 grgl_status grgl_hip_set_tuning(grgl_graph* graph, grgl_direction direction, grgl_dtype dtype, uint32_t rows,

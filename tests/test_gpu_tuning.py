@@ -238,3 +238,41 @@ def test_graph_cache_threads_past_its_bound(gpu):
         assert tuner.result() > 0
     assert bad == []
     cg.close()
+
+
+@pytest.mark.parametrize("name", ["heavy", "tree"])
+@pytest.mark.parametrize("dtype", [np.float32, np.int32, np.float64, np.int64])
+def test_tile_shapes_give_the_same_bits(gpu, name, dtype):
+    """Wide-team level kernels with 256 or 512 threads per block and 4 or 8 row loads in flight
+    (direct and graph launches) give the bits of the default shape; get_tuning reports the
+    shape that runs."""
+    import torch
+
+    arrays = _graphs()[name]
+    cg = capi.Graph(arrays)
+    dt = capi.NUMPY_TO_DTYPE[np.dtype(dtype)]
+    rng = np.random.default_rng(17)
+    stream = torch.cuda.current_stream().cuda_stream
+    for direction in (capi.UP, capi.DOWN):
+        cin = cg.num_samples if direction == capi.UP else cg.num_mutations
+        cout = cg.num_mutations if direction == capi.UP else cg.num_samples
+        for K in (40, 200, 511):
+            x = torch.from_numpy(make_input(rng, dtype, (K, cin))).cuda()
+            cg.set_tuning(direction, dt, K)
+            ref = _run(cg, dt, x, K, direction, cin, cout, stream).cpu().numpy()
+            for tw in (8, 16, 32):
+                for blk in (256, 512):
+                    for unr in (4, 8):
+                        for use_graph in ((0, 1) if (blk, unr) == (512, 8) else (0,)):
+                            cg.set_tuning(direction, dt, K, team_lanes=tw, block_threads=blk, loads_in_flight=unr,
+                                          use_graph=use_graph)
+                            t = cg.get_tuning(direction, dt, K)
+                            if t["team_lanes"] == tw and K * np.dtype(dtype).itemsize >= 16 * tw:
+                                assert (t["block_threads"], t["loads_in_flight"]) == (blk, unr), t
+                            got = _run(cg, dt, x, K, direction, cin, cout, stream)
+                            torch.cuda.synchronize()
+                            assert bits_equal(got.cpu().numpy(), ref), (name, direction, K, tw, blk, unr, use_graph)
+            cg.set_tuning(direction, dt, K)
+    with pytest.raises(RuntimeError):
+        cg.set_tuning(capi.UP, dt, 8, block_threads=384)
+    cg.close()
