@@ -384,6 +384,19 @@ uint32_t residentBlocks(int device, const void* func) {
     return n;
 }
 
+// Every dispatch is checked on the host before it is issued: a grid of at least one block in
+// each dimension, within the hardware's limits (x < 2^31, y and z < 2^16), and a block size the
+// kernels are built for.  A violation is an internal error reported as GRGL_ERR_DEVICE; no
+// malformed dispatch reaches the device.
+void checkLaunch(dim3 grid, uint32_t block, const char* what) {
+    const bool ok = grid.x >= 1 && grid.y >= 1 && grid.z == 1 && grid.x <= 0x7fffffffU && grid.y <= 65535U &&
+                    (block == 256 || block == 512);
+    if (!ok)
+        throw DeviceError(std::string("internal error: malformed ") + what + " launch (grid " + std::to_string(grid.x) +
+                          " x " + std::to_string(grid.y) + " x " + std::to_string(grid.z) + ", block " +
+                          std::to_string(block) + ")");
+}
+
 template <typename T, bool XOR> void launchSeedT(dim3 grid, const SeedArgs& a, bool vr, bool vm, hipStream_t s) {
     constexpr int V = 16 / (int)sizeof(T);
     if (vr && vm) hipLaunchKernelGGL((seed_rows_kernel<T, XOR, V, V>), grid, dim3(kBlock), 0, s, a);
@@ -392,6 +405,7 @@ template <typename T, bool XOR> void launchSeedT(dim3 grid, const SeedArgs& a, b
     else hipLaunchKernelGGL((seed_rows_kernel<T, XOR, 1, 1>), grid, dim3(kBlock), 0, s, a);
 }
 void launchSeed(TypeClass tc, dim3 grid, const SeedArgs& a, bool vr, bool vm, hipStream_t s) {
+    checkLaunch(grid, kBlock, "seed");
     switch (tc) {
     case kF64: launchSeedT<double, false>(grid, a, vr, vm, s); break;
     case kF32: launchSeedT<float, false>(grid, a, vr, vm, s); break;
@@ -421,6 +435,7 @@ template <typename T, bool XOR> void launchStackedT(bool seed, dim3 grid, const 
 #undef GRGL_STACK
 }
 void launchStacked(TypeClass tc, bool seed, dim3 grid, const StackArgs& a, bool vr, bool vm, hipStream_t s) {
+    checkLaunch(grid, kBlock, seed ? "stacked seed" : "stacked read-out");
     switch (tc) {
     case kF64: launchStackedT<double, false>(seed, grid, a, vr, vm, s); break;
     case kF32: launchStackedT<float, false>(seed, grid, a, vr, vm, s); break;
@@ -446,6 +461,7 @@ template <typename T, bool XOR> void launchReadoutT(dim3 grid, const ReadoutArgs
     else launchReadoutM<T, XOR, 1>(grid, a, vr, vm, s);
 }
 void launchReadout(TypeClass tc, dim3 grid, const ReadoutArgs& a, bool vr, bool vm, hipStream_t s) {
+    checkLaunch(grid, kBlock, "read-out");
     switch (tc) {
     case kF64: launchReadoutT<double, false>(grid, a, vr, vm, s); break;
     case kF32: launchReadoutT<float, false>(grid, a, vr, vm, s); break;
@@ -461,6 +477,7 @@ void launchInit(TypeClass tc, const InitArgs& a, hipStream_t s) {
     const uint64_t n = (uint64_t)(a.rowEnd - a.rowBegin) * a.K;
     if (n == 0) return;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + kBlock - 1) / kBlock, 65536);
+    checkLaunch(dim3(blocks), kBlock, "row initialisation");
     switch (tc) {
     case kF64: hipLaunchKernelGGL((init_rows_kernel<double>), dim3(blocks), dim3(kBlock), 0, s, a); break;
     case kF32: hipLaunchKernelGGL((init_rows_kernel<float>), dim3(blocks), dim3(kBlock), 0, s, a); break;
@@ -735,6 +752,7 @@ void launchDirect(grgl_graph* g, std::vector<LevelLaunch>& launches, hipStream_t
             GRGL_HIP_CHECK(hipEventCreate(&ev1));
             GRGL_HIP_CHECK(hipEventRecord(ev0, stream));
         }
+        checkLaunch(L.grid, L.block, L.tail ? "tail level" : "level");
         void* args[1] = {L.argPtr()};
         GRGL_HIP_CHECK(hipLaunchKernel(L.func, L.grid, dim3(L.block), args, 0, stream));
         if (timing) {
@@ -752,6 +770,7 @@ void buildLevelGraph(std::vector<LevelLaunch>& launches, CapturedLevels& out) {
     GRGL_HIP_CHECK(hipGraphCreate(&out.graph, 0));
     hipGraphNode_t prev = nullptr;
     for (LevelLaunch& L : launches) {
+        checkLaunch(L.grid, L.block, L.tail ? "tail level" : "level");
         hipKernelNodeParams p{};
         void* args[1] = {L.argPtr()};
         p.func = const_cast<void*>(L.func);
@@ -820,6 +839,7 @@ bool launchChained(grgl_graph* g, const DeviceDirPlan& plan, const LaunchTable& 
     c.levelBegin[n] = plan.levelStart[plan.numLevels];
     c.totalTasks = task;
     const uint32_t grid = std::min<uint32_t>(task, residentBlocks(g->device, func));
+    checkLaunch(dim3(grid), kBlock, "chained level");
     void* args[1] = {&c};
     GRGL_HIP_CHECK(hipLaunchKernel(func, dim3(grid), dim3(kBlock), args, 0, stream));
     return true;
