@@ -29,7 +29,9 @@ try:
         MutationMappingStats,
         TraversalDirection,
         device_count,
+        GraphBuildStats,
         dot_product,
+        fast_grg_from_samples,
         get_device,
         grg_from_arrays,
         grg_from_coalescent,
@@ -59,7 +61,9 @@ __all__ = [
     "UP",
     "DOWN",
     "device_count",
+    "GraphBuildStats",
     "dot_product",
+    "fast_grg_from_samples",
     "get_device",
     "grg_from_arrays",
     "grg_from_coalescent",

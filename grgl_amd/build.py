@@ -22,7 +22,7 @@ ARCH = os.environ.get("GRGL_AMD_ARCH", "gfx950")
 HIP_SOURCES = ["grgl_hip.hip", "server.cpp", "levelize.cpp", "host_graph.cpp", "stacking.cpp", "stream_ceiling.hip",
                "noalloc.hip"]
 PY_SOURCES = ["pygrgl_module.cpp", "synth.cpp", "host_graph.cpp", "levelize.cpp", "stacking.cpp", "grg_io.cpp",
-              "construct.cpp"]
+              "construct.cpp", "fast_build.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")

@@ -1,0 +1,762 @@
+// Graph construction from haplotypes (see fast_build.h).  The steps follow the reference's
+// fastGRGFromSamples so that the graphs have its shape; the data structures are this
+// library's own: windows are fixed 256-bit words, the nearest-neighbour index is an
+// index-based BK-tree over node IDs, trees are built in parallel (each tree only reads its
+// own rows) and merged in tree order, so the result does not depend on the thread count.
+#include "fast_build.h"
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <thread>
+#include <unordered_map>
+
+namespace grgl_amd {
+
+HostGrg simplifyGrg(const HostGrg& g); // grg_io.cpp
+
+namespace {
+
+[[noreturn]] void misuse(const std::string& m) { throw std::runtime_error(m); }
+
+constexpr uint32_t kNone = std::numeric_limits<uint32_t>::max();
+constexpr uint32_t kMaxHapLength = 254; // src/hap_segment.h:50 (distances fit a byte there)
+constexpr uint32_t kPloidyCoal = 2;     // coalescences are tracked for diploids only (fast_build.cpp:365-369)
+
+// One haplotype segment: bit j set when the segment carries the window's j-th variant.
+using Seg = std::array<uint64_t, 4>;
+
+inline uint32_t popcount(const Seg& s) {
+    return (uint32_t)(__builtin_popcountll(s[0]) + __builtin_popcountll(s[1]) + __builtin_popcountll(s[2]) +
+                      __builtin_popcountll(s[3]));
+}
+
+struct SegHash {
+    size_t operator()(const Seg& s) const noexcept {
+        uint64_t h = 0x9e3779b97f4a7c15ULL;
+        for (uint64_t w : s) {
+            h ^= w + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2);
+            h *= 0xff51afd7ed558ccdULL;
+        }
+        return (size_t)(h ^ (h >> 33));
+    }
+};
+
+// A window of consecutive tree-mapped variants and its unique segments, numbered in order of
+// first appearance with 0 the empty segment (createWindow, fast_build.cpp:136-182;
+// HaplotypeWindow::addOrGetHaplotype, hap_segment.h:70-85).
+struct Window {
+    uint32_t firstVariant = 0; // index into the tree's variant list
+    uint32_t numVariants = 0;
+    std::vector<Seg> segs;
+    std::vector<uint32_t> bits;
+    std::unordered_map<Seg, uint32_t, SegHash> ids;
+
+    Window() { addOrGet(Seg{}, 0); }
+    uint32_t addOrGet(const Seg& s, uint32_t pc) {
+        auto ins = ids.emplace(s, (uint32_t)segs.size());
+        if (ins.second) {
+            segs.push_back(s);
+            bits.push_back(pc);
+        }
+        return ins.first->second;
+    }
+};
+
+// Rows of one tree, cut from the input by the tree-splitting rule, with their windows.
+struct TreeInput {
+    std::vector<uint32_t> variants;   // tree-mapped rows, in window bit order
+    std::vector<uint32_t> directRows; // rows mapped directly
+    std::vector<Window> windows;
+    std::vector<std::vector<uint32_t>> sampleSegs; // per sample: segment ID per window
+};
+
+// Per-tree graph, built in node-ID order (every node's children are known when it is made).
+struct TreeGraph {
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> kids;
+    std::vector<uint32_t> coal;
+    std::vector<uint32_t> numParents;
+    std::vector<uint32_t> mutNode; // per tree mutation: variants, then directRows
+    FastBuildStats stats;
+
+    uint32_t numNodes() const { return (uint32_t)off.size() - 1; }
+    uint32_t addNode(const uint32_t* c, size_t n) {
+        const uint32_t id = numNodes();
+        for (size_t i = 0; i < n; i++) {
+            kids.push_back(c[i]);
+            numParents[c[i]]++;
+        }
+        off.push_back(kids.size());
+        numParents.push_back(0);
+        return id;
+    }
+};
+
+// Diploid coalescence bookkeeping (propagateCoalInformation and getCoalsForParent,
+// fast_build.cpp:359-427, grg_helpers.h:418-474): a node's list holds the individuals with
+// exactly one haplotype below it, sorted; a parent's count is the number of individuals that
+// appear in two of its children's lists.
+struct Coals {
+    bool on;
+    uint32_t numSamples;
+    std::vector<std::vector<uint32_t>> lists; // by node ID; samples are implicit
+    std::vector<uint32_t> tmp;
+
+    void gather(const uint32_t* c, size_t n) {
+        tmp.clear();
+        for (size_t i = 0; i < n; i++) {
+            if (c[i] < numSamples) {
+                tmp.push_back(c[i] / kPloidyCoal);
+            } else if (c[i] < lists.size()) {
+                tmp.insert(tmp.end(), lists[c[i]].begin(), lists[c[i]].end());
+            }
+        }
+        std::sort(tmp.begin(), tmp.end());
+    }
+    // Pairs up tmp: returns the coalescence count, leaves the uncoalesced individuals in out.
+    uint32_t pairUp(std::vector<uint32_t>& out) const {
+        out.clear();
+        uint32_t pairs = 0;
+        for (size_t i = 0; i < tmp.size(); i++) {
+            if (i + 1 < tmp.size() && tmp[i + 1] == tmp[i]) {
+                pairs++;
+                i++;
+            } else {
+                out.push_back(tmp[i]);
+            }
+        }
+        return pairs;
+    }
+    std::vector<uint32_t>& at(uint32_t node) {
+        if (node >= lists.size()) lists.resize((size_t)node + 1);
+        return lists[node];
+    }
+    void drop(uint32_t node) {
+        if (node < lists.size()) std::vector<uint32_t>().swap(lists[node]);
+    }
+};
+
+// Nearest-neighbour index over nodes by summed Hamming distance of their segments: a
+// BK-tree whose vertices hold groups of identical nodes, searched from the query's own vertex
+// outwards (children and parent) with a comparison budget per query (src/lean_bk_tree.h and
+// src/hap_index.cpp; HaplotypeIndex::getMostSimilarNodes with collectAll = false).
+template <typename Dist> class NodeIndex {
+public:
+    NodeIndex(const Dist& d, double rebuildProportion) : dist_(d), rebuild_(rebuildProportion) {}
+
+    void add(uint32_t node) {
+        if (node >= where_.size()) {
+            where_.resize((size_t)node + 1, kNone);
+            next_.resize((size_t)node + 1, kNone);
+        }
+        where_[node] = insert(node);
+    }
+
+    // A new level: the per-query allowance, with five times that to start
+    // (HaplotypeIndex::setMaxComparePerQuery, hap_index.h:48-53).
+    void setMaxCompare(uint64_t perQuery) {
+        perQuery_ = perQuery;
+        budget_ = perQuery * 5;
+    }
+
+    // The members of the nearest group other than the query (removed from the index), and the
+    // query removed too.  hap_index.cpp:43-107.
+    void mostSimilar(uint32_t query, std::vector<uint32_t>& result, FastBuildStats& st) {
+        result.clear();
+        budget_ = std::max(perQuery_, budget_ + perQuery_);
+        uint64_t comps = 0;
+        const uint32_t best = lookup(where_.at(query), query, comps);
+        budget_ = comps > budget_ ? 0 : budget_ - comps;
+        where_[query] = kNone;
+        st.queries++;
+        st.comparisons += comps;
+        if (truncated_) st.truncatedQueries++;
+        if (best != kNone) {
+            moveElements(best, result);
+            deleted_++;
+        }
+        result.erase(std::remove(result.begin(), result.end(), query), result.end());
+        for (uint32_t x : result) where_[x] = kNone;
+        if (rebuild_ < 1.0 && (double)deleted_ / (double)verts_.size() >= rebuild_) rebuild(st);
+    }
+
+private:
+    struct Vert {
+        uint32_t head, tail; // group members, chained through next_
+        uint32_t parent, parentDist;
+        bool deleted;
+        std::vector<std::pair<uint32_t, uint32_t>> kids; // (distance, vertex)
+    };
+
+    uint32_t newVert(uint32_t node, uint32_t parent, uint32_t parentDist) {
+        verts_.push_back(Vert{node, node, parent, parentDist, false, {}});
+        next_[node] = kNone;
+        return (uint32_t)verts_.size() - 1;
+    }
+
+    // lean_bk_tree.h:88-118: walk from the root by distance; an identical node joins the
+    // vertex (reviving it if it was emptied).
+    uint32_t insert(uint32_t node) {
+        if (verts_.empty()) return newVert(node, kNone, kNone);
+        uint32_t cur = 0;
+        for (;;) {
+            const uint32_t d = dist_(verts_[cur].head, node);
+            if (d == 0) {
+                Vert& v = verts_[cur];
+                if (v.deleted) {
+                    v.head = v.tail = node;
+                    next_[node] = kNone;
+                    v.deleted = false;
+                    deleted_--;
+                } else {
+                    next_[v.tail] = node;
+                    next_[node] = kNone;
+                    v.tail = node;
+                }
+                return cur;
+            }
+            uint32_t nxt = kNone;
+            for (const auto& k : verts_[cur].kids)
+                if (k.first == d) nxt = k.second;
+            if (nxt == kNone) {
+                const uint32_t v = newVert(node, cur, d);
+                verts_[cur].kids.emplace_back(d, v);
+                return v;
+            }
+            cur = nxt;
+        }
+    }
+
+    // lean_bk_tree.h:120-175: depth-first from the query's vertex, pruned by the triangle
+    // inequality, until the budget is spent and a candidate is known.  A vertex holding only
+    // the query is emptied on the way.
+    uint32_t lookup(uint32_t start, uint32_t query, uint64_t& comps) {
+        truncated_ = false;
+        if (verts_.empty()) return kNone;
+        if (++epoch_ == 0) {
+            std::fill(seen_.begin(), seen_.end(), 0);
+            epoch_ = 1;
+        }
+        seen_.resize(verts_.size(), 0);
+        if (start == kNone) misuse("graph constructor: a query node is missing from the index");
+        uint32_t best = kNone;
+        uint64_t bestDist = std::numeric_limits<uint64_t>::max();
+        stack_.assign(1, start);
+        uint64_t c = 0;
+        while (!stack_.empty() && (c < budget_ || best == kNone)) {
+            const uint32_t n = stack_.back();
+            stack_.pop_back();
+            seen_[n] = epoch_;
+            Vert& v = verts_[n];
+            const uint64_t d = dist_(v.head, query);
+            c++;
+            bool ignore = v.deleted;
+            if (!ignore && v.head == v.tail && v.head == query) {
+                ignore = true;
+                v.deleted = true;
+                deleted_++;
+            }
+            if (!ignore && d < bestDist) {
+                best = n;
+                bestDist = d;
+            }
+            for (const auto& k : v.kids) {
+                if (seen_[k.second] == epoch_) continue;
+                const uint64_t bound = k.first > d ? k.first - d : d - k.first;
+                if (bound < bestDist) stack_.push_back(k.second);
+            }
+            if (v.parent != kNone && seen_[v.parent] != epoch_) {
+                const uint64_t bound = v.parentDist > d ? v.parentDist - d : d - v.parentDist;
+                if (bound < bestDist) stack_.push_back(v.parent);
+            }
+        }
+        truncated_ = !stack_.empty();
+        comps += c;
+        return best;
+    }
+
+    void moveElements(uint32_t vert, std::vector<uint32_t>& out) {
+        Vert& v = verts_[vert];
+        for (uint32_t x = v.head; x != kNone; x = next_[x]) out.push_back(x);
+        v.deleted = true;
+    }
+
+    // hap_index.cpp:93-104 with lean_bk_tree.h:189-209: the live members, depth-first from the
+    // root, reinserted into a fresh tree in that order.
+    void rebuild(FastBuildStats& st) {
+        std::vector<uint32_t> live;
+        std::vector<uint32_t> work(1, 0);
+        while (!work.empty()) {
+            const uint32_t n = work.back();
+            work.pop_back();
+            if (!verts_[n].deleted) moveElements(n, live);
+            for (const auto& k : verts_[n].kids) work.push_back(k.second);
+        }
+        verts_.clear();
+        deleted_ = 0;
+        std::fill(seen_.begin(), seen_.end(), 0);
+        epoch_ = 0;
+        for (uint32_t x : live) where_[x] = insert(x);
+        st.rebuilds++;
+    }
+
+    const Dist& dist_;
+    double rebuild_;
+    uint64_t perQuery_ = std::numeric_limits<uint64_t>::max();
+    uint64_t budget_ = std::numeric_limits<uint64_t>::max();
+    std::vector<Vert> verts_;
+    std::vector<uint32_t> where_, next_;
+    uint64_t deleted_ = 0;
+    std::vector<uint32_t> seen_, stack_;
+    uint32_t epoch_ = 0;
+    bool truncated_ = false;
+};
+
+// Close the current window: every sample's segment gets its ID (zero first, then in sample
+// order of first appearance), and the bit words are cleared for the next window.
+void closeWindow(TreeInput& t, std::vector<Seg>& cur, uint32_t first, uint32_t count) {
+    Window w;
+    w.firstVariant = first;
+    w.numVariants = count;
+    for (size_t s = 0; s < cur.size(); s++) {
+        const uint32_t id = w.addOrGet(cur[s], popcount(cur[s]));
+        t.sampleSegs[s].push_back(id);
+        cur[s] = Seg{};
+    }
+    t.windows.push_back(std::move(w));
+}
+
+// Cut the rows into trees and windows (getHapSegments, fast_build.cpp:266-342, and the
+// driver's loops, :857-947).  Automatic mode ends a tree after the window that brings the
+// summed unique-segment count to the target; with a tree count, trees take ceil(rows/count)
+// rows each (direct-mapped rows included) and the last one takes the rest.
+std::vector<TreeInput> cutTrees(uint32_t N, const std::vector<std::vector<uint32_t>>& carriers,
+                                const FastBuildParams& p) {
+    const size_t M = carriers.size();
+    const uint32_t L = p.hapLength;
+    const size_t directBelow = p.noTreeBelow < 1.0 ? (size_t)((double)N * p.noTreeBelow) : (size_t)p.noTreeBelow;
+    const bool byCount = p.treeCount > 0;
+    // getOptimalHapSegSum, fast_build.cpp:736-752 (a regression of the summed count on ln N).
+    const double predicted = 2383.46 * std::log((double)N) - 18143.49;
+    const size_t target = (size_t)std::min(20000.0, std::max(1000.0, predicted * p.hapSumMultiplier));
+    const size_t perTree = byCount ? (M + p.treeCount - 1) / p.treeCount : 0;
+
+    std::vector<TreeInput> trees;
+    std::vector<Seg> cur(N);
+    size_t row = 0;
+    while (row < M) {
+        TreeInput t;
+        t.sampleSegs.assign(N, {});
+        const bool last = byCount && trees.size() + 1 == p.treeCount;
+        size_t soFar = 0;
+        uint32_t windex = 0, first = 0;
+        bool done = false;
+        while (!done && row < M) {
+            const std::vector<uint32_t>& S = carriers[row];
+            if (byCount && ++soFar == perTree && !last) done = true;
+            if (S.size() < directBelow) {
+                t.directRows.push_back((uint32_t)row++);
+                continue;
+            }
+            for (uint32_t s : S) cur[s][windex >> 6] |= 1ULL << (windex & 63);
+            t.variants.push_back((uint32_t)row++);
+            if (++windex == L) {
+                closeWindow(t, cur, first, windex);
+                first = (uint32_t)t.variants.size();
+                windex = 0;
+                if (!byCount) {
+                    soFar += t.windows.back().segs.size();
+                    if (soFar >= target) done = true;
+                }
+            }
+        }
+        if (windex > 0) closeWindow(t, cur, first, windex);
+        trees.push_back(std::move(t));
+    }
+    return trees;
+}
+
+// One tree: cluster the nodes bottom-up, then map the tree's variants onto the roots and the
+// direct rows onto new nodes (buildTree, fast_build.cpp:430-706; directMap, :710-727).
+TreeGraph buildTree(uint32_t N, uint32_t ploidy, TreeInput& in, const std::vector<std::vector<uint32_t>>& carriers,
+                    const FastBuildParams& p) {
+    TreeGraph g;
+    g.off.assign((size_t)N + 1, 0);
+    g.numParents.assign(N, 0);
+    g.stats.windows = in.windows.size();
+    for (const Window& w : in.windows) g.stats.uniqueHapSegments += w.segs.size();
+    Coals coals{ploidy == kPloidyCoal, N, {}, {}};
+    const size_t W = in.windows.size();
+
+    // Segment IDs of every node, W per node.
+    std::vector<uint32_t> segOf;
+    segOf.reserve((size_t)N * W * 2);
+    for (uint32_t s = 0; s < N; s++) segOf.insert(segOf.end(), in.sampleSegs[s].begin(), in.sampleSegs[s].end());
+    std::vector<std::vector<uint32_t>>().swap(in.sampleSegs);
+
+    if (W > 0) {
+        auto dist = [&](uint32_t a, uint32_t b) -> uint32_t {
+            const uint32_t* x = segOf.data() + (size_t)a * W;
+            const uint32_t* y = segOf.data() + (size_t)b * W;
+            uint32_t d = 0;
+            for (size_t w = 0; w < W; w++) {
+                if (x[w] == y[w]) continue;
+                const Seg& u = in.windows[w].segs[x[w]];
+                const Seg& v = in.windows[w].segs[y[w]];
+                d += (uint32_t)(__builtin_popcountll(u[0] ^ v[0]) + __builtin_popcountll(u[1] ^ v[1]) +
+                                __builtin_popcountll(u[2] ^ v[2]) + __builtin_popcountll(u[3] ^ v[3]));
+            }
+            return d;
+        };
+        NodeIndex<decltype(dist)> index(dist, p.rebuildProportion);
+        std::vector<uint32_t> level(N);
+        for (uint32_t s = 0; s < N; s++) {
+            index.add(s);
+            level[s] = s;
+        }
+        std::vector<uint8_t> covered(N, 0);
+        std::vector<uint32_t> similar, inter(W), only1(W), only2(W), right, scratch;
+        uint64_t levelNo = 1;
+        bool created = true;
+        while (created) {
+            std::vector<uint32_t> nextLevel;
+            // fast_build.cpp:491-494: the allowance grows with the level.
+            const uint64_t perQuery = std::max<uint64_t>(50, (uint64_t)((double)levelNo * 50.0 * std::log2((double)N)));
+            index.setMaxCompare(perQuery);
+            created = false;
+            while (level.size() > 1) {
+                const uint32_t node = level.back();
+                level.pop_back();
+                if (covered[node]) continue;
+                covered[node] = 1;
+                index.mostSimilar(node, similar, g.stats);
+                if (similar.empty()) continue;
+                const uint32_t other = similar.front();
+                uint64_t isize = 0, size1 = 0, size2 = 0;
+                for (size_t w = 0; w < W; w++) {
+                    Window& win = in.windows[w];
+                    const uint32_t h1 = segOf[(size_t)node * W + w], h2 = segOf[(size_t)other * W + w];
+                    if (h1 == h2) {
+                        inter[w] = h1;
+                        only1[w] = only2[w] = 0;
+                        isize += win.bits[h1];
+                        continue;
+                    }
+                    const Seg a = win.segs[h1], b = win.segs[h2];
+                    const Seg both = {a[0] & b[0], a[1] & b[1], a[2] & b[2], a[3] & b[3]};
+                    const uint32_t nb = popcount(both);
+                    if (nb == 0) {
+                        inter[w] = 0;
+                        only1[w] = h1;
+                        only2[w] = h2;
+                        size1 += win.bits[h1];
+                        size2 += win.bits[h2];
+                        continue;
+                    }
+                    inter[w] = win.addOrGet(both, nb);
+                    const Seg a1 = {a[0] & ~b[0], a[1] & ~b[1], a[2] & ~b[2], a[3] & ~b[3]};
+                    const Seg b1 = {b[0] & ~a[0], b[1] & ~a[1], b[2] & ~a[2], b[3] & ~a[3]};
+                    const uint32_t n1 = popcount(a1), n2 = popcount(b1);
+                    only1[w] = n1 ? win.addOrGet(a1, n1) : 0;
+                    only2[w] = n2 ? win.addOrGet(b1, n2) : 0;
+                    size1 += n1;
+                    size2 += n2;
+                    isize += nb;
+                }
+                for (uint32_t x : similar) covered[x] = 1;
+                created = true;
+                if (isize == 0) continue;
+
+                // Intersection node over the query and its group; nodes for what each side
+                // holds alone (only the intersection joins the index).
+                scratch.assign(1, node);
+                scratch.insert(scratch.end(), similar.begin(), similar.end());
+                const uint32_t iNode = g.addNode(scratch.data(), scratch.size());
+                segOf.insert(segOf.end(), inter.begin(), inter.end());
+                covered.push_back(0);
+                nextLevel.push_back(iNode);
+                index.add(iNode);
+                uint32_t sub1 = kNone, sub2 = kNone;
+                if (size1 > 0) {
+                    sub1 = g.addNode(&node, 1);
+                    segOf.insert(segOf.end(), only1.begin(), only1.end());
+                    covered.push_back(0);
+                }
+                if (size2 > 0) {
+                    sub2 = g.addNode(similar.data(), similar.size());
+                    segOf.insert(segOf.end(), only2.begin(), only2.end());
+                    covered.push_back(0);
+                }
+                if (coals.on) {
+                    coals.gather(similar.data(), similar.size());
+                    const uint32_t rightPairs = coals.pairUp(right);
+                    coals.gather(scratch.data(), scratch.size());
+                    const uint32_t allPairs = coals.pairUp(coals.at(iNode));
+                    g.coal.resize(g.numNodes(), kCoalNotSet);
+                    g.coal[iNode] = allPairs;
+                    if (sub2 != kNone) {
+                        g.coal[sub2] = rightPairs;
+                        coals.at(sub2) = right;
+                    }
+                    if (sub1 != kNone) {
+                        g.coal[sub1] = 0;
+                        if (node < N) {
+                            coals.at(sub1).assign(1, node / kPloidyCoal);
+                        } else {
+                            std::vector<uint32_t> moved = std::move(coals.at(node));
+                            coals.at(sub1) = std::move(moved);
+                        }
+                    }
+                    coals.drop(node);
+                    for (uint32_t x : similar) coals.drop(x);
+                }
+            }
+            levelNo++;
+            level = std::move(nextLevel);
+        }
+    }
+
+    // Tree mapping (fast_build.cpp:656-706): each variant goes onto the roots whose segments
+    // carry it, through a new node when there are several.
+    const uint32_t built = g.numNodes();
+    const size_t V = in.variants.size();
+    g.mutNode.assign(V + in.directRows.size(), kInvalidNode);
+    if (p.treeMap) {
+        std::vector<std::vector<uint32_t>> onRoots(V);
+        for (uint32_t r = 0; r < built; r++) {
+            if (g.numParents[r] != 0) continue;
+            for (size_t w = 0; w < W; w++) {
+                const Window& win = in.windows[w];
+                const Seg& s = win.segs[segOf[(size_t)r * W + w]];
+                for (uint32_t k = 0; k < 4; k++) {
+                    uint64_t bits = s[k];
+                    while (bits) {
+                        const uint32_t j = k * 64 + (uint32_t)__builtin_ctzll(bits);
+                        bits &= bits - 1;
+                        onRoots[win.firstVariant + j].push_back(r);
+                    }
+                }
+            }
+        }
+        std::vector<uint32_t> unco;
+        for (size_t v = 0; v < V; v++) {
+            const std::vector<uint32_t>& roots = onRoots[v];
+            if (roots.empty()) {
+                g.stats.unmapped++;
+                continue;
+            }
+            g.stats.treeMapped++;
+            if (roots.size() == 1) {
+                g.mutNode[v] = roots[0];
+                continue;
+            }
+            const uint32_t mn = g.addNode(roots.data(), roots.size());
+            g.mutNode[v] = mn;
+            if (coals.on) {
+                coals.gather(roots.data(), roots.size());
+                g.coal.resize(g.numNodes(), kCoalNotSet);
+                g.coal[mn] = coals.pairUp(unco);
+            }
+        }
+        // Rows below the tree threshold: one node, an edge per carrier.
+        for (size_t k = 0; k < in.directRows.size(); k++) {
+            const std::vector<uint32_t>& S = carriers[in.directRows[k]];
+            const uint32_t mn = g.addNode(S.data(), S.size());
+            g.mutNode[V + k] = mn;
+            g.stats.directMapped++;
+            if (coals.on) {
+                coals.gather(S.data(), S.size());
+                g.coal.resize(g.numNodes(), kCoalNotSet);
+                g.coal[mn] = coals.pairUp(unco);
+            }
+        }
+    }
+    if (coals.on) g.coal.resize(g.numNodes(), kCoalNotSet);
+    return g;
+}
+
+// The tree as a graph file would hold it: children sorted, then simplified as the writer
+// does (each tree is saved before the merge, fast_build.cpp:893-897).
+HostGrg simplifiedTree(uint32_t N, uint32_t ploidy, TreeGraph& t) {
+    HostGrg h;
+    h.numSamples = N;
+    h.ploidy = ploidy;
+    h.downOff = std::move(t.off);
+    h.downChild = std::move(t.kids);
+    for (uint32_t n = N; n + 1 < h.downOff.size(); n++)
+        std::sort(h.downChild.begin() + (ptrdiff_t)h.downOff[n], h.downChild.begin() + (ptrdiff_t)h.downOff[n + 1]);
+    h.coal = std::move(t.coal);
+    h.mutNode = std::move(t.mutNode);
+    h.mutationsOrdered = true;
+    return simplifyGrg(h);
+}
+
+// Trees merged in order, combining nodes with equal (mapped, sorted) child lists
+// (MutableGRG::merge with combineNodes and the child-list hasher, grg_merge.cpp:188-279 and
+// :281-330).  The first tree's own duplicates are kept, as the hasher there indexes it
+// without merging.
+class Merger {
+public:
+    explicit Merger(uint32_t N) : N_(N) { off_.assign((size_t)N + 1, 0); }
+
+    // Returns the node map of the tree (tree node ID -> merged node ID).
+    std::vector<uint32_t> add(const HostGrg& t, bool first, FastBuildStats& st) {
+        const uint32_t V = t.numNodes();
+        std::vector<uint32_t> map(V, kNone);
+        for (uint32_t s = 0; s < N_; s++) map[s] = s;
+        std::vector<uint32_t> c;
+        for (uint32_t n = N_; n < V; n++) {
+            c.clear();
+            for (uint64_t e = t.downOff[n]; e < t.downOff[n + 1]; e++) c.push_back(map[t.downChild[e]]);
+            std::sort(c.begin(), c.end());
+            const uint64_t h = hashOf(c);
+            const uint32_t found = find(h, c);
+            if (!first && found != kNone) {
+                map[n] = found;
+                st.combinedNodes++;
+                continue;
+            }
+            const uint32_t id = (uint32_t)(off_.size() - 1);
+            kids_.insert(kids_.end(), c.begin(), c.end());
+            off_.push_back(kids_.size());
+            coal_.push_back(t.hasCoals() ? t.coal[n] : kCoalNotSet);
+            if (found == kNone) {
+                auto ins = table_.emplace(h, id);
+                chain_.push_back(ins.second ? kNone : ins.first->second);
+                if (!ins.second) ins.first->second = id;
+            } else {
+                chain_.push_back(kNone); // a duplicate inside the first tree: not indexed
+            }
+            map[n] = id;
+        }
+        return map;
+    }
+
+    uint32_t numNodes() const { return (uint32_t)(off_.size() - 1); }
+    const std::vector<uint64_t>& off() const { return off_; }
+    const std::vector<uint32_t>& kids() const { return kids_; }
+    uint32_t coal(uint32_t n) const { return coal_[n - N_]; }
+
+private:
+    static uint64_t hashOf(const std::vector<uint32_t>& c) {
+        uint64_t h = 0xcbf29ce484222325ULL ^ c.size();
+        for (uint32_t x : c) {
+            h ^= x;
+            h *= 0x100000001b3ULL;
+            h ^= h >> 29;
+        }
+        return h;
+    }
+    uint32_t find(uint64_t h, const std::vector<uint32_t>& c) const {
+        auto it = table_.find(h);
+        for (uint32_t n = it == table_.end() ? kNone : it->second; n != kNone; n = chain_[n - N_]) {
+            const uint64_t b = off_[n], e = off_[n + 1];
+            if (e - b == c.size() && std::equal(c.begin(), c.end(), kids_.begin() + (ptrdiff_t)b)) return n;
+        }
+        return kNone;
+    }
+
+    uint32_t N_;
+    std::vector<uint64_t> off_;
+    std::vector<uint32_t> kids_, coal_, chain_;
+    std::unordered_map<uint64_t, uint32_t> table_;
+};
+
+} // namespace
+
+MutableGraph fastBuildFromSamples(uint32_t numSamples, uint32_t ploidy, const std::vector<MutationRecordIn>& rows,
+                                  const std::vector<std::vector<uint32_t>>& carriersIn, const FastBuildParams& params,
+                                  FastBuildStats& stats, uint32_t threads) {
+    if (rows.size() != carriersIn.size()) misuse("mutations and samples must be the same length");
+    if (numSamples == 0) misuse("num_samples must be positive");
+    if (params.hapLength == 0 || params.hapLength > kMaxHapLength) misuse("hap_length must be in 1..254");
+    if (!(params.rebuildProportion > 0.0)) misuse("rebuild_proportion must be positive");
+    if (!(params.hapSumMultiplier > 0.0)) misuse("the segment-sum multiplier must be positive");
+    if (params.noTreeBelow < 0.0) misuse("lf_no_tree must not be negative");
+    MutableGraph out(numSamples, ploidy, true);
+    std::vector<std::vector<uint32_t>> carriers(carriersIn.size());
+    for (size_t i = 0; i < rows.size(); i++) {
+        if (rows[i].allele == ".") misuse("missing-data rows are not supported by the graph constructor");
+        carriers[i] = carriersIn[i];
+        std::sort(carriers[i].begin(), carriers[i].end());
+        carriers[i].erase(std::unique(carriers[i].begin(), carriers[i].end()), carriers[i].end());
+        if (!carriers[i].empty() && carriers[i].back() >= numSamples) misuse("sample ID out of range");
+    }
+    stats = FastBuildStats{};
+
+    std::vector<TreeInput> inputs = cutTrees(numSamples, carriers, params);
+    const size_t T = inputs.size();
+    std::vector<HostGrg> trees(T);
+    std::vector<FastBuildStats> treeStats(T);
+    std::vector<std::string> errors(T);
+    if (threads == 0) threads = std::max(1U, std::thread::hardware_concurrency());
+    threads = (uint32_t)std::min<size_t>(threads, std::max<size_t>(T, 1));
+    std::atomic<size_t> nextTree{0};
+    auto worker = [&]() {
+        for (size_t t = nextTree++; t < T; t = nextTree++) {
+            try {
+                TreeGraph g = buildTree(numSamples, ploidy, inputs[t], carriers, params);
+                treeStats[t] = g.stats;
+                trees[t] = simplifiedTree(numSamples, ploidy, g);
+                TreeInput().sampleSegs.swap(inputs[t].sampleSegs);
+                std::vector<Window>().swap(inputs[t].windows);
+            } catch (const std::exception& e) {
+                errors[t] = e.what();
+            }
+        }
+    };
+    if (threads <= 1) {
+        worker();
+    } else {
+        std::vector<std::thread> pool;
+        for (uint32_t i = 0; i < threads; i++) pool.emplace_back(worker);
+        for (auto& th : pool) th.join();
+    }
+    for (const std::string& e : errors)
+        if (!e.empty()) throw std::runtime_error(e);
+
+    Merger merged(numSamples);
+    std::vector<uint32_t> nodeOfRow(rows.size(), kInvalidNode);
+    for (size_t t = 0; t < T; t++) {
+        const FastBuildStats& s = treeStats[t];
+        stats.trees++;
+        stats.windows += s.windows;
+        stats.uniqueHapSegments += s.uniqueHapSegments;
+        stats.directMapped += s.directMapped;
+        stats.treeMapped += s.treeMapped;
+        stats.unmapped += s.unmapped;
+        stats.comparisons += s.comparisons;
+        stats.queries += s.queries;
+        stats.truncatedQueries += s.truncatedQueries;
+        stats.rebuilds += s.rebuilds;
+        const std::vector<uint32_t> map = merged.add(trees[t], t == 0, stats);
+        const TreeInput& in = inputs[t];
+        for (size_t k = 0; k < trees[t].mutNode.size(); k++) {
+            const uint32_t n = trees[t].mutNode[k];
+            const uint32_t row = k < in.variants.size() ? in.variants[k] : in.directRows[k - in.variants.size()];
+            nodeOfRow[row] = n == kInvalidNode ? kInvalidNode : map[n];
+        }
+        trees[t] = HostGrg();
+    }
+    if (!params.treeMap) stats.unmapped = 0;
+
+    const uint32_t V = merged.numNodes();
+    if (V > numSamples) out.makeNode(V - numSamples);
+    for (uint32_t n = numSamples; n < V; n++) {
+        for (uint64_t e = merged.off()[n]; e < merged.off()[n + 1]; e++) out.connect(n, merged.kids()[e]);
+        if (ploidy == kPloidyCoal && merged.coal(n) != kCoalNotSet) out.setNumIndividualCoals(n, merged.coal(n));
+    }
+    // Without tree mapping the graph is returned without mutations (the caller maps them,
+    // fast_build.cpp:652-653 and grgl.cpp:331-350).
+    if (params.treeMap) {
+        for (size_t i = 0; i < rows.size(); i++) out.addMutation(rows[i], nodeOfRow[i]);
+    }
+    return out;
+}
+
+} // namespace grgl_amd

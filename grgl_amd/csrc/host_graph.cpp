@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <stdexcept>
+#include <thread>
 
 namespace grgl_amd {
 
@@ -55,6 +56,57 @@ GraphStats computeGraphStats(const HostGrg& g) {
     st.upLevels = maxH + 1;
     while (st.fanInHistogram.size() > 1 && st.fanInHistogram.back() == 0) st.fanInHistogram.pop_back();
     return st;
+}
+
+void mutationCarriers(const HostGrg& g, std::vector<uint64_t>& off, std::vector<uint32_t>& samples,
+                      uint32_t threads) {
+    validateHostGrg(g);
+    const uint32_t M = g.numMutations(), V = g.numNodes(), N = g.numSamples;
+    if (threads == 0) threads = std::max(1U, std::thread::hardware_concurrency());
+    threads = std::max(1U, std::min<uint32_t>(threads, (M + 255) / 256));
+    std::vector<std::vector<uint32_t>> part(threads);
+    std::vector<std::vector<uint64_t>> counts(threads);
+    auto work = [&](uint32_t t) {
+        const uint32_t b = (uint32_t)((uint64_t)M * t / threads), e = (uint32_t)((uint64_t)M * (t + 1) / threads);
+        std::vector<uint32_t> stamp(V, 0), stack;
+        uint32_t epoch = 0;
+        for (uint32_t m = b; m < e; m++) {
+            const size_t before = part[t].size();
+            const uint32_t root = g.mutNode[m];
+            if (root != kInvalidNode) {
+                epoch++;
+                stack.assign(1, root);
+                stamp[root] = epoch;
+                while (!stack.empty()) {
+                    const uint32_t n = stack.back();
+                    stack.pop_back();
+                    if (n < N) {
+                        part[t].push_back(n);
+                        continue;
+                    }
+                    for (uint64_t i = g.downOff[n]; i < g.downOff[n + 1]; i++) {
+                        const uint32_t c = g.downChild[i];
+                        if (stamp[c] != epoch) {
+                            stamp[c] = epoch;
+                            stack.push_back(c);
+                        }
+                    }
+                }
+                std::sort(part[t].begin() + (ptrdiff_t)before, part[t].end());
+            }
+            counts[t].push_back(part[t].size() - before);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < threads; t++) pool.emplace_back(work, t);
+    work(0);
+    for (auto& th : pool) th.join();
+    off.assign(1, 0);
+    samples.clear();
+    for (uint32_t t = 0; t < threads; t++) {
+        for (uint64_t c : counts[t]) off.push_back(off.back() + c);
+        samples.insert(samples.end(), part[t].begin(), part[t].end());
+    }
 }
 
 } // namespace grgl_amd

@@ -19,6 +19,7 @@
 
 #include "grgl_hip.h"
 #include "construct.h"
+#include "fast_build.h"
 #include "host_graph.h"
 #include "levelize.h"
 #include "stacking.h"
@@ -559,6 +560,18 @@ PYBIND11_MODULE(_grgl_amd, m) {
                  return d;
              },
              "The graph as plain arrays (down-edge CSR, mutation map, coalescence counts).")
+        .def("mutation_carriers",
+             [](PyGRG& g) {
+                 std::vector<uint64_t> off;
+                 std::vector<uint32_t> samples;
+                 {
+                     py::gil_scoped_release release;
+                     mutationCarriers(g.H(), off, samples);
+                 }
+                 return py::make_tuple(toArray(off), toArray(samples));
+             },
+             "The samples carrying each mutation as CSR arrays (offsets of length num_mutations+1, sorted "
+             "sample IDs): the sparse genotype matrix the graph encodes.")
         .def("graph_stats",
              [](PyGRG& g) {
                  const GraphStats st = computeGraphStats(g.H());
@@ -684,6 +697,98 @@ PYBIND11_MODULE(_grgl_amd, m) {
         "Nodes already in the graph when a batch starts are reused where they fit.  size_order=True (not in "
         "the reference) maps the sets smallest first; with mutation_batch_size=1 this builds a compact graph "
         "from an empty one.  Mutation IDs follow the input order.");
+
+    py::class_<FastBuildStats>(m, "GraphBuildStats")
+        .def_readonly("trees", &FastBuildStats::trees)
+        .def_readonly("windows", &FastBuildStats::windows)
+        .def_readonly("unique_hap_segments", &FastBuildStats::uniqueHapSegments)
+        .def_readonly("direct_mapped", &FastBuildStats::directMapped)
+        .def_readonly("tree_mapped", &FastBuildStats::treeMapped)
+        .def_readonly("unmapped", &FastBuildStats::unmapped)
+        .def_readonly("comparisons", &FastBuildStats::comparisons)
+        .def_readonly("queries", &FastBuildStats::queries)
+        .def_readonly("truncated_queries", &FastBuildStats::truncatedQueries)
+        .def_readonly("rebuilds", &FastBuildStats::rebuilds)
+        .def_readonly("combined_nodes", &FastBuildStats::combinedNodes);
+
+    m.def(
+        "fast_grg_from_samples",
+        [toRecord](uint32_t numSamples, uint32_t ploidy, py::object mutations, py::object samples,
+                   py::object carrierOffsets, py::object carriers, py::object positions, py::object trees,
+                   double lfNoTree, bool noTreeMap, double rebuildProportion, uint32_t threads, bool returnStats) {
+            FastBuildParams p;
+            p.noTreeBelow = lfNoTree;
+            p.treeMap = !noTreeMap;
+            p.rebuildProportion = rebuildProportion;
+            // The --trees option of the reference's builder (src/grgl.cpp:295-311).
+            if (py::isinstance<py::str>(trees)) {
+                const std::string t = trees.cast<std::string>();
+                if (t == "faster1") {
+                    p.hapSumMultiplier = 0.75;
+                } else if (t == "faster2") {
+                    p.hapSumMultiplier = 0.5;
+                } else if (t != "optimal") {
+                    throw py::value_error("trees must be 'optimal', 'faster1', 'faster2' or a positive integer");
+                }
+            } else {
+                const long long n = trees.cast<long long>();
+                API_CHECK(n > 0, "Must provide non-zero trees argument");
+                p.treeCount = (uint32_t)n;
+            }
+            std::vector<std::vector<uint32_t>> sets;
+            if (!samples.is_none()) {
+                API_CHECK(carrierOffsets.is_none() && carriers.is_none(), "give samples or carrier arrays, not both");
+                sets = samples.cast<std::vector<std::vector<uint32_t>>>();
+            } else {
+                API_CHECK(!carrierOffsets.is_none() && !carriers.is_none(),
+                          "give samples (a list per mutation) or carrier_offsets and carriers");
+                auto off = carrierOffsets.cast<py::array_t<int64_t, py::array::c_style | py::array::forcecast>>();
+                auto car = carriers.cast<py::array_t<uint32_t, py::array::c_style | py::array::forcecast>>();
+                API_CHECK(off.ndim() == 1 && off.size() >= 1, "carrier_offsets must be a 1-D array of M+1 offsets");
+                const int64_t* o = off.data();
+                const uint32_t* c = car.data();
+                sets.resize((size_t)off.size() - 1);
+                for (size_t i = 0; i + 1 < (size_t)off.size(); i++) {
+                    API_CHECK(o[i] >= 0 && o[i] <= o[i + 1] && o[i + 1] <= (int64_t)car.size(), "bad carrier_offsets");
+                    sets[i].assign(c + o[i], c + o[i + 1]);
+                }
+            }
+            std::vector<MutationRecordIn> recs;
+            if (!mutations.is_none()) {
+                for (const auto& x : mutations.cast<std::vector<PyMutation>>()) recs.push_back(toRecord(x));
+            } else {
+                std::vector<double> pos;
+                if (!positions.is_none()) pos = positions.cast<std::vector<double>>();
+                API_CHECK(pos.empty() || pos.size() == sets.size(), "positions must have one entry per mutation");
+                recs.resize(sets.size());
+                for (size_t i = 0; i < sets.size(); i++) {
+                    recs[i].position = pos.empty() ? (double)i : pos[i];
+                    recs[i].allele = "G";
+                    recs[i].refAllele = "A";
+                }
+            }
+            API_CHECK(recs.size() == sets.size(), "mutations and samples must be the same length");
+            auto g = std::make_shared<PyMutableGRG>(numSamples, ploidy, true);
+            FastBuildStats st;
+            {
+                py::gil_scoped_release release;
+                g->mg = fastBuildFromSamples(numSamples, ploidy, recs, sets, p, st, threads);
+            }
+            if (returnStats) return py::object(py::make_tuple(g, st));
+            return py::object(py::cast(g));
+        },
+        py::arg("num_samples"), py::arg("ploidy"), py::arg("mutations") = py::none(), py::arg("samples") = py::none(),
+        py::kw_only(), py::arg("carrier_offsets") = py::none(), py::arg("carriers") = py::none(),
+        py::arg("positions") = py::none(), py::arg("trees") = py::str("optimal"), py::arg("lf_no_tree") = 0.0,
+        py::arg("no_tree_map") = false, py::arg("rebuild_proportion") = 0.10, py::arg("threads") = 0,
+        py::arg("return_stats") = false,
+        "Build a GRG from haplotypes, as the reference's graph construction does (fastGRGFromSamples, "
+        "src/fast_build.cpp:794-986): the variant rows are cut into trees, each tree clusters the samples by "
+        "haplotype-segment similarity and maps its rows onto the roots, and the trees are merged combining equal "
+        "nodes.  Input: one carrier list per mutation (samples=...), or CSR arrays (carrier_offsets, carriers), in "
+        "genome order.  trees: 'optimal', 'faster1', 'faster2' or a count; lf_no_tree: rows carried by fewer "
+        "samples (a frequency when < 1) are mapped directly; no_tree_map: return the graph without mutations.  "
+        "Mutation IDs follow the row order.  Missing-data rows are not supported.");
 
     m.def("matmul", &matMul, py::arg("grg"), py::arg("input"), py::arg("direction"), py::arg("emit_all_nodes") = false,
           py::arg("by_individual") = false, py::arg("init") = py::none(), py::arg("miss") = py::none(),
