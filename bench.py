@@ -427,8 +427,9 @@ def grg_io_benchmark(g):
 
 
 def construction_benchmark():
-    """map_mutations (smallest carrier sets first, one mutation per batch) on the carrier lists
-    of the reference's test VCF (400 haplotypes x 10,893 sites), on one host core."""
+    """Graph construction from the carrier lists of the reference's test VCF (400 haplotypes x
+    10,893 sites): map_mutations (smallest carrier sets first, one mutation per batch, one host
+    core) and the haplotype-clustering constructor (fast_grg_from_samples, default options)."""
     import numpy as np
 
     import grgl_amd
@@ -442,10 +443,132 @@ def construction_benchmark():
     grgl_amd.map_mutations(g, muts, sets, mutation_batch_size=1, size_order=True)
     dt = time.perf_counter() - t
     entries = int(sum(len(x) for x in sets))
+    t = time.perf_counter()
+    fb = grgl_amd.fast_grg_from_samples(int(fx["num_samples"]), int(fx["ploidy"]), carrier_offsets=off, carriers=car,
+                                        positions=fx["positions"])
+    fb_s = time.perf_counter() - t
     return {"workload": "test-200-samples VCF carriers: 400 haplotypes x 10,893 sites",
             "seconds": round(dt, 3), "mutations_per_s": round(len(sets) / dt, 1),
             "nodes": g.num_nodes, "edges": g.num_edges, "carrier_entries": entries,
-            "edges_per_carrier": round(g.num_edges / entries, 4)}
+            "edges_per_carrier": round(g.num_edges / entries, 4),
+            "fast_grg_from_samples": {"seconds": round(fb_s, 3), "nodes": fb.num_nodes, "edges": fb.num_edges,
+                                      "edges_per_carrier": round(fb.num_edges / entries, 4)}}
+
+
+def constructed_graph_benchmark(haplotypes=20_000, sites=100_000, trees=20, autotune=True):
+    """The traversal on the graph shape the haplotype-clustering constructor builds
+    (fast_grg_from_samples), beside the tree-direct graph of the same genotype matrix:
+    `haplotypes` x `sites` from `trees` seeded coalescent segment trees, carriers taken from the
+    tree-direct graph, the constructed graph saved and reloaded (simplified on write, as its
+    .grg file holds it).  Per graph: shape (nodes, edges, levels, fan-in histogram), plain f32
+    ms at K = 1, 16, 256 with the default launch shapes and autotuned, and the K ladder
+    (511 stacked RHS) in one traversal with the level kernels' live HIP-event time.  The int32
+    products of the two graphs must be bit-identical (one genotype matrix)."""
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    import grgl_amd
+    from grgl_amd import capi
+
+    src = grgl_amd.grg_from_coalescent(haplotypes, sites, trees, seed=SEED)
+    off, car = src.mutation_carriers()
+    t = time.perf_counter()
+    built, st = grgl_amd.fast_grg_from_samples(haplotypes, 2, carrier_offsets=off, carriers=car,
+                                               positions=src.arrays()["positions"], return_stats=True)
+    build_s = time.perf_counter() - t
+    with tempfile.TemporaryDirectory() as tmp:
+        fn = os.path.join(tmp, "constructed.grg")
+        grgl_amd.save_grg(built, fn)
+        con = grgl_amd.load_immutable_grg(fn)
+    del built
+    out = {"workload": f"{haplotypes:,} haplotypes x {sites:,} sites ({trees} coalescent segment trees); the "
+                       "constructed graph built from the carrier lists",
+           "carrier_entries": int(len(car)), "build_s": round(build_s, 2),
+           "build": {"trees": st.trees, "windows": st.windows, "unique_hap_segments": st.unique_hap_segments,
+                     "queries": st.queries, "comparisons": st.comparisons,
+                     "truncated_queries": st.truncated_queries, "combined_nodes": st.combined_nodes}}
+    del off, car
+    sp = torch.cuda.current_stream().cuda_stream
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(SEED + 3)
+    checks = {}
+    for name, g in (("tree_direct", src), ("constructed", con)):
+        a = g.arrays()
+        cg = capi.Graph(a)
+        info = cg.info()
+        V, E, N, M = info["num_nodes"], info["num_edges"], info["num_samples"], info["num_mutations"]
+        shape = g.graph_stats()
+        rec = {"nodes": V, "edges": E, "levels_up": info["up_levels"], "levels_down": info["down_levels"],
+               "max_fanin_up": info["max_fanin_up"], "max_fanin_down": info["max_fanin_down"],
+               "fanin_histogram_log2": shape["fan_in_histogram"]}
+        for d, dn in ((capi.UP, "UP"), (capi.DOWN, "DOWN")):
+            cin, cout = (N, M) if d == capi.UP else (M, N)
+            for K in (1, 16, 256):
+                x = torch.randn((K, cin), device="cuda", generator=gen)
+                y = torch.zeros((K, cout), device="cuda")
+
+                def timed():
+                    cg.matmul_ptr(capi.F32, x.data_ptr(), cin, K, d, y.data_ptr(), y.numel(), stream=sp)
+                    torch.cuda.synchronize()
+                    reps = 10
+                    t0 = time.perf_counter()
+                    for _ in range(reps):
+                        cg.matmul_ptr(capi.F32, x.data_ptr(), cin, K, d, y.data_ptr(), y.numel(), stream=sp)
+                    torch.cuda.synchronize()
+                    return (time.perf_counter() - t0) / reps * 1e3
+
+                ms_default = timed()
+                r = {"ms_default": round(ms_default, 4)}
+                if autotune:
+                    best, _ = cg.autotune(d, capi.F32, K)
+                    ms_tuned = timed()
+                    r.update({"ms_tuned": round(ms_tuned, 4), "tuning": best})
+                ms = min(ms_default, r.get("ms_tuned", ms_default))
+                r["edge_rhs_per_s"] = E * K / (ms / 1e3)
+                r["alg_GBps"] = round(algorithmic_bytes(V, E, K, 4, cin, cout) / (ms / 1e3) / 1e9, 1)
+                rec[f"plain_f32_{dn}_K{K}"] = r
+            # The K ladder stacked in one traversal, level kernels timed with HIP events.
+            Ktot = sum(K_LADDER)
+            x = torch.randn((Ktot, cin), device="cuda", generator=gen)
+            y = torch.zeros((Ktot, cout), device="cuda")
+            reqs, k0 = [], 0
+            for K in K_LADDER:
+                reqs.append((K, x[k0:k0 + K].data_ptr(), y[k0:k0 + K].data_ptr()))
+                k0 += K
+            if autotune:
+                cg.autotune(d, capi.F32, Ktot)
+            cg.matmul_batch_ptr(capi.F32, d, reqs, stream=sp)
+            torch.cuda.synchronize()
+            cg.kernel_timing_read(reset=True)
+            reps = 5
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                cg.kernel_timing(True)
+                cg.matmul_batch_ptr(capi.F32, d, reqs, stream=sp)
+                cg.kernel_timing(False)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) / reps * 1e3
+            k_ms, k_n, k_bytes = cg.kernel_timing_read(reset=True)
+            achieved = k_bytes / (k_ms / 1e3) / 1e9 if k_ms > 0 else 0.0
+            rec[f"batched_f32_{dn}_511rhs"] = {
+                "ms": round(ms, 4), "edge_rhs_per_s": E * Ktot / (ms / 1e3),
+                "alg_GBps": round(algorithmic_bytes(V, E, Ktot, 4, cin, cout) / (ms / 1e3) / 1e9, 1),
+                "level_kernels_GBps": round(achieved, 1), "level_kernels_frac": round(achieved / HBM_PEAK_GBS, 4),
+                "level_launches": k_n // reps}
+            # int32 products for the cross-graph check
+            xi = (torch.rand((16, cin), device="cuda", generator=torch.Generator(device="cuda").manual_seed(
+                SEED + (1 if d == capi.UP else 2))) < 0.5).to(torch.int32)
+            yi = torch.zeros((16, cout), device="cuda", dtype=torch.int32)
+            cg.matmul_ptr(capi.I32, xi.data_ptr(), cin, 16, d, yi.data_ptr(), yi.numel(), stream=sp)
+            torch.cuda.synchronize()
+            checks.setdefault(dn, []).append(yi.cpu().numpy())
+        cg.close()
+        out[name] = rec
+        log(f"{name}: {json.dumps(rec)}")
+    out["int32_products_identical"] = bool(all(np.array_equal(v[0], v[1]) for v in checks.values()))
+    return out
 
 
 def fp32_errors(arrays3, f32_results):
@@ -662,6 +785,8 @@ def main():
                     help="only time one-request FP32 traversals of the benchmark graph at K = 1, 4, 16 (UP and "
                          "DOWN, library default tuning), 20 calls per shape with 0.3 s between shapes, so a "
                          "kernel trace splits by shape")
+    ap.add_argument("--constructed-only", action="store_true",
+                    help="only measure the constructed-graph shape beside the tree-direct graph (20,000 x 100,000)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -677,6 +802,9 @@ def main():
         return
     if args.latency_trace:
         latency_trace(args)
+        return
+    if args.constructed_only:
+        print(json.dumps(constructed_graph_benchmark(autotune=not args.no_autotune)), flush=True)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -851,6 +979,8 @@ def main():
         log("options, .grg file and construction ...")
         extras = {"options_K16_f32": options_benchmark(cg, N, M, V, E), "host_path_f32": host_path_benchmark(g, E),
                   "grg_io": grg_io_benchmark(g), "construction": construction_benchmark()}
+        log("constructed-graph shape ...")
+        extras["constructed_graph"] = constructed_graph_benchmark(autotune=not args.no_autotune)
         log(f"extras: {json.dumps(extras)}")
     cpu = None
     if not args.no_cpu_baseline:

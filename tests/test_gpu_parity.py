@@ -93,6 +93,31 @@ def test_golden_vcf_allele_counts(gpu):
     np.testing.assert_array_equal(zyg, fx["expect_Aa"] + 2 * fx["expect_aa"])
 
 
+def test_golden_vcf_built_graph(gpu):
+    """The reference's expect files on the graph the constructor builds from the same VCF:
+    allele counts, and homozygous counts from the coalescence counts (init="xtx")."""
+    fx = vcf_fixture()
+    g = _pg(_built_vcf())
+    counts = grgl_amd.matmul(g, np.ones((1, g.num_samples), np.int32), UP)[0]
+    np.testing.assert_array_equal(counts, fx["expect_alt_count"])
+    hom2 = grgl_amd.matmul(g, np.zeros((1, g.num_samples), np.int64), UP, init="xtx")[0]
+    np.testing.assert_array_equal(hom2, 2 * fx["expect_aa"])
+    zyg = grgl_amd.matmul(g, np.ones((1, g.num_individuals), np.int32), UP, by_individual=True)[0]
+    np.testing.assert_array_equal(zyg, fx["expect_Aa"] + 2 * fx["expect_aa"])
+
+
+def test_built_graph_equals_tree_graph(gpu):
+    """Two graphs of one genotype matrix (tree-direct and constructor-built, 1,000 x 10,000):
+    integer products are bit-identical in both directions."""
+    src = grgl_amd.grg_from_coalescent(1000, 10000, 1)
+    built = _pg(_built_from(src))
+    rng = np.random.default_rng(12)
+    for direction, cols in ((UP, src.num_samples), (DOWN, src.num_mutations)):
+        for K in (1, 16):
+            x = rng.integers(-9, 10, size=(K, cols)).astype(np.int32)
+            assert bits_equal(grgl_amd.matmul(built, x, direction), grgl_amd.matmul(src, x, direction)), (direction, K)
+
+
 def test_matmul_bad(gpu):
     """test_grg.cpp:126-161 on the device path."""
     g = _pg(depth3_bin_tree([1, 5]))
@@ -103,12 +128,26 @@ def test_matmul_bad(gpu):
 
 # --- oracle parity -----------------------------------------------------------------------
 
+def _built_from(src):
+    off, car = src.mutation_carriers()
+    return grgl_amd.fast_grg_from_samples(src.num_samples, src.ploidy, carrier_offsets=off, carriers=car).arrays()
+
+
+def _built_vcf():
+    fx = vcf_fixture()
+    return grgl_amd.fast_grg_from_samples(int(fx["num_samples"]), int(fx["ploidy"]), carrier_offsets=fx["carrier_off"],
+                                          carriers=fx["carriers"], positions=fx["positions"]).arrays()
+
+
 GRAPHS = {
     "sample8": lambda: sample8_grg(list(range(8, 14)) + [0, 3]),
     "random_heavy": lambda: random_grg(7, num_samples=300, num_internal=2000, max_children=8,
                                        heavy=(90, 700, 3000, 1500), num_mutations=3000),
     "tree_cfg1": lambda: grgl_amd.grg_from_coalescent(1000, 10000, 1).arrays(),
     "vcf": lambda: direct_map_grg(vcf_fixture()),
+    # the same genotypes built by the haplotype-clustering constructor
+    "built_cfg1": lambda: _built_from(grgl_amd.grg_from_coalescent(1000, 10000, 1)),
+    "built_vcf": lambda: _built_vcf(),
 }
 _cache = {}
 
@@ -134,7 +173,7 @@ def test_parity_all_dtypes(gpu, name, dtype):
             if np.dtype(dtype).kind in "biu":
                 assert bits_equal(out, og.matmul(x, _odir(direction))), (direction, K)
             else:
-                _check_float(out, x, og, direction, bar=FP32_BAR if name == "tree_cfg1" else None)
+                _check_float(out, x, og, direction, bar=FP32_BAR if name in ("tree_cfg1", "built_cfg1") else None)
                 exact = grgl_amd.matmul(g, x, direction, order_preserving=True)
                 assert bits_equal(exact, og.matmul(x, _odir(direction))), (direction, K)
 
