@@ -455,15 +455,17 @@ def construction_benchmark():
                                       "edges_per_carrier": round(fb.num_edges / entries, 4)}}
 
 
-def constructed_graph_benchmark(haplotypes=20_000, sites=100_000, trees=20, autotune=True):
+def constructed_graph_benchmark(haplotypes=100_000, sites=500_000, trees=20, autotune=True):
     """The traversal on the graph shape the haplotype-clustering constructor builds
     (fast_grg_from_samples), beside the tree-direct graph of the same genotype matrix:
     `haplotypes` x `sites` from `trees` seeded coalescent segment trees, carriers taken from the
     tree-direct graph, the constructed graph saved and reloaded (simplified on write, as its
     .grg file holds it).  Per graph: shape (nodes, edges, levels, fan-in histogram), plain f32
     ms at K = 1, 16, 256 with the default launch shapes and autotuned, and the K ladder
-    (511 stacked RHS) in one traversal with the level kernels' live HIP-event time.  The int32
-    products of the two graphs must be bit-identical (one genotype matrix)."""
+    (511 stacked RHS) in one traversal with the level kernels' live HIP-event time.  Checked
+    after timing: the int32 products of the two graphs are bit-identical (one genotype matrix),
+    and the constructed graph's f32 K = 1 and 16 results are within 1e-6 normwise of the FP64
+    oracle.  Default size: the 100,000 x 500,000 (20 trees) workload."""
     import tempfile
 
     import numpy as np
@@ -472,12 +474,18 @@ def constructed_graph_benchmark(haplotypes=20_000, sites=100_000, trees=20, auto
     import grgl_amd
     from grgl_amd import capi
 
+    import oracle
+    from oracle import fp_bounds
+
+    t = time.perf_counter()
     src = grgl_amd.grg_from_coalescent(haplotypes, sites, trees, seed=SEED)
     off, car = src.mutation_carriers()
+    log(f"constructed graph: {len(car)} carrier entries in {time.perf_counter() - t:.1f} s; building ...")
     t = time.perf_counter()
     built, st = grgl_amd.fast_grg_from_samples(haplotypes, 2, carrier_offsets=off, carriers=car,
                                                positions=src.arrays()["positions"], return_stats=True)
     build_s = time.perf_counter() - t
+    log(f"constructed graph: built in {build_s:.1f} s ({st.trees} trees)")
     with tempfile.TemporaryDirectory() as tmp:
         fn = os.path.join(tmp, "constructed.grg")
         grgl_amd.save_grg(built, fn)
@@ -494,8 +502,11 @@ def constructed_graph_benchmark(haplotypes=20_000, sites=100_000, trees=20, auto
     gen = torch.Generator(device="cuda")
     gen.manual_seed(SEED + 3)
     checks = {}
+    f32_checks = []
     for name, g in (("tree_direct", src), ("constructed", con)):
         a = g.arrays()
+        if name == "constructed":
+            con_arrays = a
         cg = capi.Graph(a)
         info = cg.info()
         V, E, N, M = info["num_nodes"], info["num_edges"], info["num_samples"], info["num_mutations"]
@@ -529,6 +540,8 @@ def constructed_graph_benchmark(haplotypes=20_000, sites=100_000, trees=20, auto
                 r["edge_rhs_per_s"] = E * K / (ms / 1e3)
                 r["alg_GBps"] = round(algorithmic_bytes(V, E, K, 4, cin, cout) / (ms / 1e3) / 1e9, 1)
                 rec[f"plain_f32_{dn}_K{K}"] = r
+                if name == "constructed" and K in (1, 16):
+                    f32_checks.append((dn, K, x.cpu().numpy(), y.cpu().numpy()))
             # The K ladder stacked in one traversal, level kernels timed with HIP events.
             Ktot = sum(K_LADDER)
             x = torch.randn((Ktot, cin), device="cuda", generator=gen)
@@ -568,6 +581,13 @@ def constructed_graph_benchmark(haplotypes=20_000, sites=100_000, trees=20, auto
         out[name] = rec
         log(f"{name}: {json.dumps(rec)}")
     out["int32_products_identical"] = bool(all(np.array_equal(v[0], v[1]) for v in checks.values()))
+    og = oracle.OracleGraph(con_arrays)
+    errs = []
+    for dn, K, x, y in f32_checks:
+        rep = fp_bounds.fp32_report(og, x, oracle.UP if dn == "UP" else oracle.DOWN, y, with_reference=False)
+        errs.append({"direction": dn, "K": K, "normwise": float("%.4g" % rep["normwise"]),
+                     "bound_ratio": round(rep["bound_ratio"], 4)})
+    out["constructed_fp32_error"] = errs
     return out
 
 
@@ -786,7 +806,10 @@ def main():
                          "DOWN, library default tuning), 20 calls per shape with 0.3 s between shapes, so a "
                          "kernel trace splits by shape")
     ap.add_argument("--constructed-only", action="store_true",
-                    help="only measure the constructed-graph shape beside the tree-direct graph (20,000 x 100,000)")
+                    help="only measure the constructed-graph shape beside the tree-direct graph")
+    ap.add_argument("--constructed-size", default="100000,500000,20",
+                    help="haplotypes,sites,trees of the constructed-graph comparison (default: 100,000 x 500,000, "
+                         "20 trees)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -804,7 +827,8 @@ def main():
         latency_trace(args)
         return
     if args.constructed_only:
-        print(json.dumps(constructed_graph_benchmark(autotune=not args.no_autotune)), flush=True)
+        n, m, t = (int(v) for v in args.constructed_size.split(","))
+        print(json.dumps(constructed_graph_benchmark(n, m, t, autotune=not args.no_autotune)), flush=True)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -980,7 +1004,8 @@ def main():
         extras = {"options_K16_f32": options_benchmark(cg, N, M, V, E), "host_path_f32": host_path_benchmark(g, E),
                   "grg_io": grg_io_benchmark(g), "construction": construction_benchmark()}
         log("constructed-graph shape ...")
-        extras["constructed_graph"] = constructed_graph_benchmark(autotune=not args.no_autotune)
+        n, m, t = (int(v) for v in args.constructed_size.split(","))
+        extras["constructed_graph"] = constructed_graph_benchmark(n, m, t, autotune=not args.no_autotune)
         log(f"extras: {json.dumps(extras)}")
     cpu = None
     if not args.no_cpu_baseline:

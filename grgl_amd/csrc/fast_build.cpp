@@ -50,8 +50,6 @@ struct SegHash {
 // first appearance with 0 the empty segment (createWindow, fast_build.cpp:136-182;
 // HaplotypeWindow::addOrGetHaplotype, hap_segment.h:70-85).
 struct Window {
-    uint32_t firstVariant = 0; // index into the tree's variant list
-    uint32_t numVariants = 0;
     std::vector<Seg> segs;
     std::vector<uint32_t> bits;
     std::unordered_map<Seg, uint32_t, SegHash> ids;
@@ -67,12 +65,28 @@ struct Window {
     }
 };
 
-// Rows of one tree, cut from the input by the tree-splitting rule, with their windows.
+// Carrier lists as CSR: row r's sorted, distinct samples are samples[off[r] .. off[r+1]).
+struct CarrierRows {
+    const uint64_t* off;
+    const uint32_t* samples;
+    size_t rows;
+    size_t size(size_t r) const { return (size_t)(off[r + 1] - off[r]); }
+    const uint32_t* begin(size_t r) const { return samples + off[r]; }
+};
+
+// One window of tree rows and, once built, its segments and every sample's segment ID.
+struct WindowJob {
+    const uint32_t* rows = nullptr; // row indices, in order
+    uint32_t count = 0;
+    Window win;
+    std::vector<uint32_t> ids; // by sample
+};
+
+// Rows of one tree (a contiguous row range), with its windows.
 struct TreeInput {
     std::vector<uint32_t> variants;   // tree-mapped rows, in window bit order
     std::vector<uint32_t> directRows; // rows mapped directly
-    std::vector<Window> windows;
-    std::vector<std::vector<uint32_t>> sampleSegs; // per sample: segment ID per window
+    std::vector<WindowJob*> windows;
 };
 
 // Per-tree graph, built in node-ID order (every node's children are known when it is made).
@@ -317,87 +331,157 @@ private:
     bool truncated_ = false;
 };
 
-// Close the current window: every sample's segment gets its ID (zero first, then in sample
-// order of first appearance), and the bit words are cleared for the next window.
-void closeWindow(TreeInput& t, std::vector<Seg>& cur, uint32_t first, uint32_t count) {
-    Window w;
-    w.firstVariant = first;
-    w.numVariants = count;
-    for (size_t s = 0; s < cur.size(); s++) {
-        const uint32_t id = w.addOrGet(cur[s], popcount(cur[s]));
-        t.sampleSegs[s].push_back(id);
+// Build one window: every sample's segment gets its ID (zero first, then in sample order of
+// first appearance).  `cur` is the caller's all-zero scratch of N segments, left all zero.
+void buildWindow(uint32_t N, const CarrierRows& car, WindowJob& job, std::vector<Seg>& cur) {
+    for (uint32_t j = 0; j < job.count; j++) {
+        const uint32_t r = job.rows[j];
+        const uint32_t* S = car.begin(r);
+        const size_t n = car.size(r);
+        for (size_t i = 0; i < n; i++) cur[S[i]][j >> 6] |= 1ULL << (j & 63);
+    }
+    job.ids.resize(N);
+    for (uint32_t s = 0; s < N; s++) {
+        job.ids[s] = job.win.addOrGet(cur[s], popcount(cur[s]));
         cur[s] = Seg{};
     }
-    t.windows.push_back(std::move(w));
+}
+
+// Runs fn(i) for i in [0, n) on up to `threads` threads.
+template <typename Fn> void parallelFor(size_t n, uint32_t threads, Fn&& fn) {
+    std::atomic<size_t> next{0};
+    std::vector<std::string> errors(threads);
+    auto worker = [&](uint32_t t) {
+        try {
+            for (size_t i = next++; i < n; i = next++) fn(i, t);
+        } catch (const std::exception& e) {
+            errors[t] = e.what();
+            next = n;
+        }
+    };
+    if (threads <= 1 || n <= 1) {
+        worker(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (uint32_t t = 1; t < threads; t++) pool.emplace_back(worker, t);
+        worker(0);
+        for (auto& th : pool) th.join();
+    }
+    for (const std::string& e : errors)
+        if (!e.empty()) throw std::runtime_error(e);
 }
 
 // Cut the rows into trees and windows (getHapSegments, fast_build.cpp:266-342, and the
-// driver's loops, :857-947).  Automatic mode ends a tree after the window that brings the
-// summed unique-segment count to the target; with a tree count, trees take ceil(rows/count)
-// rows each (direct-mapped rows included) and the last one takes the rest.
-std::vector<TreeInput> cutTrees(uint32_t N, const std::vector<std::vector<uint32_t>>& carriers,
-                                const FastBuildParams& p) {
-    const size_t M = carriers.size();
+// driver's loops, :857-947).  Rows carried by fewer samples than the threshold are mapped
+// directly; the others are the tree rows, in windows of hapLength.
+//  * With a tree count, trees take ceil(rows/count) rows each (direct rows included) and the
+//    last one takes the rest; each tree's windows start at its first tree row.
+//  * Automatic: a tree ends after the window that brings its summed unique-segment count to
+//    the target, so every tree but the last holds whole windows and the windows are the
+//    global 254-row blocks of tree rows; rows after a tree's last window start the next tree
+//    (a tree of direct rows only, when no tree row follows).
+// Windows are built in parallel (each depends only on its rows); trees then own their windows.
+std::vector<TreeInput> cutTrees(uint32_t N, const CarrierRows& car, const FastBuildParams& p, uint32_t threads,
+                                std::vector<WindowJob>& jobs, std::vector<uint32_t>& treeRowsAll) {
+    const size_t M = car.rows;
     const uint32_t L = p.hapLength;
     const size_t directBelow = p.noTreeBelow < 1.0 ? (size_t)((double)N * p.noTreeBelow) : (size_t)p.noTreeBelow;
     const bool byCount = p.treeCount > 0;
-    // getOptimalHapSegSum, fast_build.cpp:736-752 (a regression of the summed count on ln N).
-    const double predicted = 2383.46 * std::log((double)N) - 18143.49;
-    const size_t target = (size_t)std::min(20000.0, std::max(1000.0, predicted * p.hapSumMultiplier));
-    const size_t perTree = byCount ? (M + p.treeCount - 1) / p.treeCount : 0;
-
-    std::vector<TreeInput> trees;
-    std::vector<Seg> cur(N);
-    size_t row = 0;
-    while (row < M) {
-        TreeInput t;
-        t.sampleSegs.assign(N, {});
-        const bool last = byCount && trees.size() + 1 == p.treeCount;
-        size_t soFar = 0;
-        uint32_t windex = 0, first = 0;
-        bool done = false;
-        while (!done && row < M) {
-            const std::vector<uint32_t>& S = carriers[row];
-            if (byCount && ++soFar == perTree && !last) done = true;
-            if (S.size() < directBelow) {
-                t.directRows.push_back((uint32_t)row++);
-                continue;
-            }
-            for (uint32_t s : S) cur[s][windex >> 6] |= 1ULL << (windex & 63);
-            t.variants.push_back((uint32_t)row++);
-            if (++windex == L) {
-                closeWindow(t, cur, first, windex);
-                first = (uint32_t)t.variants.size();
-                windex = 0;
-                if (!byCount) {
-                    soFar += t.windows.back().segs.size();
-                    if (soFar >= target) done = true;
-                }
+    std::vector<uint8_t> direct(M);
+    treeRowsAll.clear();
+    for (size_t r = 0; r < M; r++) {
+        direct[r] = car.size(r) < directBelow;
+        if (!direct[r]) treeRowsAll.push_back((uint32_t)r);
+    }
+    // Row ranges [b, e) of the trees and, per tree, the ranges of treeRowsAll in its windows.
+    std::vector<std::pair<size_t, size_t>> ranges;
+    jobs.clear();
+    auto addWindows = [&](size_t tb, size_t te) { // windows over treeRowsAll[tb, te)
+        for (size_t w = tb; w < te; w += L) {
+            WindowJob j;
+            j.rows = treeRowsAll.data() + w;
+            j.count = (uint32_t)std::min<size_t>(L, te - w);
+            jobs.push_back(std::move(j));
+        }
+    };
+    if (byCount) {
+        const size_t perTree = (M + p.treeCount - 1) / p.treeCount;
+        size_t tr = 0;
+        for (size_t t = 0, b = 0; b < M; t++) {
+            const size_t e = t + 1 == p.treeCount ? M : std::min(M, b + perTree);
+            const size_t tb = tr;
+            while (tr < treeRowsAll.size() && treeRowsAll[tr] < e) tr++;
+            addWindows(tb, tr);
+            ranges.emplace_back(b, e);
+            b = e;
+        }
+    } else {
+        addWindows(0, treeRowsAll.size());
+    }
+    std::vector<std::vector<Seg>> cur(threads, std::vector<Seg>());
+    parallelFor(jobs.size(), threads, [&](size_t i, uint32_t t) {
+        if (cur[t].size() != N) cur[t].assign(N, Seg{});
+        buildWindow(N, car, jobs[i], cur[t]);
+    });
+    if (!byCount) {
+        // getOptimalHapSegSum, fast_build.cpp:736-752 (a regression of the summed count on ln N).
+        const double predicted = 2383.46 * std::log((double)N) - 18143.49;
+        const size_t target = (size_t)std::min(20000.0, std::max(1000.0, predicted * p.hapSumMultiplier));
+        size_t b = 0, sum = 0;
+        for (size_t w = 0; w < jobs.size(); w++) {
+            sum += jobs[w].win.segs.size();
+            if (sum >= target && jobs[w].count == L) {
+                const size_t e = (size_t)jobs[w].rows[L - 1] + 1;
+                ranges.emplace_back(b, e);
+                b = e;
+                sum = 0;
             }
         }
-        if (windex > 0) closeWindow(t, cur, first, windex);
-        trees.push_back(std::move(t));
+        if (b < M) ranges.emplace_back(b, M);
+    }
+    std::vector<TreeInput> trees(ranges.size());
+    size_t w = 0;
+    for (size_t t = 0; t < ranges.size(); t++) {
+        TreeInput& in = trees[t];
+        for (size_t r = ranges[t].first; r < ranges[t].second; r++) {
+            if (direct[r])
+                in.directRows.push_back((uint32_t)r);
+            else
+                in.variants.push_back((uint32_t)r);
+        }
+        size_t covered = 0;
+        while (covered < in.variants.size()) {
+            in.windows.push_back(&jobs[w]);
+            covered += jobs[w++].count;
+        }
     }
     return trees;
 }
 
 // One tree: cluster the nodes bottom-up, then map the tree's variants onto the roots and the
 // direct rows onto new nodes (buildTree, fast_build.cpp:430-706; directMap, :710-727).
-TreeGraph buildTree(uint32_t N, uint32_t ploidy, TreeInput& in, const std::vector<std::vector<uint32_t>>& carriers,
-                    const FastBuildParams& p) {
+TreeGraph buildTree(uint32_t N, uint32_t ploidy, TreeInput& in, const CarrierRows& car, const FastBuildParams& p) {
     TreeGraph g;
     g.off.assign((size_t)N + 1, 0);
     g.numParents.assign(N, 0);
     g.stats.windows = in.windows.size();
-    for (const Window& w : in.windows) g.stats.uniqueHapSegments += w.segs.size();
-    Coals coals{ploidy == kPloidyCoal, N, {}, {}};
     const size_t W = in.windows.size();
+    std::vector<Window*> wins(W);
+    std::vector<uint32_t> firstVariant(W, 0);
+    for (size_t w = 0; w < W; w++) {
+        wins[w] = &in.windows[w]->win;
+        g.stats.uniqueHapSegments += wins[w]->segs.size();
+        if (w + 1 < W) firstVariant[w + 1] = firstVariant[w] + in.windows[w]->count;
+    }
+    Coals coals{ploidy == kPloidyCoal, N, {}, {}};
 
-    // Segment IDs of every node, W per node.
-    std::vector<uint32_t> segOf;
-    segOf.reserve((size_t)N * W * 2);
-    for (uint32_t s = 0; s < N; s++) segOf.insert(segOf.end(), in.sampleSegs[s].begin(), in.sampleSegs[s].end());
-    std::vector<std::vector<uint32_t>>().swap(in.sampleSegs);
+    // Segment IDs of every node, W per node (the windows' per-sample IDs are freed here).
+    std::vector<uint32_t> segOf((size_t)N * W);
+    for (size_t w = 0; w < W; w++) {
+        const std::vector<uint32_t>& ids = in.windows[w]->ids;
+        for (uint32_t s = 0; s < N; s++) segOf[(size_t)s * W + w] = ids[s];
+        std::vector<uint32_t>().swap(in.windows[w]->ids);
+    }
 
     if (W > 0) {
         auto dist = [&](uint32_t a, uint32_t b) -> uint32_t {
@@ -406,8 +490,8 @@ TreeGraph buildTree(uint32_t N, uint32_t ploidy, TreeInput& in, const std::vecto
             uint32_t d = 0;
             for (size_t w = 0; w < W; w++) {
                 if (x[w] == y[w]) continue;
-                const Seg& u = in.windows[w].segs[x[w]];
-                const Seg& v = in.windows[w].segs[y[w]];
+                const Seg& u = wins[w]->segs[x[w]];
+                const Seg& v = wins[w]->segs[y[w]];
                 d += (uint32_t)(__builtin_popcountll(u[0] ^ v[0]) + __builtin_popcountll(u[1] ^ v[1]) +
                                 __builtin_popcountll(u[2] ^ v[2]) + __builtin_popcountll(u[3] ^ v[3]));
             }
@@ -439,7 +523,7 @@ TreeGraph buildTree(uint32_t N, uint32_t ploidy, TreeInput& in, const std::vecto
                 const uint32_t other = similar.front();
                 uint64_t isize = 0, size1 = 0, size2 = 0;
                 for (size_t w = 0; w < W; w++) {
-                    Window& win = in.windows[w];
+                    Window& win = *wins[w];
                     const uint32_t h1 = segOf[(size_t)node * W + w], h2 = segOf[(size_t)other * W + w];
                     if (h1 == h2) {
                         inter[w] = h1;
@@ -531,14 +615,13 @@ TreeGraph buildTree(uint32_t N, uint32_t ploidy, TreeInput& in, const std::vecto
         for (uint32_t r = 0; r < built; r++) {
             if (g.numParents[r] != 0) continue;
             for (size_t w = 0; w < W; w++) {
-                const Window& win = in.windows[w];
-                const Seg& s = win.segs[segOf[(size_t)r * W + w]];
+                const Seg& s = wins[w]->segs[segOf[(size_t)r * W + w]];
                 for (uint32_t k = 0; k < 4; k++) {
                     uint64_t bits = s[k];
                     while (bits) {
                         const uint32_t j = k * 64 + (uint32_t)__builtin_ctzll(bits);
                         bits &= bits - 1;
-                        onRoots[win.firstVariant + j].push_back(r);
+                        onRoots[firstVariant[w] + j].push_back(r);
                     }
                 }
             }
@@ -565,12 +648,12 @@ TreeGraph buildTree(uint32_t N, uint32_t ploidy, TreeInput& in, const std::vecto
         }
         // Rows below the tree threshold: one node, an edge per carrier.
         for (size_t k = 0; k < in.directRows.size(); k++) {
-            const std::vector<uint32_t>& S = carriers[in.directRows[k]];
-            const uint32_t mn = g.addNode(S.data(), S.size());
+            const uint32_t r = in.directRows[k];
+            const uint32_t mn = g.addNode(car.begin(r), car.size(r));
             g.mutNode[V + k] = mn;
             g.stats.directMapped++;
             if (coals.on) {
-                coals.gather(S.data(), S.size());
+                coals.gather(car.begin(r), car.size(r));
                 g.coal.resize(g.numNodes(), kCoalNotSet);
                 g.coal[mn] = coals.pairUp(unco);
             }
@@ -670,55 +753,57 @@ private:
 } // namespace
 
 MutableGraph fastBuildFromSamples(uint32_t numSamples, uint32_t ploidy, const std::vector<MutationRecordIn>& rows,
-                                  const std::vector<std::vector<uint32_t>>& carriersIn, const FastBuildParams& params,
-                                  FastBuildStats& stats, uint32_t threads) {
-    if (rows.size() != carriersIn.size()) misuse("mutations and samples must be the same length");
+                                  const uint64_t* carrierOff, const uint32_t* carrierSamples,
+                                  const FastBuildParams& params, FastBuildStats& stats, uint32_t threads) {
     if (numSamples == 0) misuse("num_samples must be positive");
     if (params.hapLength == 0 || params.hapLength > kMaxHapLength) misuse("hap_length must be in 1..254");
     if (!(params.rebuildProportion > 0.0)) misuse("rebuild_proportion must be positive");
     if (!(params.hapSumMultiplier > 0.0)) misuse("the segment-sum multiplier must be positive");
     if (params.noTreeBelow < 0.0) misuse("lf_no_tree must not be negative");
-    MutableGraph out(numSamples, ploidy, true);
-    std::vector<std::vector<uint32_t>> carriers(carriersIn.size());
-    for (size_t i = 0; i < rows.size(); i++) {
-        if (rows[i].allele == ".") misuse("missing-data rows are not supported by the graph constructor");
-        carriers[i] = carriersIn[i];
-        std::sort(carriers[i].begin(), carriers[i].end());
-        carriers[i].erase(std::unique(carriers[i].begin(), carriers[i].end()), carriers[i].end());
-        if (!carriers[i].empty() && carriers[i].back() >= numSamples) misuse("sample ID out of range");
+    const size_t M = rows.size();
+    if (carrierOff[0] != 0) misuse("carrier offsets must start at 0");
+    // Rows must be sorted and distinct; a row that is not gets a normalised copy.
+    std::vector<uint64_t> normOff;
+    std::vector<uint32_t> normSamples;
+    bool clean = true;
+    for (size_t i = 0; i < M && clean; i++) {
+        if (carrierOff[i + 1] < carrierOff[i]) misuse("carrier offsets must be non-decreasing");
+        for (uint64_t e = carrierOff[i] + 1; e < carrierOff[i + 1]; e++)
+            if (carrierSamples[e] <= carrierSamples[e - 1]) clean = false;
     }
+    if (!clean) {
+        normOff.assign(1, 0);
+        for (size_t i = 0; i < M; i++) {
+            const size_t b = normSamples.size();
+            normSamples.insert(normSamples.end(), carrierSamples + carrierOff[i], carrierSamples + carrierOff[i + 1]);
+            std::sort(normSamples.begin() + (ptrdiff_t)b, normSamples.end());
+            normSamples.erase(std::unique(normSamples.begin() + (ptrdiff_t)b, normSamples.end()), normSamples.end());
+            normOff.push_back(normSamples.size());
+        }
+        carrierOff = normOff.data();
+        carrierSamples = normSamples.data();
+    }
+    const CarrierRows car{carrierOff, carrierSamples, M};
+    for (size_t i = 0; i < M; i++) {
+        if (rows[i].allele == ".") misuse("missing-data rows are not supported by the graph constructor");
+        if (car.size(i) && car.begin(i)[car.size(i) - 1] >= numSamples) misuse("sample ID out of range");
+    }
+    MutableGraph out(numSamples, ploidy, true);
     stats = FastBuildStats{};
+    if (threads == 0) threads = std::max(1U, std::thread::hardware_concurrency());
 
-    std::vector<TreeInput> inputs = cutTrees(numSamples, carriers, params);
+    std::vector<WindowJob> jobs;
+    std::vector<uint32_t> treeRowsAll;
+    std::vector<TreeInput> inputs = cutTrees(numSamples, car, params, threads, jobs, treeRowsAll);
     const size_t T = inputs.size();
     std::vector<HostGrg> trees(T);
     std::vector<FastBuildStats> treeStats(T);
-    std::vector<std::string> errors(T);
-    if (threads == 0) threads = std::max(1U, std::thread::hardware_concurrency());
-    threads = (uint32_t)std::min<size_t>(threads, std::max<size_t>(T, 1));
-    std::atomic<size_t> nextTree{0};
-    auto worker = [&]() {
-        for (size_t t = nextTree++; t < T; t = nextTree++) {
-            try {
-                TreeGraph g = buildTree(numSamples, ploidy, inputs[t], carriers, params);
-                treeStats[t] = g.stats;
-                trees[t] = simplifiedTree(numSamples, ploidy, g);
-                TreeInput().sampleSegs.swap(inputs[t].sampleSegs);
-                std::vector<Window>().swap(inputs[t].windows);
-            } catch (const std::exception& e) {
-                errors[t] = e.what();
-            }
-        }
-    };
-    if (threads <= 1) {
-        worker();
-    } else {
-        std::vector<std::thread> pool;
-        for (uint32_t i = 0; i < threads; i++) pool.emplace_back(worker);
-        for (auto& th : pool) th.join();
-    }
-    for (const std::string& e : errors)
-        if (!e.empty()) throw std::runtime_error(e);
+    parallelFor(T, threads, [&](size_t t, uint32_t) {
+        TreeGraph g = buildTree(numSamples, ploidy, inputs[t], car, params);
+        treeStats[t] = g.stats;
+        trees[t] = simplifiedTree(numSamples, ploidy, g);
+        for (WindowJob* j : inputs[t].windows) j->win = Window();
+    });
 
     Merger merged(numSamples);
     std::vector<uint32_t> nodeOfRow(rows.size(), kInvalidNode);
@@ -757,6 +842,19 @@ MutableGraph fastBuildFromSamples(uint32_t numSamples, uint32_t ploidy, const st
         for (size_t i = 0; i < rows.size(); i++) out.addMutation(rows[i], nodeOfRow[i]);
     }
     return out;
+}
+
+MutableGraph fastBuildFromSamples(uint32_t numSamples, uint32_t ploidy, const std::vector<MutationRecordIn>& rows,
+                                  const std::vector<std::vector<uint32_t>>& carriers, const FastBuildParams& params,
+                                  FastBuildStats& stats, uint32_t threads) {
+    if (rows.size() != carriers.size()) misuse("mutations and samples must be the same length");
+    std::vector<uint64_t> off(1, 0);
+    std::vector<uint32_t> flat;
+    for (const auto& c : carriers) {
+        flat.insert(flat.end(), c.begin(), c.end());
+        off.push_back(flat.size());
+    }
+    return fastBuildFromSamples(numSamples, ploidy, rows, off.data(), flat.data(), params, stats, threads);
 }
 
 } // namespace grgl_amd

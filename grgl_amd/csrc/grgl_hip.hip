@@ -615,16 +615,19 @@ Tuning chooseTuning(grgl_graph* g, grgl_direction dir, TypeClass tc, uint32_t K,
     // then idle lanes, put fewer rows in each wavefront, so a level's rows reach more
     // compute units and more of them are in flight at once (latency-bound small K: on the
     // 200k x 1M graph FP32 DOWN K = 1 goes from 0.35 to 0.22 ms, K = 8 from 0.36 to 0.25 ms).
-    // Rows of 256 B - 1 KB (FP32 K = 64..256) take 8 lanes too: over six graphs (tree-direct
-    // 20k x 100k with 1,000 trees, 100k x 500k, 200k x 1M; haplotype-clustered 20k x 100k,
-    // 40k x 200k, 20k x 100k from 1,000 trees) 8 lanes are within 11% of the best team width
-    // for every direction and K, where 16 lanes lost up to 45% (UP K = 64 on the 1,000-tree
-    // graph, 0.66 vs 0.46 ms) and gained at most 11% (DOWN K = 256, 20k x 100k clustered).
+    // Rows of 256 B - 1 KB (FP32 K = 64..256) take 8 lanes UP and 16 DOWN.  Over seven graphs
+    // (tree-direct 20k x 100k with 1,000 trees, 100k x 500k, 200k x 1M; haplotype-clustered
+    // 20k x 100k, 40k x 200k, 100k x 500k, 20k x 100k from 1,000 trees) that keeps every
+    // K = 64..256 within 10% of the best team width.  16 lanes UP lost up to 45% (K = 64 on
+    // the 1,000-tree graph, 0.66 vs 0.46 ms); 8 lanes DOWN lost up to 17% (K = 256 on the
+    // clustered 100k x 500k graph, 2.20 vs 1.87 ms).
+    // 16-byte rows (FP32 K = 4) take 4 lanes UP: 8 lanes lost up to 11% on the same graphs,
+    // 4 lanes at most 5%; DOWN keeps 8 (4 lanes lost 11% on the 200k x 1M graph).
     if (tw == 0) {
         if (geo.lanes < 3)
-            tw = geo.rowBytes <= 8 ? 4 : 8;
+            tw = (geo.rowBytes <= 8 || (dir == GRGL_DIRECTION_UP && geo.rowBytes <= 16)) ? 4 : 8;
         else if (geo.rowBytes >= 256 && geo.rowBytes <= 1024)
-            tw = 8;
+            tw = dir == GRGL_DIRECTION_UP ? 8 : 16;
         else
             tw = std::min<uint32_t>(16, pow2ceil(geo.lanes));
     }

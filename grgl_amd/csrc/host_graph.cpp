@@ -58,20 +58,22 @@ GraphStats computeGraphStats(const HostGrg& g) {
     return st;
 }
 
-void mutationCarriers(const HostGrg& g, std::vector<uint64_t>& off, std::vector<uint32_t>& samples,
-                      uint32_t threads) {
+CarrierParts mutationCarriers(const HostGrg& g, uint32_t threads) {
     validateHostGrg(g);
     const uint32_t M = g.numMutations(), V = g.numNodes(), N = g.numSamples;
     if (threads == 0) threads = std::max(1U, std::thread::hardware_concurrency());
     threads = std::max(1U, std::min<uint32_t>(threads, (M + 255) / 256));
-    std::vector<std::vector<uint32_t>> part(threads);
+    CarrierParts out;
+    out.parts.resize(threads);
     std::vector<std::vector<uint64_t>> counts(threads);
     auto work = [&](uint32_t t) {
         const uint32_t b = (uint32_t)((uint64_t)M * t / threads), e = (uint32_t)((uint64_t)M * (t + 1) / threads);
+        std::vector<uint32_t>& part = out.parts[t];
         std::vector<uint32_t> stamp(V, 0), stack;
+        std::vector<uint64_t> bits(((size_t)N + 63) / 64, 0); // sorts long lists by a bit scan
         uint32_t epoch = 0;
         for (uint32_t m = b; m < e; m++) {
-            const size_t before = part[t].size();
+            const size_t before = part.size();
             const uint32_t root = g.mutNode[m];
             if (root != kInvalidNode) {
                 epoch++;
@@ -81,7 +83,7 @@ void mutationCarriers(const HostGrg& g, std::vector<uint64_t>& off, std::vector<
                     const uint32_t n = stack.back();
                     stack.pop_back();
                     if (n < N) {
-                        part[t].push_back(n);
+                        part.push_back(n);
                         continue;
                     }
                     for (uint64_t i = g.downOff[n]; i < g.downOff[n + 1]; i++) {
@@ -92,21 +94,34 @@ void mutationCarriers(const HostGrg& g, std::vector<uint64_t>& off, std::vector<
                         }
                     }
                 }
-                std::sort(part[t].begin() + (ptrdiff_t)before, part[t].end());
+                const size_t n = part.size() - before;
+                if (n <= 64) {
+                    std::sort(part.begin() + (ptrdiff_t)before, part.end());
+                } else {
+                    uint32_t lo = N, hi = 0;
+                    for (size_t i = before; i < part.size(); i++) {
+                        bits[part[i] >> 6] |= 1ULL << (part[i] & 63);
+                        lo = std::min(lo, part[i]);
+                        hi = std::max(hi, part[i]);
+                    }
+                    size_t o = before;
+                    for (uint32_t w = lo >> 6; w <= (hi >> 6); w++) {
+                        for (uint64_t x = bits[w]; x; x &= x - 1) part[o++] = w * 64 + (uint32_t)__builtin_ctzll(x);
+                        bits[w] = 0;
+                    }
+                }
             }
-            counts[t].push_back(part[t].size() - before);
+            counts[t].push_back(part.size() - before);
         }
     };
     std::vector<std::thread> pool;
     for (uint32_t t = 1; t < threads; t++) pool.emplace_back(work, t);
     work(0);
     for (auto& th : pool) th.join();
-    off.assign(1, 0);
-    samples.clear();
-    for (uint32_t t = 0; t < threads; t++) {
-        for (uint64_t c : counts[t]) off.push_back(off.back() + c);
-        samples.insert(samples.end(), part[t].begin(), part[t].end());
-    }
+    out.off.assign(1, 0);
+    for (uint32_t t = 0; t < threads; t++)
+        for (uint64_t c : counts[t]) out.off.push_back(out.off.back() + c);
+    return out;
 }
 
 } // namespace grgl_amd

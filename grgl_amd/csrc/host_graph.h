@@ -70,9 +70,14 @@ struct GraphStats {
 GraphStats computeGraphStats(const HostGrg& g);
 
 // The samples below each mutation's node (its carriers), sorted, as CSR: off has one entry per
-// mutation plus one.  Samples reached by several paths are listed once.
-void mutationCarriers(const HostGrg& g, std::vector<uint64_t>& off, std::vector<uint32_t>& samples,
-                      uint32_t threads = 0);
+// mutation plus one.  Samples reached by several paths are listed once.  The samples array
+// comes in consecutive parts (one per worker thread), so a caller can move them into its own
+// buffer one at a time.
+struct CarrierParts {
+    std::vector<uint64_t> off;
+    std::vector<std::vector<uint32_t>> parts;
+};
+CarrierParts mutationCarriers(const HostGrg& g, uint32_t threads = 0);
 
 // .grg files (format v5.x of the reference, src/serialize.cpp:518-669 and :697-890).
 // Reading accepts versions 5.0-5.3 (u32 node index up to 5.1, u64 after); writing produces

@@ -562,13 +562,19 @@ PYBIND11_MODULE(_grgl_amd, m) {
              "The graph as plain arrays (down-edge CSR, mutation map, coalescence counts).")
         .def("mutation_carriers",
              [](PyGRG& g) {
-                 std::vector<uint64_t> off;
-                 std::vector<uint32_t> samples;
+                 CarrierParts cp;
                  {
                      py::gil_scoped_release release;
-                     mutationCarriers(g.H(), off, samples);
+                     cp = mutationCarriers(g.H());
                  }
-                 return py::make_tuple(toArray(off), toArray(samples));
+                 py::array_t<uint32_t> samples((py::ssize_t)cp.off.back());
+                 uint32_t* dst = samples.mutable_data();
+                 for (auto& part : cp.parts) { // one part at a time: the peak stays near one copy
+                     if (!part.empty()) std::memcpy(dst, part.data(), part.size() * sizeof(uint32_t));
+                     dst += part.size();
+                     std::vector<uint32_t>().swap(part);
+                 }
+                 return py::make_tuple(toArray(cp.off), samples);
              },
              "The samples carrying each mutation as CSR arrays (offsets of length num_mutations+1, sorted "
              "sample IDs): the sparse genotype matrix the graph encodes.")
@@ -735,44 +741,54 @@ PYBIND11_MODULE(_grgl_amd, m) {
                 API_CHECK(n > 0, "Must provide non-zero trees argument");
                 p.treeCount = (uint32_t)n;
             }
-            std::vector<std::vector<uint32_t>> sets;
+            // Carrier lists as CSR (offsets checked here; rows are sorted by the builder if needed).
+            std::vector<uint64_t> off(1, 0);
+            std::vector<uint32_t> flat;
+            py::array_t<uint32_t, py::array::c_style | py::array::forcecast> carArr;
+            const uint32_t* carPtr = nullptr;
             if (!samples.is_none()) {
                 API_CHECK(carrierOffsets.is_none() && carriers.is_none(), "give samples or carrier arrays, not both");
-                sets = samples.cast<std::vector<std::vector<uint32_t>>>();
+                for (const auto& c : samples.cast<std::vector<std::vector<uint32_t>>>()) {
+                    flat.insert(flat.end(), c.begin(), c.end());
+                    off.push_back(flat.size());
+                }
+                carPtr = flat.data();
             } else {
                 API_CHECK(!carrierOffsets.is_none() && !carriers.is_none(),
                           "give samples (a list per mutation) or carrier_offsets and carriers");
-                auto off = carrierOffsets.cast<py::array_t<int64_t, py::array::c_style | py::array::forcecast>>();
-                auto car = carriers.cast<py::array_t<uint32_t, py::array::c_style | py::array::forcecast>>();
-                API_CHECK(off.ndim() == 1 && off.size() >= 1, "carrier_offsets must be a 1-D array of M+1 offsets");
-                const int64_t* o = off.data();
-                const uint32_t* c = car.data();
-                sets.resize((size_t)off.size() - 1);
-                for (size_t i = 0; i + 1 < (size_t)off.size(); i++) {
-                    API_CHECK(o[i] >= 0 && o[i] <= o[i + 1] && o[i + 1] <= (int64_t)car.size(), "bad carrier_offsets");
-                    sets[i].assign(c + o[i], c + o[i + 1]);
+                auto offArr = carrierOffsets.cast<py::array_t<int64_t, py::array::c_style | py::array::forcecast>>();
+                carArr = carriers.cast<py::array_t<uint32_t, py::array::c_style | py::array::forcecast>>();
+                API_CHECK(offArr.ndim() == 1 && offArr.size() >= 1, "carrier_offsets must be a 1-D array of M+1 offsets");
+                const int64_t* o = offArr.data();
+                API_CHECK(o[0] == 0, "carrier_offsets must start at 0");
+                off.resize((size_t)offArr.size());
+                for (size_t i = 1; i < (size_t)offArr.size(); i++) {
+                    API_CHECK(o[i] >= o[i - 1] && o[i] <= (int64_t)carArr.size(), "bad carrier_offsets");
+                    off[i] = (uint64_t)o[i];
                 }
+                carPtr = carArr.data();
             }
+            const size_t M = off.size() - 1;
             std::vector<MutationRecordIn> recs;
             if (!mutations.is_none()) {
                 for (const auto& x : mutations.cast<std::vector<PyMutation>>()) recs.push_back(toRecord(x));
             } else {
                 std::vector<double> pos;
                 if (!positions.is_none()) pos = positions.cast<std::vector<double>>();
-                API_CHECK(pos.empty() || pos.size() == sets.size(), "positions must have one entry per mutation");
-                recs.resize(sets.size());
-                for (size_t i = 0; i < sets.size(); i++) {
+                API_CHECK(pos.empty() || pos.size() == M, "positions must have one entry per mutation");
+                recs.resize(M);
+                for (size_t i = 0; i < M; i++) {
                     recs[i].position = pos.empty() ? (double)i : pos[i];
                     recs[i].allele = "G";
                     recs[i].refAllele = "A";
                 }
             }
-            API_CHECK(recs.size() == sets.size(), "mutations and samples must be the same length");
+            API_CHECK(recs.size() == M, "mutations and samples must be the same length");
             auto g = std::make_shared<PyMutableGRG>(numSamples, ploidy, true);
             FastBuildStats st;
             {
                 py::gil_scoped_release release;
-                g->mg = fastBuildFromSamples(numSamples, ploidy, recs, sets, p, st, threads);
+                g->mg = fastBuildFromSamples(numSamples, ploidy, recs, off.data(), carPtr, p, st, threads);
             }
             if (returnStats) return py::object(py::make_tuple(g, st));
             return py::object(py::cast(g));

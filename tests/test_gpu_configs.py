@@ -1,5 +1,10 @@
 """The benchmarked paths at their full workload sizes, against the oracle.
 
+* The 100,000 haplotypes x 500,000 sites workload (20 segment trees) built by the
+  haplotype-clustering constructor (fast_grg_from_samples, saved and reloaded as its .grg file
+  holds it): K = 1 and 16, UP and DOWN; int32 bit-exact against the oracle and equal to the
+  tree-direct graph's products (one genotype matrix), FP64 <= 1e-12 and FP32 <= 1e-6 normwise
+  against the FP64 oracle, FP32 within the rounding bound.
 * Batched right-hand sides, 200,000 haplotypes x 1,000,000 sites (50 segment trees): the K
   ladder {1, 2, ..., 256} stacked as 511 rows in ONE traversal (grgl_hip_matmul_batch), with
   the tuning the autotuner picks for 511 rows, f32 N(0,1) and int32 Bernoulli(0.5), UP and DOWN.
@@ -51,6 +56,35 @@ def _config3():
         a = g.arrays()
         _G["c3"] = (a, capi.Graph(a), oracle.OracleGraph(a))
     return _G["c3"]
+
+
+def test_config2_constructed_graph(gpu, tmp_path):
+    import os
+
+    src = grgl_amd.grg_from_coalescent(100_000, 500_000, 20, seed=SEED)
+    off, car = src.mutation_carriers()
+    built = grgl_amd.fast_grg_from_samples(100_000, 2, carrier_offsets=off, carriers=car,
+                                           positions=src.arrays()["positions"])
+    del off, car
+    fn = os.path.join(str(tmp_path), "c2.grg")
+    grgl_amd.save_grg(built, fn)
+    del built
+    g = grgl_amd.load_immutable_grg(fn)
+    assert g.num_mutations == src.num_mutations
+    og = oracle.OracleGraph(g.arrays())
+    rng = np.random.default_rng(SEED + 11)
+    for d, cols in ((grgl_amd.UP, g.num_samples), (grgl_amd.DOWN, g.num_mutations)):
+        od = oracle.UP if d == grgl_amd.UP else oracle.DOWN
+        for K in (1, 16):
+            xi = rng.integers(-9, 10, size=(K, cols)).astype(np.int32)
+            yi = grgl_amd.matmul(g, xi, d)
+            assert bits_equal(yi, og.matmul(xi, od)), (d, K)
+            assert bits_equal(yi, grgl_amd.matmul(src, xi, d)), (d, K)
+            x64 = rng.standard_normal((K, cols))
+            assert fp_bounds.normwise_rel_err(grgl_amd.matmul(g, x64, d), og.matmul(x64, od)) <= 1e-12, (d, K)
+            x32 = x64.astype(np.float32)
+            rep = fp_bounds.fp32_report(og, x32, od, grgl_amd.matmul(g, x32, d), with_reference=False)
+            assert rep["bound_ratio"] <= 1.0 and rep["normwise"] <= 1e-6, (d, K, rep)
 
 
 @pytest.mark.parametrize("dtype", [np.int32, np.float32])
