@@ -455,7 +455,8 @@ def construction_benchmark():
                                       "edges_per_carrier": round(fb.num_edges / entries, 4)}}
 
 
-def constructed_graph_benchmark(haplotypes=100_000, sites=500_000, trees=20, autotune=True):
+def constructed_graph_benchmark(haplotypes=100_000, sites=500_000, trees=20, autotune=True,
+                                graphs=("tree_direct", "constructed")):
     """The traversal on the graph shape the haplotype-clustering constructor builds
     (fast_grg_from_samples), beside the tree-direct graph of the same genotype matrix:
     `haplotypes` x `sites` from `trees` seeded coalescent segment trees, carriers taken from the
@@ -504,6 +505,8 @@ def constructed_graph_benchmark(haplotypes=100_000, sites=500_000, trees=20, aut
     checks = {}
     f32_checks = []
     for name, g in (("tree_direct", src), ("constructed", con)):
+        if name not in graphs:
+            continue
         a = g.arrays()
         if name == "constructed":
             con_arrays = a
@@ -569,7 +572,8 @@ def constructed_graph_benchmark(haplotypes=100_000, sites=500_000, trees=20, aut
                 "ms": round(ms, 4), "edge_rhs_per_s": E * Ktot / (ms / 1e3),
                 "alg_GBps": round(algorithmic_bytes(V, E, Ktot, 4, cin, cout) / (ms / 1e3) / 1e9, 1),
                 "level_kernels_GBps": round(achieved, 1), "level_kernels_frac": round(achieved / HBM_PEAK_GBS, 4),
-                "level_launches": k_n // reps}
+                "level_launches": k_n // reps, "level_ms_per_traversal": round(k_ms / reps, 4),
+                "level_alg_bytes_per_traversal": int(k_bytes / reps)}
             # int32 products for the cross-graph check
             xi = (torch.rand((16, cin), device="cuda", generator=torch.Generator(device="cuda").manual_seed(
                 SEED + (1 if d == capi.UP else 2))) < 0.5).to(torch.int32)
@@ -580,7 +584,10 @@ def constructed_graph_benchmark(haplotypes=100_000, sites=500_000, trees=20, aut
         cg.close()
         out[name] = rec
         log(f"{name}: {json.dumps(rec)}")
-    out["int32_products_identical"] = bool(all(np.array_equal(v[0], v[1]) for v in checks.values()))
+    if len(graphs) == 2:
+        out["int32_products_identical"] = bool(all(np.array_equal(v[0], v[1]) for v in checks.values()))
+    if "constructed" not in graphs:
+        return out
     og = oracle.OracleGraph(con_arrays)
     errs = []
     for dn, K, x, y in f32_checks:
@@ -807,6 +814,8 @@ def main():
                          "kernel trace splits by shape")
     ap.add_argument("--constructed-only", action="store_true",
                     help="only measure the constructed-graph shape beside the tree-direct graph")
+    ap.add_argument("--constructed-graphs", default="tree_direct,constructed",
+                    help="which graphs the constructed-graph comparison measures (for profiling one of them)")
     ap.add_argument("--constructed-size", default="100000,500000,20",
                     help="haplotypes,sites,trees of the constructed-graph comparison (default: 100,000 x 500,000, "
                          "20 trees)")
@@ -828,7 +837,8 @@ def main():
         return
     if args.constructed_only:
         n, m, t = (int(v) for v in args.constructed_size.split(","))
-        print(json.dumps(constructed_graph_benchmark(n, m, t, autotune=not args.no_autotune)), flush=True)
+        print(json.dumps(constructed_graph_benchmark(n, m, t, autotune=not args.no_autotune,
+                                                     graphs=tuple(args.constructed_graphs.split(",")))), flush=True)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
