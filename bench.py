@@ -426,6 +426,44 @@ def grg_io_benchmark(g):
             "read_MBps": round(size / r / 1e6, 1), "round_trip_equal": bool(same)}
 
 
+def levelized_file_benchmark(arrays, cg, create_s):
+    """The benchmark graph's levelized-graph file (grgl_hip_graph_store / grgl_hip_graph_load,
+    the reference's storeGPUGRGToDisk / loadGPUGRGFromDisk): write time, size, and load time
+    beside grgl_hip_graph_create's (levelize on the host + upload); the loaded graph's int32
+    UP K = 4 result must equal the created graph's."""
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from grgl_amd import capi
+
+    out = {"graph_create_s": round(create_s, 3)}
+    with tempfile.TemporaryDirectory() as tmp:
+        fn = os.path.join(tmp, "bench.lvlz")
+        t = time.perf_counter()
+        cg.store(fn)
+        out["store_s"] = round(time.perf_counter() - t, 3)
+        out["file_bytes"] = os.path.getsize(fn)
+        t = time.perf_counter()
+        capi.levelize_to_file(arrays, fn + ".host")
+        out["levelize_to_file_s"] = round(time.perf_counter() - t, 3)
+        t = time.perf_counter()
+        loaded = capi.Graph.load(fn)
+        out["load_s"] = round(time.perf_counter() - t, 3)
+    sp = torch.cuda.current_stream().cuda_stream
+    x = (torch.rand((4, cg.num_samples), device="cuda") < 0.5).to(torch.int32)
+    ys = []
+    for g in (cg, loaded):
+        y = torch.zeros((4, g.num_mutations), dtype=torch.int32, device="cuda")
+        g.matmul_ptr(capi.I32, x.data_ptr(), g.num_samples, 4, capi.UP, y.data_ptr(), y.numel(), stream=sp)
+        torch.cuda.synchronize()
+        ys.append(y.cpu().numpy())
+    out["loaded_equals_created"] = bool(np.array_equal(ys[0], ys[1]))
+    loaded.close()
+    return out
+
+
 def construction_benchmark():
     """Graph construction from the carrier lists of the reference's test VCF (400 haplotypes x
     10,893 sites): map_mutations (smallest carrier sets first, one mutation per batch, one host
@@ -1012,7 +1050,8 @@ def main():
     if not args.no_extras:
         log("options, .grg file and construction ...")
         extras = {"options_K16_f32": options_benchmark(cg, N, M, V, E), "host_path_f32": host_path_benchmark(g, E),
-                  "grg_io": grg_io_benchmark(g), "construction": construction_benchmark()}
+                  "grg_io": grg_io_benchmark(g), "construction": construction_benchmark(),
+                  "levelized_file": levelized_file_benchmark(arrays, cg, graph_create_s)}
         log("constructed-graph shape ...")
         n, m, t = (int(v) for v in args.constructed_size.split(","))
         extras["constructed_graph"] = constructed_graph_benchmark(n, m, t, autotune=not args.no_autotune)

@@ -56,6 +56,10 @@ EXPORTED_SYMBOLS = [
     "grgl_hip_graph_create",
     "grgl_hip_graph_destroy",
     "grgl_hip_graph_info",
+    "grgl_hip_graph_store",
+    "grgl_hip_graph_load",
+    "grgl_hip_levelize_to_file",
+    "grgl_hip_levelized_file_info",
     "grgl_hip_matmul",
     "grgl_hip_matmul_noalloc",
     "grgl_hip_matmul_batch",
@@ -185,6 +189,14 @@ def lib():
     L.grgl_hip_graph_destroy.argtypes = [vp]
     L.grgl_hip_graph_info.restype = ci
     L.grgl_hip_graph_info.argtypes = [vp, ctypes.POINTER(GraphInfo)]
+    L.grgl_hip_graph_store.restype = ci
+    L.grgl_hip_graph_store.argtypes = [vp, ctypes.c_char_p]
+    L.grgl_hip_graph_load.restype = ci
+    L.grgl_hip_graph_load.argtypes = [ctypes.c_char_p, ci, ctypes.POINTER(vp)]
+    L.grgl_hip_levelize_to_file.restype = ci
+    L.grgl_hip_levelize_to_file.argtypes = [ctypes.POINTER(GraphDesc), ctypes.c_char_p]
+    L.grgl_hip_levelized_file_info.restype = ci
+    L.grgl_hip_levelized_file_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(GraphInfo)]
     L.grgl_hip_matmul.restype = ci
     L.grgl_hip_matmul.argtypes = [vp, ci, vp, sz, sz, ci, vp, sz, ci, ci, vp, ci, vp, u32, vp]
     if hasattr(L, "grgl_hip_matmul_noalloc"):
@@ -256,31 +268,71 @@ def stream_ceiling(device=0, nbytes=2 << 30):
     return {"read_GBps": r.value, "copy_GBps": c.value}
 
 
+def _desc(arrays):
+    """A GraphDesc over contiguous copies of the arrays (returned with the copies to keep)."""
+    keep = {
+        "off": np.ascontiguousarray(arrays["down_offsets"], dtype=np.uint64),
+        "child": np.ascontiguousarray(arrays["down_children"], dtype=np.uint32),
+        "mut": np.ascontiguousarray(arrays["mutation_nodes"], dtype=np.uint32),
+    }
+    miss = arrays.get("mutation_miss_nodes")
+    coal = arrays.get("coal_counts")
+    if miss is not None:
+        keep["miss"] = np.ascontiguousarray(miss, dtype=np.uint32)
+    if coal is not None:
+        keep["coal"] = np.ascontiguousarray(coal, dtype=np.uint32)
+    d = GraphDesc()
+    d.num_samples = int(arrays["num_samples"])
+    d.num_nodes = len(keep["off"]) - 1
+    d.ploidy = int(arrays["ploidy"])
+    d.num_mutations = len(keep["mut"])
+    d.down_offsets = _ptr(keep["off"])
+    d.down_children = _ptr(keep["child"])
+    d.mutation_nodes = _ptr(keep["mut"])
+    d.mutation_miss_nodes = _ptr(keep.get("miss"))
+    d.coal_counts = _ptr(keep.get("coal"))
+    return d, keep
+
+
+def levelize_to_file(arrays, path):
+    """Levelize a graph on the host (no device needed) into a levelized-graph file that
+    Graph.load reads (grgl_hip_levelize_to_file)."""
+    d, keep = _desc(arrays)
+    check(lib().grgl_hip_levelize_to_file(ctypes.byref(d), os.fsencode(path)))
+
+
+def levelized_file_info(path):
+    """Validate a levelized-graph file on the host; its sizes (grgl_hip_levelized_file_info)."""
+    i = GraphInfo()
+    check(lib().grgl_hip_levelized_file_info(os.fsencode(path), ctypes.byref(i)))
+    return {f: getattr(i, f) for f, _ in GraphInfo._fields_}
+
+
 class Graph:
-    """A device-resident graph created through grgl_hip_graph_create."""
+    """A device-resident graph created through grgl_hip_graph_create (or loaded from a
+    levelized-graph file with Graph.load)."""
+
+    @classmethod
+    def load(cls, path, device=0):
+        """grgl_hip_graph_load: the device graph a levelized-graph file holds."""
+        h = ctypes.c_void_p()
+        check(lib().grgl_hip_graph_load(os.fsencode(path), device, ctypes.byref(h)))
+        self = cls.__new__(cls)
+        self._keep = {}
+        self.handle = h
+        info = self.info()
+        self.num_samples = info["num_samples"]
+        self.num_nodes = info["num_nodes"]
+        self.num_mutations = info["num_mutations"]
+        self.ploidy = info["ploidy"]
+        return self
+
+    def store(self, path):
+        """grgl_hip_graph_store: write this graph's plans to a levelized-graph file."""
+        check(lib().grgl_hip_graph_store(self.handle, os.fsencode(path)))
 
     def __init__(self, arrays, device=0):
-        self._keep = {
-            "off": np.ascontiguousarray(arrays["down_offsets"], dtype=np.uint64),
-            "child": np.ascontiguousarray(arrays["down_children"], dtype=np.uint32),
-            "mut": np.ascontiguousarray(arrays["mutation_nodes"], dtype=np.uint32),
-        }
-        miss = arrays.get("mutation_miss_nodes")
-        coal = arrays.get("coal_counts")
-        if miss is not None:
-            self._keep["miss"] = np.ascontiguousarray(miss, dtype=np.uint32)
-        if coal is not None:
-            self._keep["coal"] = np.ascontiguousarray(coal, dtype=np.uint32)
-        d = GraphDesc()
-        d.num_samples = int(arrays["num_samples"])
-        d.num_nodes = len(self._keep["off"]) - 1
-        d.ploidy = int(arrays["ploidy"])
-        d.num_mutations = len(self._keep["mut"])
-        d.down_offsets = _ptr(self._keep["off"])
-        d.down_children = _ptr(self._keep["child"])
-        d.mutation_nodes = _ptr(self._keep["mut"])
-        d.mutation_miss_nodes = _ptr(self._keep.get("miss"))
-        d.coal_counts = _ptr(self._keep.get("coal"))
+        d, self._keep = _desc(arrays)
         h = ctypes.c_void_p()
         check(lib().grgl_hip_graph_create(ctypes.byref(d), device, ctypes.byref(h)))
         self.handle = h

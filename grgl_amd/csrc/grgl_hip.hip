@@ -1217,6 +1217,91 @@ void freePlan(DeviceDirPlan& d) {
     d = DeviceDirPlan();
 }
 
+// The host form of a graph descriptor, validated (misuse errors for bad input).
+HostGrg hostGrgOf(const grgl_graph_desc& desc) {
+    HostGrg h;
+    h.numSamples = desc.num_samples;
+    h.ploidy = desc.ploidy;
+    if (!desc.down_offsets) throw MisuseError("down_offsets is required");
+    h.downOff.assign(desc.down_offsets, desc.down_offsets + (size_t)desc.num_nodes + 1);
+    if (h.downOff.back() > 0 && !desc.down_children) throw MisuseError("down_children is required");
+    if (desc.num_mutations > 0 && !desc.mutation_nodes) throw MisuseError("mutation_nodes is required");
+    if (h.downOff.back() > 0) h.downChild.assign(desc.down_children, desc.down_children + h.downOff.back());
+    if (desc.num_mutations > 0) h.mutNode.assign(desc.mutation_nodes, desc.mutation_nodes + desc.num_mutations);
+    if (desc.mutation_miss_nodes) {
+        h.mutMiss.assign(desc.mutation_miss_nodes, desc.mutation_miss_nodes + desc.num_mutations);
+        bool any = false;
+        for (uint32_t x : h.mutMiss) any |= (x != kInvalidNode);
+        if (!any) h.mutMiss.clear();
+    }
+    if (desc.coal_counts) h.coal.assign(desc.coal_counts, desc.coal_counts + desc.num_nodes);
+    try {
+        validateHostGrg(h);
+    } catch (const std::runtime_error& e) {
+        throw MisuseError(e.what());
+    }
+    return h;
+}
+
+// A device graph from both plans: upload, control words, clock rate.
+std::unique_ptr<grgl_graph> makeDeviceGraph(LevelizedGraph lg, int device) {
+    int ndev = 0;
+    GRGL_HIP_CHECK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) throw MisuseError("invalid device ordinal " + std::to_string(device));
+    GRGL_HIP_CHECK(hipSetDevice(device));
+    std::unique_ptr<grgl_graph> g(new grgl_graph());
+    g->device = device;
+    g->N = lg.N;
+    g->V = lg.V;
+    g->M = lg.M;
+    g->P = lg.P;
+    g->E = lg.E;
+    g->hasMiss = lg.hasMiss;
+    g->hasCoal = lg.hasCoal;
+    try {
+        uploadPlan(g->up, lg.up, g->deviceBytes);
+        uploadPlan(g->down, lg.down, g->deviceBytes);
+        void* ctl = nullptr;
+        GRGL_HIP_CHECK(hipHostMalloc(&ctl, sizeof(ChainControl), hipHostMallocMapped));
+        g->chainCtl = static_cast<ChainControl*>(ctl);
+        std::memset(g->chainCtl, 0, sizeof(ChainControl));
+        void* dctl = nullptr;
+        GRGL_HIP_CHECK(hipHostGetDevicePointer(&dctl, ctl, 0));
+        g->chainCtlDev = static_cast<ChainControl*>(dctl);
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+            g->wallClockKHz = (uint64_t)khz;
+    } catch (...) {
+        freePlan(g->up);
+        freePlan(g->down);
+        if (g->chainCtl) (void)hipHostFree(g->chainCtl);
+        throw;
+    }
+    return g;
+}
+
+template <typename T> void downloadVec(const T* d, size_t n, std::vector<T>& v) {
+    v.resize(n);
+    if (n) GRGL_HIP_CHECK(hipMemcpy(v.data(), d, n * sizeof(T), hipMemcpyDeviceToHost));
+}
+
+// The host plan of a device plan (the arrays uploadPlan copied; outRows and missRows as
+// counted by the direction).
+void downloadPlan(const DeviceDirPlan& d, DirPlan& p, uint32_t V, uint32_t outRows, bool miss) {
+    p.numNodes = d.numNodes;
+    p.numLevels = d.numLevels;
+    p.firstPulledLevel = d.firstPulledLevel;
+    p.maxFanIn = d.maxFanIn;
+    p.levelStart = d.levelStart;
+    downloadVec(d.off, (size_t)V + 1, p.off);
+    downloadVec(d.src, d.numEntries, p.src);
+    downloadVec(d.order, V, p.order);
+    downloadVec(d.rank, V, p.rank);
+    downloadVec(d.outRow, outRows, p.outRow);
+    downloadVec(d.missRow, miss ? outRows : 0, p.missRow);
+    downloadVec(d.coalByRow, V, p.coalByRow);
+}
+
 } // namespace
 
 GRGL_EXPORT const char* grgl_hip_last_error(void) { return g_lastError.c_str(); }
@@ -1246,88 +1331,82 @@ GRGL_EXPORT grgl_status grgl_hip_graph_create(const grgl_graph_desc* desc, int d
     GRGL_API_BEGIN
     if (!desc || !out) throw MisuseError("null argument");
     *out = nullptr;
-    HostGrg h;
-    h.numSamples = desc->num_samples;
-    h.ploidy = desc->ploidy;
-    if (!desc->down_offsets) throw MisuseError("down_offsets is required");
-    h.downOff.assign(desc->down_offsets, desc->down_offsets + (size_t)desc->num_nodes + 1);
-    if (h.downOff.back() > 0 && !desc->down_children) throw MisuseError("down_children is required");
-    if (desc->num_mutations > 0 && !desc->mutation_nodes) throw MisuseError("mutation_nodes is required");
-    if (h.downOff.back() > 0) h.downChild.assign(desc->down_children, desc->down_children + h.downOff.back());
-    if (desc->num_mutations > 0) h.mutNode.assign(desc->mutation_nodes, desc->mutation_nodes + desc->num_mutations);
-    if (desc->mutation_miss_nodes) {
-        h.mutMiss.assign(desc->mutation_miss_nodes, desc->mutation_miss_nodes + desc->num_mutations);
-        bool any = false;
-        for (uint32_t x : h.mutMiss) any |= (x != kInvalidNode);
-        if (!any) h.mutMiss.clear();
-    }
-    if (desc->coal_counts) h.coal.assign(desc->coal_counts, desc->coal_counts + desc->num_nodes);
+    HostGrg h = hostGrgOf(*desc);
+    if ((uint64_t)h.numNodes() + 2ULL * h.numMutations() >= 0x7fffffffULL)
+        throw MisuseError("graph too large: nodes + 2*mutations must stay below 2^31");
+    *out = makeDeviceGraph(levelize(h), device).release();
+    GRGL_API_END
+}
+
+GRGL_EXPORT grgl_status grgl_hip_levelize_to_file(const grgl_graph_desc* desc, const char* path) {
+    GRGL_API_BEGIN
+    if (!desc || !path) throw MisuseError("null argument");
+    HostGrg h = hostGrgOf(*desc);
+    if ((uint64_t)h.numNodes() + 2ULL * h.numMutations() >= 0x7fffffffULL)
+        throw MisuseError("graph too large: nodes + 2*mutations must stay below 2^31");
+    writeLevelizedFile(levelize(h), path);
+    GRGL_API_END
+}
+
+GRGL_EXPORT grgl_status grgl_hip_graph_store(const grgl_graph* g, const char* path) {
+    GRGL_API_BEGIN
+    if (!g || !path) throw MisuseError("null argument");
+    GRGL_HIP_CHECK(hipSetDevice(g->device));
+    LevelizedGraph lg;
+    lg.N = g->N;
+    lg.V = g->V;
+    lg.M = g->M;
+    lg.P = g->P;
+    lg.E = g->E;
+    lg.hasMiss = g->hasMiss;
+    lg.hasCoal = g->hasCoal;
+    downloadPlan(g->up, lg.up, lg.V, lg.M, lg.hasMiss);
+    downloadPlan(g->down, lg.down, lg.V, lg.N, false);
+    writeLevelizedFile(lg, path);
+    GRGL_API_END
+}
+
+GRGL_EXPORT grgl_status grgl_hip_graph_load(const char* path, int device, grgl_graph** out) {
+    GRGL_API_BEGIN
+    if (!path || !out) throw MisuseError("null argument");
+    *out = nullptr;
+    LevelizedGraph lg;
     try {
-        validateHostGrg(h);
+        lg = readLevelizedFile(path);
     } catch (const std::runtime_error& e) {
         throw MisuseError(e.what());
     }
-    if ((uint64_t)h.numNodes() + 2ULL * h.numMutations() >= 0x7fffffffULL)
-        throw MisuseError("graph too large: nodes + 2*mutations must stay below 2^31");
-    // The two directions' plans are independent: build DOWN on a second thread.
-    DirPlan up, down;
-    std::exception_ptr downError;
-    std::thread downThread([&] {
-        try {
-            down = buildDownPlan(h);
-        } catch (...) {
-            downError = std::current_exception();
-        }
-    });
+    *out = makeDeviceGraph(std::move(lg), device).release();
+    GRGL_API_END
+}
+
+GRGL_EXPORT grgl_status grgl_hip_levelized_file_info(const char* path, grgl_graph_info* info) {
+    GRGL_API_BEGIN
+    if (!path || !info) throw MisuseError("null argument");
+    LevelizedGraph lg;
     try {
-        up = buildUpPlan(h);
-    } catch (...) {
-        downThread.join();
-        throw;
+        lg = readLevelizedFile(path);
+    } catch (const std::runtime_error& e) {
+        throw MisuseError(e.what());
     }
-    downThread.join();
-    if (downError) std::rethrow_exception(downError);
-    int ndev = 0;
-    GRGL_HIP_CHECK(hipGetDeviceCount(&ndev));
-    if (device < 0 || device >= ndev) throw MisuseError("invalid device ordinal " + std::to_string(device));
-    GRGL_HIP_CHECK(hipSetDevice(device));
-    std::unique_ptr<grgl_graph> g(new grgl_graph());
-    g->device = device;
-    g->N = h.numSamples;
-    g->V = h.numNodes();
-    g->M = h.numMutations();
-    g->P = h.ploidy;
-    g->E = h.numEdges();
-    g->hasMiss = !h.mutMiss.empty();
-    g->hasCoal = h.hasCoals();
-    if (g->hasCoal) {
-        for (uint32_t v = g->N; v < g->V; v++) {
-            if (h.coal[v] == kCoalNotSet) {
-                g->hasCoal = false;
-                break;
-            }
-        }
-    }
-    try {
-        uploadPlan(g->up, up, g->deviceBytes);
-        uploadPlan(g->down, down, g->deviceBytes);
-        void* ctl = nullptr;
-        GRGL_HIP_CHECK(hipHostMalloc(&ctl, sizeof(ChainControl), hipHostMallocMapped));
-        g->chainCtl = static_cast<ChainControl*>(ctl);
-        std::memset(g->chainCtl, 0, sizeof(ChainControl));
-        void* dctl = nullptr;
-        GRGL_HIP_CHECK(hipHostGetDevicePointer(&dctl, ctl, 0));
-        g->chainCtlDev = static_cast<ChainControl*>(dctl);
-        int khz = 0;
-        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
-            g->wallClockKHz = (uint64_t)khz;
-    } catch (...) {
-        freePlan(g->up);
-        freePlan(g->down);
-        if (g->chainCtl) (void)hipHostFree(g->chainCtl);
-        throw;
-    }
-    *out = g.release();
+    *info = grgl_graph_info{};
+    info->num_samples = lg.N;
+    info->num_nodes = lg.V;
+    info->num_mutations = lg.M;
+    info->ploidy = lg.P;
+    info->num_edges = lg.E;
+    info->up_levels = lg.up.numLevels;
+    info->down_levels = lg.down.numLevels;
+    info->max_fanin_up = lg.up.maxFanIn;
+    info->max_fanin_down = lg.down.maxFanIn;
+    info->up_gather_entries = lg.up.src.size();
+    info->down_gather_entries = lg.down.src.size();
+    uint64_t bytes = 0;
+    for (const DirPlan* p : {&lg.up, &lg.down})
+        bytes += 4 * (p->off.size() + p->src.size() + p->order.size() + p->rank.size() + p->outRow.size() +
+                      p->missRow.size() + p->coalByRow.size());
+    info->device_bytes = bytes;
+    info->device = -1;
     GRGL_API_END
 }
 

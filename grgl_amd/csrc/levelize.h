@@ -18,6 +18,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "host_graph.h"
@@ -49,5 +50,25 @@ DirPlan buildDownPlan(const HostGrg& g);
 
 // Number of rows at the front of level `lvl` whose gather lists are longer than `heavy`.
 uint32_t heavyPrefix(const DirPlan& p, uint32_t lvl, uint32_t heavy);
+
+// Both plans of a graph and what the device graph needs besides them.
+struct LevelizedGraph {
+    uint32_t N = 0, V = 0, M = 0, P = 1;
+    uint64_t E = 0;
+    bool hasMiss = false, hasCoal = false;
+    DirPlan up, down;
+};
+
+// Builds both plans (the two directions on two threads).  Throws std::runtime_error.
+LevelizedGraph levelize(const HostGrg& g);
+
+// Levelized-graph files: a header (magic, format version, graph sizes), both plans' arrays,
+// and a 64-bit checksum of everything before it.  readLevelizedFile checks the checksum and
+// every invariant the device kernels rely on (in-range and level-ordered gather entries,
+// list lengths non-increasing within a level, permutations, read-out maps), so a file that
+// loads cannot make a traversal read or write out of bounds.  Both throw std::runtime_error.
+void writeLevelizedFile(const LevelizedGraph& lg, const std::string& path);
+LevelizedGraph readLevelizedFile(const std::string& path);
+void validateLevelized(const LevelizedGraph& lg);
 
 } // namespace grgl_amd

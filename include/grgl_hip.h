@@ -110,6 +110,29 @@ grgl_status grgl_hip_graph_destroy(grgl_graph* graph);
 grgl_status grgl_hip_graph_info(const grgl_graph* graph, grgl_graph_info* info);
 
 /*
+ * Levelized-graph files: a device graph's level-bucketed plans (both directions) on disk, so
+ * that a large graph is levelized once and later loaded straight to the device.  The file
+ * format is this library's own (magic "GRGLLVLZ", version, sizes, arrays, 64-bit checksum);
+ * loading checks the checksum and every structural invariant the kernels rely on, and
+ * rejects a file that fails with GRGL_ERR_MISUSE.
+ *   grgl_hip_graph_store  <- storeGPUGRGToDisk (include/grgl/cuda/gpu_grg.h:473-480,
+ *                            src/cuda/gpu_grg.cu:10-68): writes a device graph's plans.
+ *   grgl_hip_graph_load   <- loadGPUGRGFromDisk (include/grgl/cuda/gpu_grg.h:482-489,
+ *                            src/cuda/gpu_grg.cu:70-141): a device graph from such a file,
+ *                            as grgl_hip_graph_create would have built it.
+ *   grgl_hip_levelize_to_file <- no reference counterpart (the reference levelizes only on
+ *                            a machine with a GPU): levelizes a graph on the host, no device
+ *                            needed, and writes the file.
+ *   grgl_hip_levelized_file_info <- no reference counterpart: validates a file on the host
+ *                            and reports its sizes (device = -1, device_bytes = what a load
+ *                            would allocate).
+ */
+grgl_status grgl_hip_graph_store(const grgl_graph* graph, const char* path);
+grgl_status grgl_hip_graph_load(const char* path, int device, grgl_graph** out);
+grgl_status grgl_hip_levelize_to_file(const grgl_graph_desc* desc, const char* path);
+grgl_status grgl_hip_levelized_file_info(const char* path, grgl_graph_info* info);
+
+/*
  * GRG::matrixMultiplication on the device, argument for argument
  * (include/grgl/grg.h:656-667), plus dtype, flags and a stream:
  *   input        input_rows (K) x input_cols, row-major.  UP: input_cols = N (or N/ploidy

@@ -13,6 +13,7 @@
 #define GRGL_HIP_HPP
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -89,6 +90,22 @@ public:
     DeviceGRG& operator=(const DeviceGRG&) = delete;
     ~DeviceGRG() { grgl_hip_graph_destroy(m_graph); }
 
+    /* loadGPUGRGFromDisk (include/grgl/cuda/gpu_grg.h:482-489): a graph from a levelized-graph
+     * file written by store() or grgl_hip_levelize_to_file. */
+    static std::unique_ptr<DeviceGRG> load(const std::string& path, int device = 0) {
+        grgl_graph* g = nullptr;
+        check(grgl_hip_graph_load(path.c_str(), device, &g));
+        grgl_graph_info info{};
+        const grgl_status st = grgl_hip_graph_info(g, &info);
+        if (st != GRGL_OK) {
+            grgl_hip_graph_destroy(g);
+            check(st);
+        }
+        return std::unique_ptr<DeviceGRG>(new DeviceGRG(g, info));
+    }
+    /* storeGPUGRGToDisk (include/grgl/cuda/gpu_grg.h:473-480). */
+    void store(const std::string& path) const { check(grgl_hip_graph_store(m_graph, path.c_str())); }
+
     uint32_t numSamples() const { return m_numSamples; }
     uint32_t numIndividuals() const { return m_numSamples / m_ploidy; }
     uint32_t numNodes() const { return m_numNodes; }
@@ -125,6 +142,13 @@ public:
     }
 
 private:
+    DeviceGRG(grgl_graph* g, const grgl_graph_info& info)
+        : m_graph(g),
+          m_numSamples(info.num_samples),
+          m_ploidy(info.ploidy),
+          m_numNodes(info.num_nodes),
+          m_numMutations(info.num_mutations) {}
+
     grgl_graph* m_graph = nullptr;
     uint32_t m_numSamples, m_ploidy, m_numNodes, m_numMutations;
 };

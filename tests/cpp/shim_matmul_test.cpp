@@ -1,6 +1,8 @@
 // The reference's C++ matmul unit tests (test/unit/test_grg.cpp:41-161), run against the
 // device-resident graph through grgl_hip.hpp.  Exit code 0 = all checks passed.
 #include <cstdio>
+#include <cstdlib>
+#include <memory>
 #include <vector>
 
 #include "grgl_hip.hpp"
@@ -52,6 +54,17 @@ int main() {
     try { grg.matMul(std::vector<double>(9), 2, grgl_hip::DIRECTION_UP); } catch (const grgl_hip::ApiMisuseFailure&) { thrown++; }
     try { grg.matMul(std::vector<double>(9), 9, grgl_hip::DIRECTION_UP); } catch (const grgl_hip::ApiMisuseFailure&) { thrown++; }
     EXPECT(thrown == 4);
+
+    // storeGPUGRGToDisk / loadGPUGRGFromDisk: the reloaded graph multiplies the same.
+    const char* path = std::getenv("GRGL_SHIM_LVLZ");
+    if (path) {
+        grg.store(path);
+        std::unique_ptr<grgl_hip::DeviceGRG> again = grgl_hip::DeviceGRG::load(path);
+        EXPECT(again->numSamples() == 4 && again->numMutations() == 3 && again->numNodes() == 7);
+        EXPECT((again->matMul(sv, 2, grgl_hip::DIRECTION_UP) == std::vector<double>{1.0, 2.0, 4.0, 0.25, 0.5, 1.0}));
+        EXPECT((again->matMul(mv, 2, grgl_hip::DIRECTION_DOWN) ==
+                std::vector<double>{1.0, 2.0, 3.0, 3.0, 1.0, 1.0, 2.0, 2.0}));
+    }
 
     std::printf("%s (%d failures)\n", failures ? "FAIL" : "OK", failures);
     return failures ? 1 : 0;

@@ -387,6 +387,10 @@ def test_cpp_shim_reference_unit_tests(gpu, tmp_path):
 
     from test_host import _build_shim_test
 
+    import os
+
     exe = _build_shim_test(tmp_path)
-    proc = subprocess.run([str(exe)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    env = dict(os.environ, GRGL_SHIM_LVLZ=str(tmp_path / "shim.lvlz"))
+    proc = subprocess.run([str(exe)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120,
+                          env=env)
     assert proc.returncode == 0, proc.stdout
