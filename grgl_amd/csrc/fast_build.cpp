@@ -373,7 +373,8 @@ template <typename Fn> void parallelFor(size_t n, uint32_t threads, Fn&& fn) {
 
 // Cut the rows into trees and windows (getHapSegments, fast_build.cpp:266-342, and the
 // driver's loops, :857-947).  Rows carried by fewer samples than the threshold are mapped
-// directly; the others are the tree rows, in windows of hapLength.
+// directly, unless their site has missing data; the others are the tree rows, in windows of
+// hapLength.
 //  * With a tree count, trees take ceil(rows/count) rows each (direct rows included) and the
 //    last one takes the rest; each tree's windows start at its first tree row.
 //  * Automatic: a tree ends after the window that brings its summed unique-segment count to
@@ -381,8 +382,9 @@ template <typename Fn> void parallelFor(size_t n, uint32_t threads, Fn&& fn) {
 //    global 254-row blocks of tree rows; rows after a tree's last window start the next tree
 //    (a tree of direct rows only, when no tree row follows).
 // Windows are built in parallel (each depends only on its rows); trees then own their windows.
-std::vector<TreeInput> cutTrees(uint32_t N, const CarrierRows& car, const FastBuildParams& p, uint32_t threads,
-                                std::vector<WindowJob>& jobs, std::vector<uint32_t>& treeRowsAll) {
+std::vector<TreeInput> cutTrees(uint32_t N, const CarrierRows& car, const std::vector<uint8_t>& atMissingSite,
+                                const FastBuildParams& p, uint32_t threads, std::vector<WindowJob>& jobs,
+                                std::vector<uint32_t>& treeRowsAll) {
     const size_t M = car.rows;
     const uint32_t L = p.hapLength;
     const size_t directBelow = p.noTreeBelow < 1.0 ? (size_t)((double)N * p.noTreeBelow) : (size_t)p.noTreeBelow;
@@ -390,7 +392,10 @@ std::vector<TreeInput> cutTrees(uint32_t N, const CarrierRows& car, const FastBu
     std::vector<uint8_t> direct(M);
     treeRowsAll.clear();
     for (size_t r = 0; r < M; r++) {
-        direct[r] = car.size(r) < directBelow;
+        // Rows of a site with missing data (and the missing-data rows) always go through the
+        // tree, so the missingness node and the site's mutations come from one mapping
+        // (fast_build.cpp:316-322).
+        direct[r] = car.size(r) < directBelow && !atMissingSite[r];
         if (!direct[r]) treeRowsAll.push_back((uint32_t)r);
     }
     // Row ranges [b, e) of the trees and, per tree, the ranges of treeRowsAll in its windows.
@@ -784,9 +789,18 @@ MutableGraph fastBuildFromSamples(uint32_t numSamples, uint32_t ploidy, const st
         carrierSamples = normSamples.data();
     }
     const CarrierRows car{carrierOff, carrierSamples, M};
+    // A missing-data row (allele ".") lists the haplotypes with no call at its site and comes
+    // first among the site's rows, as the reference's readers emit it (mut_iterator.cpp:421-425).
+    std::vector<uint8_t> atMissingSite(M, 0);
     for (size_t i = 0; i < M; i++) {
-        if (rows[i].allele == ".") misuse("missing-data rows are not supported by the graph constructor");
         if (car.size(i) && car.begin(i)[car.size(i) - 1] >= numSamples) misuse("sample ID out of range");
+        if (rows[i].allele == ".") {
+            if (i > 0 && rows[i - 1].position == rows[i].position)
+                misuse("a missing-data row must be the first row of its site");
+            atMissingSite[i] = 1;
+        } else if (i > 0 && atMissingSite[i - 1] && rows[i - 1].position == rows[i].position) {
+            atMissingSite[i] = 1;
+        }
     }
     MutableGraph out(numSamples, ploidy, true);
     stats = FastBuildStats{};
@@ -794,7 +808,7 @@ MutableGraph fastBuildFromSamples(uint32_t numSamples, uint32_t ploidy, const st
 
     std::vector<WindowJob> jobs;
     std::vector<uint32_t> treeRowsAll;
-    std::vector<TreeInput> inputs = cutTrees(numSamples, car, params, threads, jobs, treeRowsAll);
+    std::vector<TreeInput> inputs = cutTrees(numSamples, car, atMissingSite, params, threads, jobs, treeRowsAll);
     const size_t T = inputs.size();
     std::vector<HostGrg> trees(T);
     std::vector<FastBuildStats> treeStats(T);
@@ -838,8 +852,22 @@ MutableGraph fastBuildFromSamples(uint32_t numSamples, uint32_t ploidy, const st
     }
     // Without tree mapping the graph is returned without mutations (the caller maps them,
     // fast_build.cpp:652-653 and grgl.cpp:331-350).
+    // Mutations in row order; a missing-data row gives its node to the mutations of its site as
+    // their missingness node (fast_build.cpp:672-687).
     if (params.treeMap) {
-        for (size_t i = 0; i < rows.size(); i++) out.addMutation(rows[i], nodeOfRow[i]);
+        bool haveMissing = false;
+        double missPos = 0;
+        uint32_t missNode = kInvalidNode;
+        for (size_t i = 0; i < rows.size(); i++) {
+            if (rows[i].allele == ".") {
+                haveMissing = true;
+                missPos = rows[i].position;
+                missNode = nodeOfRow[i];
+                continue;
+            }
+            const bool sameSite = haveMissing && missPos == rows[i].position;
+            out.addMutation(rows[i], nodeOfRow[i], sameSite ? missNode : kInvalidNode);
+        }
     }
     return out;
 }

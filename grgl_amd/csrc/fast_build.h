@@ -18,7 +18,9 @@
 //   5. Trees are merged (grg_merge.cpp, node mapping by child set): a node whose mapped
 //      children equal an existing node's is that node.
 // Diploid graphs get coalescence counts (individuals whose two haplotypes arrive through
-// different children).  Missing-data rows are not supported here (map_mutations takes them).
+// different children).  Missing-data rows (allele ".", listing the haplotypes with no call)
+// come first among their site's rows; they are mapped like the tree rows, and their node is
+// the missingness node of the site's mutations.
 #pragma once
 
 #include <cstdint>

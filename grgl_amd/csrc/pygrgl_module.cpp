@@ -720,7 +720,8 @@ PYBIND11_MODULE(_grgl_amd, m) {
     m.def(
         "fast_grg_from_samples",
         [toRecord](uint32_t numSamples, uint32_t ploidy, py::object mutations, py::object samples,
-                   py::object carrierOffsets, py::object carriers, py::object positions, py::object trees,
+                   py::object carrierOffsets, py::object carriers, py::object positions, py::object missing,
+                   py::object trees,
                    double lfNoTree, bool noTreeMap, double rebuildProportion, uint32_t threads, bool returnStats) {
             FastBuildParams p;
             p.noTreeBelow = lfNoTree;
@@ -776,10 +777,13 @@ PYBIND11_MODULE(_grgl_amd, m) {
                 std::vector<double> pos;
                 if (!positions.is_none()) pos = positions.cast<std::vector<double>>();
                 API_CHECK(pos.empty() || pos.size() == M, "positions must have one entry per mutation");
+                std::vector<bool> miss;
+                if (!missing.is_none()) miss = missing.cast<std::vector<bool>>();
+                API_CHECK(miss.empty() || miss.size() == M, "missing must have one entry per mutation");
                 recs.resize(M);
                 for (size_t i = 0; i < M; i++) {
                     recs[i].position = pos.empty() ? (double)i : pos[i];
-                    recs[i].allele = "G";
+                    recs[i].allele = (!miss.empty() && miss[i]) ? "." : "G";
                     recs[i].refAllele = "A";
                 }
             }
@@ -795,7 +799,8 @@ PYBIND11_MODULE(_grgl_amd, m) {
         },
         py::arg("num_samples"), py::arg("ploidy"), py::arg("mutations") = py::none(), py::arg("samples") = py::none(),
         py::kw_only(), py::arg("carrier_offsets") = py::none(), py::arg("carriers") = py::none(),
-        py::arg("positions") = py::none(), py::arg("trees") = py::str("optimal"), py::arg("lf_no_tree") = 0.0,
+        py::arg("positions") = py::none(), py::arg("missing") = py::none(), py::arg("trees") = py::str("optimal"),
+        py::arg("lf_no_tree") = 0.0,
         py::arg("no_tree_map") = false, py::arg("rebuild_proportion") = 0.10, py::arg("threads") = 0,
         py::arg("return_stats") = false,
         "Build a GRG from haplotypes, as the reference's graph construction does (fastGRGFromSamples, "
@@ -804,7 +809,9 @@ PYBIND11_MODULE(_grgl_amd, m) {
         "nodes.  Input: one carrier list per mutation (samples=...), or CSR arrays (carrier_offsets, carriers), in "
         "genome order.  trees: 'optimal', 'faster1', 'faster2' or a count; lf_no_tree: rows carried by fewer "
         "samples (a frequency when < 1) are mapped directly; no_tree_map: return the graph without mutations.  "
-        "Mutation IDs follow the row order.  Missing-data rows are not supported.");
+        "Mutation IDs follow the row order.  Missing-data rows (allele '.', or missing=True with arrays) list the "
+        "haplotypes with no call and come first among their site's rows; their node becomes the missingness node "
+        "of the site's mutations.");
 
     m.def("matmul", &matMul, py::arg("grg"), py::arg("input"), py::arg("direction"), py::arg("emit_all_nodes") = false,
           py::arg("by_individual") = false, py::arg("init") = py::none(), py::arg("miss") = py::none(),

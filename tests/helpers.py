@@ -141,3 +141,28 @@ def bits_equal(a, b):
     a = np.ascontiguousarray(a)
     b = np.ascontiguousarray(b)
     return a.shape == b.shape and a.dtype == b.dtype and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def missing_fixture_rows():
+    """The rows of the reference's test-200-samples.miss.igd (tests/golden/test200_miss.npz over
+    tests/golden/test200.npz): per site its missing-data row first (haplotypes with no call),
+    then the site's variants without those haplotypes.  Returns (num_samples, ploidy,
+    positions, is_missing, carrier lists)."""
+    fx = vcf_fixture()
+    m = np.load(os.path.join(GOLDEN, "test200_miss.npz"))
+    miss = {int(p): m["miss_carriers"][m["miss_off"][i]:m["miss_off"][i + 1]].astype(np.uint32)
+            for i, p in enumerate(m["miss_positions"])}
+    off, car, P = fx["carrier_off"], fx["carriers"], fx["positions"]
+    pos, is_miss, sets = [], [], []
+    for i, p in enumerate(P.tolist()):
+        if p in miss and (i == 0 or P[i - 1] != p):
+            pos.append(p)
+            is_miss.append(True)
+            sets.append(miss[p])
+        row = car[off[i]:off[i + 1]]
+        if p in miss:
+            row = np.setdiff1d(row, miss[p]).astype(np.uint32)
+        pos.append(p)
+        is_miss.append(False)
+        sets.append(row)
+    return int(fx["num_samples"]), int(fx["ploidy"]), np.array(pos), np.array(is_miss), sets

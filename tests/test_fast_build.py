@@ -10,7 +10,10 @@ Pinned by what a correct build must satisfy, independent of how it clusters:
   * every mutation's carrier set reproduced exactly, every sample reached by at most one path;
   * coalescence counts equal to a brute-force count on the finished graph;
   * the tree-splitting options (count, faster modes, low-frequency direct mapping) and the
-    edge cases (empty rows, duplicates, ploidy 1, errors)."""
+    edge cases (empty rows, duplicates, ploidy 1, errors);
+  * missing data: the reference's test-200-samples.miss.igd (fixture tests/golden/
+    test200_miss.npz) gives the missing-sample and missing-individual totals its test states,
+    and the matmul identities with `miss` of test/endtoend/test_missing.py:55-89."""
 
 import numpy as np
 import pytest
@@ -209,11 +212,112 @@ def test_edge_cases():
     _assert_same_carriers(g, off, car)
     with pytest.raises(RuntimeError, match="out of range"):
         _build(4, *_csr([[0, 4]]))
-    with pytest.raises(RuntimeError, match="missing-data"):
-        grgl_amd.fast_grg_from_samples(4, 2, [grgl_amd.Mutation(1, ".", "A")], [[0]])
+    # a lone missing-data row: no mutation, nothing fails
+    g = grgl_amd.fast_grg_from_samples(4, 2, [grgl_amd.Mutation(1, ".", "A")], [[0]])
+    assert g.num_mutations == 0
     with pytest.raises(RuntimeError, match="non-zero"):
         _build(4, *_csr([[0]]), trees=0)
     with pytest.raises(ValueError):
         _build(4, *_csr([[0]]), trees="fastest")
     with pytest.raises(RuntimeError, match="same length"):
         grgl_amd.fast_grg_from_samples(4, 2, [grgl_amd.Mutation(1, "G", "A")], [[0], [1]])
+
+
+def _missing_build(**kw):
+    from helpers import missing_fixture_rows
+
+    N, P, pos, is_miss, sets = missing_fixture_rows()
+    muts = [grgl_amd.Mutation(float(p), "." if m else "G", "A") for p, m in zip(pos, is_miss)]
+    g = grgl_amd.fast_grg_from_samples(N, P, muts, [s.tolist() for s in sets], **kw)
+    return g, N, pos, is_miss, sets
+
+
+def _missing_x(N, pos, is_miss, sets):
+    """Individual dosage matrix of the fixture with each missing call set to the allele frequency
+    among the called haplotypes (grg2X with diploid=True, test/endtoend/testing_utils.py:66-101,
+    built here from the input rows), and per mutation its missing haplotypes."""
+    muts = [i for i in range(len(sets)) if not is_miss[i]]
+    X = np.zeros((N // 2, len(muts)))
+    freq = np.zeros(len(muts))
+    missing = []
+    cur = None
+    k = 0
+    for i in range(len(sets)):
+        if is_miss[i]:
+            cur = (pos[i], sets[i])
+            continue
+        ms = cur[1] if cur is not None and cur[0] == pos[i] else np.zeros(0, np.uint32)
+        np.add.at(X[:, k], sets[i] // 2, 1)
+        freq[k] = len(sets[i]) / (N - len(ms)) if N > len(ms) else 0.0
+        missing.append(ms)
+        k += 1
+    for k, ms in enumerate(missing):
+        np.add.at(X[:, k], ms // 2, freq[k])
+    return X, freq, missing
+
+
+@pytest.mark.parametrize("kw", [{}, {"trees": 4}, {"lf_no_tree": 3}])
+def test_missing_data_rows(kw):
+    """The reference's missing-data test (test/endtoend/test_missing.py:19-90) on the graph the
+    constructor builds from test-200-samples.miss.igd: missingness nodes with exactly the missing
+    haplotypes, 25 missing samples and 21 missing individuals over the mutations that have a node
+    and a missingness node, and the matmul identities with `miss`."""
+    g, N, pos, is_miss, sets = _missing_build(**kw)
+    assert g.num_mutations == int(np.sum(~is_miss))
+    a = g.arrays()
+    V = len(a["down_offsets"]) - 1
+    X, freq, missing = _missing_x(N, pos, is_miss, sets)
+    # every mutation's carriers and missingness node
+    o, c = g.mutation_carriers()
+    plain = [s for s, m in zip(sets, is_miss) if not m]
+    for k, s in enumerate(plain):
+        assert np.array_equal(c[o[k]:o[k + 1]], s), k
+    miss_nodes = a["mutation_miss_nodes"]
+    assert miss_nodes is not None
+    below = {}
+    total = total_ind = 0
+    for k in range(g.num_mutations):
+        node, mn = a["mutation_nodes"][k], miss_nodes[k]
+        if len(missing[k]) == 0:
+            assert mn == INVALID, k
+            continue
+        assert mn < V, k
+        if mn not in below:
+            below[mn] = np.sort(_samples_below(a, mn))
+        assert np.array_equal(below[mn], missing[k]), k
+        if node != INVALID:
+            total += len(missing[k])
+            total_ind += len(set((missing[k] // 2).tolist()))
+    assert (total, total_ind) == (25, 21)  # MISSING_SAMPLES, MISSING_INDIVS (test_missing.py:21-22)
+    og = oracle.OracleGraph(a)
+    rng = np.random.default_rng(6)
+    K = 7
+    rv = rng.standard_normal((K, N // 2))
+    Mm = np.zeros((K, g.num_mutations))
+    up = og.matmul(rv, oracle.UP, by_individual=True, miss=Mm)
+    np.testing.assert_allclose(rv @ X, up + Mm * freq, rtol=1e-10, atol=1e-10)
+    rw = rng.standard_normal((K, g.num_mutations))
+    down = og.matmul(rw, oracle.DOWN, by_individual=True, miss=np.ascontiguousarray(freq[None, :] * rw))
+    np.testing.assert_allclose(rw @ X.T, down, rtol=1e-10, atol=1e-10)
+
+
+def _samples_below(a, node):
+    N = int(a["num_samples"])
+    off, ch = a["down_offsets"], a["down_children"]
+    out, stack, seen = [], [int(node)], set()
+    while stack:
+        n = stack.pop()
+        if n < N:
+            out.append(n)
+            continue
+        for c in ch[off[n]:off[n + 1]]:
+            if int(c) not in seen:
+                seen.add(int(c))
+                stack.append(int(c))
+    return np.array(out, dtype=np.uint32)
+
+
+def test_missing_row_order_checked():
+    muts = [grgl_amd.Mutation(5, "G", "A"), grgl_amd.Mutation(5, ".", "A")]
+    with pytest.raises(RuntimeError, match="first row of its site"):
+        grgl_amd.fast_grg_from_samples(4, 2, muts, [[0], [1]])

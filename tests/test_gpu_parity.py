@@ -106,6 +106,32 @@ def test_golden_vcf_built_graph(gpu):
     np.testing.assert_array_equal(zyg, fx["expect_Aa"] + 2 * fx["expect_aa"])
 
 
+def test_missing_data_built_graph(gpu):
+    """The reference's missing-data matmul test (test/endtoend/test_missing.py:55-89) on the
+    device, on the graph the constructor builds from test-200-samples.miss.igd: UP by individual
+    with the `miss` output and DOWN with a `miss` input reproduce the dense genotype matrix with
+    missing calls at the allele frequency; int64 results equal the oracle's bit for bit."""
+    from test_fast_build import _missing_build, _missing_x
+
+    g, N, pos, is_miss, sets = _missing_build()
+    X, freq, _ = _missing_x(N, pos, is_miss, sets)
+    a = g.arrays()
+    dg = _pg(a)
+    og = oracle.OracleGraph(a)
+    rng = np.random.default_rng(23)
+    rv = rng.standard_normal((7, N // 2))
+    M = np.zeros((7, g.num_mutations))
+    up = grgl_amd.matmul(dg, rv, UP, by_individual=True, miss=M)
+    np.testing.assert_allclose(rv @ X, up + M * freq, rtol=1e-10, atol=1e-10)
+    rw = rng.standard_normal((7, g.num_mutations))
+    down = grgl_amd.matmul(dg, rw, DOWN, by_individual=True, miss=np.ascontiguousarray(freq[None, :] * rw))
+    np.testing.assert_allclose(rw @ X.T, down, rtol=1e-10, atol=1e-10)
+    xi = rng.integers(-5, 6, size=(3, N)).astype(np.int64)
+    Mg, Mo = np.zeros((3, g.num_mutations), np.int64), np.zeros((3, g.num_mutations), np.int64)
+    assert bits_equal(grgl_amd.matmul(dg, xi, UP, miss=Mg), og.matmul(xi, oracle.UP, miss=Mo))
+    assert bits_equal(Mg, Mo)
+
+
 def test_built_graph_equals_tree_graph(gpu):
     """Two graphs of one genotype matrix (tree-direct and constructor-built, 1,000 x 10,000):
     integer products are bit-identical in both directions."""
