@@ -759,7 +759,8 @@ private:
 
 MutableGraph fastBuildFromSamples(uint32_t numSamples, uint32_t ploidy, const std::vector<MutationRecordIn>& rows,
                                   const uint64_t* carrierOff, const uint32_t* carrierSamples,
-                                  const FastBuildParams& params, FastBuildStats& stats, uint32_t threads) {
+                                  const FastBuildParams& params, FastBuildStats& stats, uint32_t threads,
+                                  bool phased) {
     if (numSamples == 0) misuse("num_samples must be positive");
     if (params.hapLength == 0 || params.hapLength > kMaxHapLength) misuse("hap_length must be in 1..254");
     if (!(params.rebuildProportion > 0.0)) misuse("rebuild_proportion must be positive");
@@ -802,7 +803,7 @@ MutableGraph fastBuildFromSamples(uint32_t numSamples, uint32_t ploidy, const st
             atMissingSite[i] = 1;
         }
     }
-    MutableGraph out(numSamples, ploidy, true);
+    MutableGraph out(numSamples, ploidy, phased);
     stats = FastBuildStats{};
     if (threads == 0) threads = std::max(1U, std::thread::hardware_concurrency());
 
@@ -874,7 +875,7 @@ MutableGraph fastBuildFromSamples(uint32_t numSamples, uint32_t ploidy, const st
 
 MutableGraph fastBuildFromSamples(uint32_t numSamples, uint32_t ploidy, const std::vector<MutationRecordIn>& rows,
                                   const std::vector<std::vector<uint32_t>>& carriers, const FastBuildParams& params,
-                                  FastBuildStats& stats, uint32_t threads) {
+                                  FastBuildStats& stats, uint32_t threads, bool phased) {
     if (rows.size() != carriers.size()) misuse("mutations and samples must be the same length");
     std::vector<uint64_t> off(1, 0);
     std::vector<uint32_t> flat;
@@ -882,7 +883,7 @@ MutableGraph fastBuildFromSamples(uint32_t numSamples, uint32_t ploidy, const st
         flat.insert(flat.end(), c.begin(), c.end());
         off.push_back(flat.size());
     }
-    return fastBuildFromSamples(numSamples, ploidy, rows, off.data(), flat.data(), params, stats, threads);
+    return fastBuildFromSamples(numSamples, ploidy, rows, off.data(), flat.data(), params, stats, threads, phased);
 }
 
 } // namespace grgl_amd

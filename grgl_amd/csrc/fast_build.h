@@ -57,10 +57,10 @@ struct FastBuildStats {
 // depend on the thread count.  Mutation IDs follow the row order.
 MutableGraph fastBuildFromSamples(uint32_t numSamples, uint32_t ploidy, const std::vector<MutationRecordIn>& rows,
                                   const std::vector<std::vector<uint32_t>>& carriers, const FastBuildParams& params,
-                                  FastBuildStats& stats, uint32_t threads = 0);
+                                  FastBuildStats& stats, uint32_t threads = 0, bool phased = true);
 // The same with the carrier lists as CSR: row i's samples are samples[off[i] .. off[i+1]).
 MutableGraph fastBuildFromSamples(uint32_t numSamples, uint32_t ploidy, const std::vector<MutationRecordIn>& rows,
                                   const uint64_t* off, const uint32_t* samples, const FastBuildParams& params,
-                                  FastBuildStats& stats, uint32_t threads = 0);
+                                  FastBuildStats& stats, uint32_t threads = 0, bool phased = true);
 
 } // namespace grgl_amd

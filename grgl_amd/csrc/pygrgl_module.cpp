@@ -451,6 +451,7 @@ PYBIND11_MODULE(_grgl_amd, m) {
                                [](PyGRG& g) { return std::make_pair(g.H().numSamples / g.H().ploidy, g.H().numMutations()); })
         .def_property_readonly("num_nodes", [](PyGRG& g) { return g.H().numNodes(); })
         .def_property_readonly("num_edges", [](PyGRG& g) { return g.H().numEdges(); })
+        .def_property_readonly("is_phased", [](PyGRG& g) { return g.H().phased; })
         .def_property_readonly("num_mutations", [](PyGRG& g) { return g.H().numMutations(); })
         .def_property_readonly("has_missing_data", [](PyGRG& g) { return !g.H().mutMiss.empty(); })
         .def_property_readonly("has_individual_coals", [](PyGRG& g) { return g.H().hasCoals(); })
@@ -722,7 +723,8 @@ PYBIND11_MODULE(_grgl_amd, m) {
         [toRecord](uint32_t numSamples, uint32_t ploidy, py::object mutations, py::object samples,
                    py::object carrierOffsets, py::object carriers, py::object positions, py::object missing,
                    py::object trees,
-                   double lfNoTree, bool noTreeMap, double rebuildProportion, uint32_t threads, bool returnStats) {
+                   double lfNoTree, bool noTreeMap, double rebuildProportion, uint32_t threads, bool returnStats,
+                   bool phased) {
             FastBuildParams p;
             p.noTreeBelow = lfNoTree;
             p.treeMap = !noTreeMap;
@@ -788,11 +790,11 @@ PYBIND11_MODULE(_grgl_amd, m) {
                 }
             }
             API_CHECK(recs.size() == M, "mutations and samples must be the same length");
-            auto g = std::make_shared<PyMutableGRG>(numSamples, ploidy, true);
+            auto g = std::make_shared<PyMutableGRG>(numSamples, ploidy, phased);
             FastBuildStats st;
             {
                 py::gil_scoped_release release;
-                g->mg = fastBuildFromSamples(numSamples, ploidy, recs, off.data(), carPtr, p, st, threads);
+                g->mg = fastBuildFromSamples(numSamples, ploidy, recs, off.data(), carPtr, p, st, threads, phased);
             }
             if (returnStats) return py::object(py::make_tuple(g, st));
             return py::object(py::cast(g));
@@ -802,7 +804,7 @@ PYBIND11_MODULE(_grgl_amd, m) {
         py::arg("positions") = py::none(), py::arg("missing") = py::none(), py::arg("trees") = py::str("optimal"),
         py::arg("lf_no_tree") = 0.0,
         py::arg("no_tree_map") = false, py::arg("rebuild_proportion") = 0.10, py::arg("threads") = 0,
-        py::arg("return_stats") = false,
+        py::arg("return_stats") = false, py::arg("phased") = true,
         "Build a GRG from haplotypes, as the reference's graph construction does (fastGRGFromSamples, "
         "src/fast_build.cpp:794-986): the variant rows are cut into trees, each tree clusters the samples by "
         "haplotype-segment similarity and maps its rows onto the roots, and the trees are merged combining equal "
@@ -811,7 +813,8 @@ PYBIND11_MODULE(_grgl_amd, m) {
         "samples (a frequency when < 1) are mapped directly; no_tree_map: return the graph without mutations.  "
         "Mutation IDs follow the row order.  Missing-data rows (allele '.', or missing=True with arrays) list the "
         "haplotypes with no call and come first among their site's rows; their node becomes the missingness node "
-        "of the site's mutations.");
+        "of the site's mutations.  phased=False marks the graph unphased (carriers of an unphased individual fill "
+        "its first haplotypes, as the reference's readers do, mut_iterator.cpp:404-410).");
 
     m.def("matmul", &matMul, py::arg("grg"), py::arg("input"), py::arg("direction"), py::arg("emit_all_nodes") = false,
           py::arg("by_individual") = false, py::arg("init") = py::none(), py::arg("miss") = py::none(),

@@ -321,3 +321,14 @@ def test_missing_row_order_checked():
     muts = [grgl_amd.Mutation(5, "G", "A"), grgl_amd.Mutation(5, ".", "A")]
     with pytest.raises(RuntimeError, match="first row of its site"):
         grgl_amd.fast_grg_from_samples(4, 2, muts, [[0], [1]])
+
+
+def test_unphased_flag():
+    """phased=False marks the graph unphased and changes nothing else."""
+    src, off, car = _coalescent_input(100, 500, 1, seed=3)
+    a = _build(100, off, car).arrays()
+    g = _build(100, off, car, phased=False)
+    assert not g.is_phased
+    b = g.arrays()
+    for k in ("down_offsets", "down_children", "mutation_nodes", "coal_counts"):
+        assert np.array_equal(a[k], b[k]), k
