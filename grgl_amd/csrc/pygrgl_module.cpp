@@ -724,8 +724,9 @@ PYBIND11_MODULE(_grgl_amd, m) {
                    py::object carrierOffsets, py::object carriers, py::object positions, py::object missing,
                    py::object trees,
                    double lfNoTree, bool noTreeMap, double rebuildProportion, uint32_t threads, bool returnStats,
-                   bool phased) {
+                   bool phased, uint32_t hapLength) {
             FastBuildParams p;
+            p.hapLength = hapLength;
             p.noTreeBelow = lfNoTree;
             p.treeMap = !noTreeMap;
             p.rebuildProportion = rebuildProportion;
@@ -804,7 +805,7 @@ PYBIND11_MODULE(_grgl_amd, m) {
         py::arg("positions") = py::none(), py::arg("missing") = py::none(), py::arg("trees") = py::str("optimal"),
         py::arg("lf_no_tree") = 0.0,
         py::arg("no_tree_map") = false, py::arg("rebuild_proportion") = 0.10, py::arg("threads") = 0,
-        py::arg("return_stats") = false, py::arg("phased") = true,
+        py::arg("return_stats") = false, py::arg("phased") = true, py::arg("hap_length") = 254,
         "Build a GRG from haplotypes, as the reference's graph construction does (fastGRGFromSamples, "
         "src/fast_build.cpp:794-986): the variant rows are cut into trees, each tree clusters the samples by "
         "haplotype-segment similarity and maps its rows onto the roots, and the trees are merged combining equal "
@@ -813,7 +814,8 @@ PYBIND11_MODULE(_grgl_amd, m) {
         "samples (a frequency when < 1) are mapped directly; no_tree_map: return the graph without mutations.  "
         "Mutation IDs follow the row order.  Missing-data rows (allele '.', or missing=True with arrays) list the "
         "haplotypes with no call and come first among their site's rows; their node becomes the missingness node "
-        "of the site's mutations.  phased=False marks the graph unphased (carriers of an unphased individual fill "
+        "of the site's mutations.  hap_length (1..254, the reference's fixed 254 by default) sets the "
+        "variants per haplotype window.  phased=False marks the graph unphased (carriers of an unphased individual fill "
         "its first haplotypes, as the reference's readers do, mut_iterator.cpp:404-410).");
 
     m.def("matmul", &matMul, py::arg("grg"), py::arg("input"), py::arg("direction"), py::arg("emit_all_nodes") = false,

@@ -332,3 +332,35 @@ def test_unphased_flag():
     b = g.arrays()
     for k in ("down_offsets", "down_children", "mutation_nodes", "coal_counts"):
         assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_inputs_keep_every_invariant(seed):
+    """Random haplotype tables (copied blocks with noise, duplicate, empty and full rows) under
+    small windows and every index-rebuild regime (often, sometimes, never): carriers exact, one
+    path per (mutation, sample), coalescence counts by brute force, thread count irrelevant."""
+    rng = np.random.default_rng(seed)
+    N = int(rng.choice([16, 40, 64]))
+    M = int(rng.integers(50, 400))
+    founders = rng.random((6, M)) < rng.uniform(0.05, 0.5)
+    H = founders[rng.integers(0, 6, size=N)].copy()
+    H ^= rng.random((N, M)) < 0.03
+    H[:, rng.integers(0, M, size=3)] = False  # empty rows
+    H[:, rng.integers(0, M, size=2)] = True   # full rows
+    H[:, 1] = H[:, 0]                          # duplicate rows
+    sets = [np.nonzero(H[:, m])[0].astype(np.uint32) for m in range(M)]
+    off, car = _csr(sets)
+    kw = dict(hap_length=int(rng.choice([3, 8, 31, 254])), rebuild_proportion=float(rng.choice([0.01, 0.1, 2.0])),
+              trees=rng.choice(["optimal", 1, 3]).item() if seed % 2 else "optimal",
+              lf_no_tree=float(rng.choice([0.0, 3.0])))
+    if isinstance(kw["trees"], str) and kw["trees"].isdigit():
+        kw["trees"] = int(kw["trees"])
+    g = _build(N, off, car, **kw)
+    _assert_same_carriers(g, off, car)
+    a = g.arrays()
+    X = dense_x(a)
+    assert np.array_equal(X, H.astype(np.float64)), kw
+    assert np.array_equal(a["coal_counts"][N:].astype(np.int64), _brute_coals(a)[N:]), kw
+    b = _build(N, off, car, threads=1, **kw).arrays()
+    for k in ("down_offsets", "down_children", "mutation_nodes"):
+        assert np.array_equal(a[k], b[k]), (k, kw)
