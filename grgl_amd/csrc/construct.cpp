@@ -1,8 +1,8 @@
 // Mutable GRG and mutation mapping (see construct.h).
 //
-// Mapping one mutation with sample set S onto the graph (the contract of the reference's
-// mapMutations, src/map_mutations.cpp:609-728 and :758-876; the method here is this
-// library's own):
+// Mapping one mutation with sample set S onto the graph.  This is the reference's greedy
+// candidate cover (mapMutations, src/map_mutations.cpp:463-607 for the candidates, :609-728
+// for the cover, :758-876 for batches), found by a different sweep:
 //   1. Coverage sweep.  Every sample of S starts with count 1.  Touched nodes are visited
 //      in increasing ID order (children before parents, since IDs are ordered); a node is
 //      "covered" when its count equals the number of samples below it, and only covered
@@ -10,11 +10,15 @@
 //      exactly when all its children are covered.  Nodes created in the current batch are
 //      not reuse candidates.
 //   2. A covered node with |S| samples is reused as the mutation's node.
-//   3. Otherwise the maximal covered nodes (no covered parent) are taken largest first,
-//      each only if its samples are disjoint from those already taken (two candidates can
-//      share samples in a DAG), and S's remaining samples become direct children of a new
-//      node.  Children of a node therefore have disjoint sample sets: every sample reaches
-//      every node by at most one path, which keeps matmul counts exact.
+//   3. Otherwise the maximal covered nodes (no covered parent) of at least two samples are
+//      taken largest first, each only if its samples are disjoint from those already taken
+//      (two candidates can share samples in a DAG), and S's remaining samples become direct
+//      children of a new node.  The reference also lists a covered node with a covered and an
+//      uncovered parent; its covered parent is larger and taken first, so the cover is the
+//      same.  Equal-size candidates go highest node ID first (the reference's order among
+//      them is unspecified: std::sort by size).  Children of a node therefore have disjoint
+//      sample sets: every sample reaches every node by at most one path, which keeps matmul
+//      counts exact.
 //   4. For diploid graphs the new node records how many individuals have their two
 //      haplotypes under different children (the coalescence count of grg_from_trees,
 //      src/python/_grgl.cpp:1018-1021).
@@ -191,6 +195,7 @@ uint32_t MutableGraph::mapSet(const std::vector<uint32_t>& S, uint32_t limit, Ma
     }
     std::vector<uint32_t> cands;
     for (uint32_t v : coveredNodes) {
+        if (count_[v] < 2) continue; // candidates cover at least two samples (map_mutations.cpp:512)
         bool maximal = true;
         for (uint32_t p : up_[v])
             if (p < limit && stamp_[p] == covered) {
