@@ -19,9 +19,9 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 ARCH = os.environ.get("GRGL_AMD_ARCH", "gfx950")
 
-HIP_SOURCES = ["grgl_hip.hip", "server.cpp", "levelize.cpp", "host_graph.cpp", "stacking.cpp", "stream_ceiling.hip",
+HIP_SOURCES = ["grgl_hip.hip", "server.cpp", "levelize.cpp", "grouping.cpp", "host_graph.cpp", "stacking.cpp", "stream_ceiling.hip",
                "noalloc.hip"]
-PY_SOURCES = ["pygrgl_module.cpp", "synth.cpp", "host_graph.cpp", "levelize.cpp", "stacking.cpp", "grg_io.cpp",
+PY_SOURCES = ["pygrgl_module.cpp", "synth.cpp", "host_graph.cpp", "levelize.cpp", "grouping.cpp", "stacking.cpp", "grg_io.cpp",
               "construct.cpp", "fast_build.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -93,8 +93,11 @@ constexpr size_t kMaxCapturedGraphs = 64;
 // front of each level are heavy, and how each level is cut into launches.  Immutable once
 // built: traversals take a reference-counted snapshot, so retuning a direction swaps in a new
 // table and never changes one that a traversal on another thread is reading.
+// Heavy threshold of a new graph's launch tables (and of its grouped-launch plans).
+constexpr uint32_t kDefaultHeavyFanin = 64;
+
 struct LaunchTable {
-    uint32_t heavyFanin = 64;
+    uint32_t heavyFanin = kDefaultHeavyFanin;
     uint64_t chunkEntries = kChunkEntries; // gather entries per launch at most
     std::vector<uint32_t> heavyCount;      // per level, rows with more than heavyFanin entries
     std::vector<uint64_t> levelEntries;    // per level, gather entries
@@ -103,6 +106,19 @@ struct LaunchTable {
     std::vector<LevelChunk> chunks;        // launches of every level, in level order
     std::vector<uint32_t> levelChunk;      // numLevels + 1 offsets into chunks
 };
+
+// A grouped-launch plan (grouping.h) on the device: its arrays in one allocation.
+struct DeviceGroupPlan {
+    uint32_t slotBytes = 0, heavyFanin = 0, levelBegin = 0, levelEnd = 0, blocks = 0;
+    uint32_t ldsBytes = 0; // LDS of the largest block
+    uint32_t* mem = nullptr;
+    const uint32_t *blockRow = nullptr, *blockSeg = nullptr, *blockHeavy = nullptr, *blockImport = nullptr,
+                   *segRow = nullptr, *segHeavy = nullptr, *segHeavyOrd = nullptr, *row = nullptr, *off = nullptr,
+                   *pre = nullptr, *src = nullptr, *remOff = nullptr, *rem = nullptr, *heavyRow = nullptr,
+                   *importRow = nullptr;
+    bool empty() const { return levelBegin >= levelEnd; }
+};
+constexpr uint32_t kDeviceGroupClasses = 5; // slots of 4, 8, 16, 32, 64 bytes
 
 struct DeviceDirPlan {
     uint32_t numNodes = 0, numLevels = 0, firstPulledLevel = 0, maxFanIn = 0;
@@ -116,6 +132,9 @@ struct DeviceDirPlan {
     uint32_t* outRow = nullptr;
     uint32_t* missRow = nullptr;
     uint32_t* coalByRow = nullptr;
+    // Grouped-launch plans per row class (built with the default heavy threshold; used only
+    // while the launch table has that threshold).  Immutable after graph creation.
+    DeviceGroupPlan groups[kDeviceGroupClasses];
     // Current launch table; replaced (never modified) under grgl_graph::mu.
     std::shared_ptr<const LaunchTable> table;
     std::shared_ptr<const LaunchTable> buildTable(uint32_t heavyFanin, uint64_t chunkEntries) const;
@@ -136,7 +155,8 @@ struct Tuning {
     uint32_t teamLanes = 0; // 0 = default
     uint32_t useGraph = 0;
     uint32_t xcdMap = 0;    // 0 = default (off), 1 = XCD-aware block order, 2 = off
-    uint32_t chain = 0;     // 0 = default, 1 = level-by-level launches, 2 = chained when eligible
+    uint32_t chain = 0;     // 0 = default, 1 = level-by-level launches, 2 = chained when eligible,
+                            // 3 = grouped levels when eligible
     uint32_t blockThreads = 0;  // wide-team level kernels: 0 = default (256), 256 or 512 threads
     uint32_t loadsInFlight = 0; // wide-team level kernels: 0 = default (4), 4 or 8 row loads per batch
 };
@@ -189,8 +209,8 @@ struct grgl_graph {
     uint64_t timedLaunches = 0;
     double timedBytes = 0.0;
     // Captured level sequences: (dir, typeClass, vb, tw, ld, rowBytes, flags and heavy threshold,
-    // rows ptr, launch size)
-    std::map<std::tuple<int, int, uint32_t, uint32_t, uint64_t, uint32_t, int, uintptr_t, uint64_t>,
+    // rows ptr, launch size, level-sequence form)
+    std::map<std::tuple<int, int, uint32_t, uint32_t, uint64_t, uint32_t, int, uintptr_t, uint64_t, uint32_t>,
              std::shared_ptr<grgl_amd::CapturedLevels>>
         graphs;
 };

@@ -12,6 +12,7 @@
 #include <memory>
 
 #include "device_graph.h"
+#include "grouping.h"
 #include "kernels.h"
 
 #define GRGL_EXPORT extern "C" __attribute__((visibility("default")))
@@ -283,6 +284,35 @@ template <typename T, bool XOR> const void* tailKernelT(uint32_t vb, uint32_t tw
     }
 #undef GRGL_TAIL
     throw std::logic_error("no tail kernel for this access width");
+}
+
+// Grouped-level kernel for slots of 4 << cls bytes (a whole row per lane), or null when a
+// row of that size cannot hold one element of the type.
+template <typename T, bool XOR> const void* groupKernelT(uint32_t cls) {
+#define GRGL_GROUP(RB) return reinterpret_cast<const void*>(&pull_group_kernel<T, RB, XOR>)
+    switch (cls) {
+    case 0:
+        if constexpr (sizeof(T) <= 4) GRGL_GROUP(4);
+        return nullptr;
+    case 1: GRGL_GROUP(8);
+    case 2: GRGL_GROUP(16);
+    case 3: GRGL_GROUP(32);
+    default: GRGL_GROUP(64);
+    }
+#undef GRGL_GROUP
+}
+
+const void* groupKernel(TypeClass tc, uint32_t cls) {
+    switch (tc) {
+    case kF64: return groupKernelT<double, false>(cls);
+    case kF32: return groupKernelT<float, false>(cls);
+    case kW64: return groupKernelT<uint64_t, false>(cls);
+    case kW32: return groupKernelT<uint32_t, false>(cls);
+    case kW16: return groupKernelT<uint16_t, false>(cls);
+    case kW8: return groupKernelT<uint8_t, false>(cls);
+    case kX8: return groupKernelT<uint8_t, true>(cls);
+    }
+    throw std::logic_error("bad type class");
 }
 
 const void* tailKernel(TypeClass tc, uint32_t vb, uint32_t tw) {
@@ -665,12 +695,40 @@ struct LevelLaunch {
     const void* func = nullptr;
     dim3 grid;
     uint32_t block = kBlock;
+    uint32_t shmem = 0;  // dynamic LDS bytes
     bool tail = false;
+    bool group = false;  // a grouped-levels launch (GroupArgs)
     PullArgs pull{};
     TailArgs tailArgs{};
+    GroupArgs groupArgs{};
     double bytes = 0.0;
-    void* argPtr() { return tail ? (void*)&tailArgs : (void*)&pull; }
+    void* argPtr() { return tail ? (void*)&tailArgs : (group ? (void*)&groupArgs : (void*)&pull); }
+    const char* what() const { return tail ? "tail level" : (group ? "grouped levels" : "level"); }
 };
+
+// Grouped levels replace the per-level launches of their levels unless a traversal asks for
+// one launch per level (chain = 1) or a chained launch (chain = 2).
+constexpr bool kGroupByDefault = false;
+
+// Slot class of a row of `rowBytes` (at most 64): slots of 4 << class bytes.
+uint32_t groupClassOf(uint32_t rowBytes) {
+    uint32_t c = 0;
+    while ((4U << c) < rowBytes) c++;
+    return c;
+}
+
+// The grouped-launch plan a traversal takes, or null: rows of at most 64 bytes, the launch
+// table's heavy threshold equal to the plan's, not order-preserving (whose lists are all
+// light) and no per-launch timing.
+const DeviceGroupPlan* groupFor(const DeviceDirPlan& plan, const LaunchTable& tab, TypeClass tc,
+                                const RowGeometry& geo, const Tuning& t, bool orderPreserving, bool timing) {
+    const bool want = t.chain == 3 || (t.chain == 0 && kGroupByDefault);
+    if (!want || timing || orderPreserving || geo.rowBytes > 64) return nullptr;
+    const uint32_t cls = groupClassOf(geo.rowBytes);
+    const DeviceGroupPlan& gp = plan.groups[cls];
+    if (gp.empty() || gp.heavyFanin != tab.heavyFanin || !groupKernel(tc, cls)) return nullptr;
+    return &gp;
+}
 
 // The launches of every pulled level of one direction, in order (the trailing small levels as
 // one tail launch).  With `timing`, every level gets a launch of its own and its bytes.
@@ -688,8 +746,40 @@ std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTa
     // The single-block tail gains nothing from spreading rows over idle lanes: rows of one
     // access take single-lane teams there (256 rows per pass instead of 256 / tw).
     const uint32_t tailTw = geo.lanes == 1 ? 1 : tw;
-    const uint32_t tailFrom = timing ? plan.numLevels : tailStart(plan, tab, geo, tailTw, orderPreserving);
+    const DeviceGroupPlan* gp = groupFor(plan, tab, tc, geo, tuning, orderPreserving, timing);
+    const uint32_t tailFrom = (timing || gp) ? plan.numLevels : tailStart(plan, tab, geo, tailTw, orderPreserving);
     for (uint32_t l = plan.firstPulledLevel; l < tailFrom; l++) {
+      if (gp && l >= gp->levelBegin && l < gp->levelEnd) {
+        if (l != gp->levelBegin) continue;
+        LevelLaunch L;
+        L.group = true;
+        L.grid = dim3(gp->blocks);
+        L.block = kBlock;
+        L.shmem = gp->ldsBytes;
+        GroupArgs& a = L.groupArgs;
+        a.rows = rows;
+        a.ld = geo.ld;
+        a.blockRow = gp->blockRow;
+        a.blockSeg = gp->blockSeg;
+        a.blockHeavy = gp->blockHeavy;
+        a.blockImport = gp->blockImport;
+        a.segRow = gp->segRow;
+        a.segHeavy = gp->segHeavy;
+        a.segHeavyOrd = gp->segHeavyOrd;
+        a.row = gp->row;
+        a.off = gp->off;
+        a.pre = gp->pre;
+        a.src = gp->src;
+        a.remOff = gp->remOff;
+        a.rem = gp->rem;
+        a.heavyRow = gp->heavyRow;
+        a.importRow = gp->importRow;
+        a.heavyFanin = gp->heavyFanin;
+        a.readSelf = readSelf ? 1 : 0;
+        L.func = groupKernel(tc, groupClassOf(geo.rowBytes));
+        out.push_back(L);
+        continue;
+      }
       for (uint32_t ci = tab.levelChunk[l]; ci < tab.levelChunk[l + 1]; ci++) {
         const LevelChunk& ck = tab.chunks[ci];
         const uint32_t begin = ck.rowBegin, end = ck.rowEnd;
@@ -767,9 +857,9 @@ void launchDirect(grgl_graph* g, std::vector<LevelLaunch>& launches, hipStream_t
             GRGL_HIP_CHECK(hipEventCreate(&ev1));
             GRGL_HIP_CHECK(hipEventRecord(ev0, stream));
         }
-        checkLaunch(L.grid, L.block, L.tail ? "tail level" : "level");
+        checkLaunch(L.grid, L.block, L.what());
         void* args[1] = {L.argPtr()};
-        GRGL_HIP_CHECK(hipLaunchKernel(L.func, L.grid, dim3(L.block), args, 0, stream));
+        GRGL_HIP_CHECK(hipLaunchKernel(L.func, L.grid, dim3(L.block), args, L.shmem, stream));
         if (timing) {
             GRGL_HIP_CHECK(hipEventRecord(ev1, stream));
             std::lock_guard<std::mutex> lk(g->mu);
@@ -785,13 +875,13 @@ void buildLevelGraph(std::vector<LevelLaunch>& launches, CapturedLevels& out) {
     GRGL_HIP_CHECK(hipGraphCreate(&out.graph, 0));
     hipGraphNode_t prev = nullptr;
     for (LevelLaunch& L : launches) {
-        checkLaunch(L.grid, L.block, L.tail ? "tail level" : "level");
+        checkLaunch(L.grid, L.block, L.what());
         hipKernelNodeParams p{};
         void* args[1] = {L.argPtr()};
         p.func = const_cast<void*>(L.func);
         p.gridDim = L.grid;
         p.blockDim = dim3(L.block);
-        p.sharedMemBytes = 0;
+        p.sharedMemBytes = L.shmem;
         p.kernelParams = args;
         p.extra = nullptr;
         hipGraphNode_t node = nullptr;
@@ -890,7 +980,7 @@ void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometr
                                      (int)readSelf * 2 + (int)orderPreserving + 4 * (int)(t.xcdMap == 1) +
                                          8 * (int)(t.blockThreads == 512) + 16 * (int)(t.loadsInFlight == 8) +
                                          32 * (int)tab->heavyFanin,
-                                     (uintptr_t)rows, tab->chunkEntries);
+                                     (uintptr_t)rows, tab->chunkEntries, t.chain);
     std::shared_ptr<CapturedLevels> cap;
     {
         std::lock_guard<std::mutex> lk(g->mu);
@@ -1203,7 +1293,66 @@ void uploadPlan(DeviceDirPlan& d, const DirPlan& p, uint64_t& bytes) {
     d.outRow = uploadVec(p.outRow, bytes);
     d.missRow = uploadVec(p.missRow, bytes);
     d.coalByRow = uploadVec(p.coalByRow, bytes);
-    d.table = d.buildTable(64, kChunkEntries);
+    d.table = d.buildTable(kDefaultHeavyFanin, kChunkEntries);
+}
+
+// Grouped-launch plans of both directions (built on two host threads).
+void buildGroups(const LevelizedGraph& lg, std::vector<GroupPlan>& up, std::vector<GroupPlan>& down) {
+    const GroupLimits lim;
+    std::exception_ptr err;
+    std::thread t([&] {
+        try {
+            down = buildGroupPlans(lg.down, groupRegionEnd(lg.down, true), kDefaultHeavyFanin, lim);
+        } catch (...) {
+            err = std::current_exception();
+        }
+    });
+    up = buildGroupPlans(lg.up, groupRegionEnd(lg.up, false), kDefaultHeavyFanin, lim);
+    t.join();
+    if (err) std::rethrow_exception(err);
+}
+
+// The arrays of a grouped-launch plan, in the order of its device allocation.
+std::vector<const std::vector<uint32_t>*> groupParts(const GroupPlan& gp) {
+    return {&gp.blockRow, &gp.blockSeg, &gp.blockHeavy, &gp.blockImport, &gp.segRow,
+            &gp.segHeavy, &gp.segHeavyOrd, &gp.row, &gp.off, &gp.pre,
+            &gp.src, &gp.remOff, &gp.rem, &gp.heavyRow, &gp.importRow};
+}
+
+// Device bytes of a grouped-launch plan (each array padded to 16 bytes).
+uint64_t groupDeviceBytes(const GroupPlan& gp) {
+    if (gp.empty()) return 0;
+    uint64_t words = 0;
+    for (const auto* v : groupParts(gp)) words += (v->size() + 3) / 4 * 4;
+    return 4 * words;
+}
+
+void uploadGroups(DeviceDirPlan& d, const std::vector<GroupPlan>& plans, uint64_t& bytes) {
+    for (uint32_t c = 0; c < kDeviceGroupClasses && c < plans.size(); c++) {
+        const GroupPlan& gp = plans[c];
+        DeviceGroupPlan& dg = d.groups[c];
+        dg = DeviceGroupPlan();
+        dg.slotBytes = gp.slotBytes;
+        dg.heavyFanin = gp.heavyFanin;
+        if (gp.empty()) continue;
+        const std::vector<const std::vector<uint32_t>*> parts = groupParts(gp);
+        std::vector<uint32_t> flat;
+        std::vector<size_t> at;
+        for (const auto* v : parts) {
+            at.push_back(flat.size());
+            flat.insert(flat.end(), v->begin(), v->end());
+            flat.resize((flat.size() + 3) / 4 * 4); // 16-byte aligned parts
+        }
+        dg.mem = uploadVec(flat, bytes);
+        const uint32_t** dst[] = {&dg.blockRow, &dg.blockSeg, &dg.blockHeavy, &dg.blockImport, &dg.segRow,
+                                  &dg.segHeavy, &dg.segHeavyOrd, &dg.row, &dg.off, &dg.pre, &dg.src, &dg.remOff,
+                                  &dg.rem, &dg.heavyRow, &dg.importRow};
+        for (size_t i = 0; i < parts.size(); i++) *dst[i] = dg.mem + at[i];
+        dg.levelBegin = gp.levelBegin;
+        dg.levelEnd = gp.levelEnd;
+        dg.blocks = gp.blocks;
+        dg.ldsBytes = gp.ldsBytes;
+    }
 }
 
 void freePlan(DeviceDirPlan& d) {
@@ -1214,6 +1363,7 @@ void freePlan(DeviceDirPlan& d) {
     freeDev(d.outRow);
     freeDev(d.missRow);
     freeDev(d.coalByRow);
+    for (auto& dg : d.groups) freeDev(dg.mem);
     d = DeviceDirPlan();
 }
 
@@ -1259,8 +1409,12 @@ std::unique_ptr<grgl_graph> makeDeviceGraph(LevelizedGraph lg, int device) {
     g->hasMiss = lg.hasMiss;
     g->hasCoal = lg.hasCoal;
     try {
+        std::vector<GroupPlan> upGroups, downGroups;
+        buildGroups(lg, upGroups, downGroups);
         uploadPlan(g->up, lg.up, g->deviceBytes);
         uploadPlan(g->down, lg.down, g->deviceBytes);
+        uploadGroups(g->up, upGroups, g->deviceBytes);
+        uploadGroups(g->down, downGroups, g->deviceBytes);
         void* ctl = nullptr;
         GRGL_HIP_CHECK(hipHostMalloc(&ctl, sizeof(ChainControl), hipHostMallocMapped));
         g->chainCtl = static_cast<ChainControl*>(ctl);
@@ -1405,6 +1559,10 @@ GRGL_EXPORT grgl_status grgl_hip_levelized_file_info(const char* path, grgl_grap
     for (const DirPlan* p : {&lg.up, &lg.down})
         bytes += 4 * (p->off.size() + p->src.size() + p->order.size() + p->rank.size() + p->outRow.size() +
                       p->missRow.size() + p->coalByRow.size());
+    std::vector<GroupPlan> upGroups, downGroups;
+    buildGroups(lg, upGroups, downGroups);
+    for (const auto* plans : {&upGroups, &downGroups})
+        for (const GroupPlan& gp : *plans) bytes += groupDeviceBytes(gp);
     info->device_bytes = bytes;
     info->device = -1;
     GRGL_API_END
@@ -1739,7 +1897,7 @@ GRGL_EXPORT grgl_status grgl_hip_set_tuning(grgl_graph* g, grgl_direction direct
     t.useGraph = tuning->use_graph;
     if (tuning->xcd_map > 2) throw MisuseError("xcd_map must be 0, 1 or 2");
     t.xcdMap = tuning->xcd_map;
-    if (tuning->chain > 2) throw MisuseError("chain must be 0, 1 or 2");
+    if (tuning->chain > 3) throw MisuseError("chain must be 0, 1, 2 or 3");
     t.chain = tuning->chain;
     if (tuning->block_threads != 0 && tuning->block_threads != 256 && tuning->block_threads != 512)
         throw MisuseError("block_threads must be 0, 256 or 512");
@@ -1880,7 +2038,7 @@ GRGL_EXPORT grgl_status grgl_hip_tuning_load(grgl_graph* g, const char* path) {
             uint64_t rows = 0, tw = 0, ug = 0, xcd = 0, chain = 0, blk = 0, unr = 0;
             if (!(f >> dir >> tc >> rows >> tw >> ug >> xcd) || (version >= 2 && !(f >> chain)) ||
                 (version >= 3 && !(f >> blk >> unr)) || (dir != GRGL_DIRECTION_UP && dir != GRGL_DIRECTION_DOWN) ||
-                tc < 0 || tc > (int)kX8 || rows == 0 || rows > 65536 || tw > 64 || ug > 1 || xcd > 2 || chain > 2 ||
+                tc < 0 || tc > (int)kX8 || rows == 0 || rows > 65536 || tw > 64 || ug > 1 || xcd > 2 || chain > 3 ||
                 (blk != 0 && blk != 256 && blk != 512) || (unr != 0 && unr != 4 && unr != 8))
                 throw MisuseError("malformed tuning file (entry)");
             Tuning t;
@@ -1978,8 +2136,12 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
         GRGL_HIP_CHECK(hipMemsetAsync(ws->ptr, 0, rowsBufferBytes(g, direction, rows, es), s));
         // Candidates: every team width with direct or graph launches and both block orders
         // (the XCD-aware order only where it differs), and, for rows of at most 64 bytes, the
-        // chained single launch at every team width.
+        // chained single launch and the grouped levels at every team width.
         std::vector<Tuning> cands;
+        Tuning grouped;
+        grouped.chain = 3;
+        const bool groupable = groupFor(direction == GRGL_DIRECTION_UP ? g->up : g->down, *launchTable(g, direction), tc,
+                                        geo, grouped, false, false) != nullptr;
         for (uint32_t tw : tws) {
             for (uint32_t useGraph = 0; useGraph < 2; useGraph++) {
                 for (uint32_t xcd = 1; xcd <= 2; xcd++) {
@@ -1998,6 +2160,14 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
                 t.xcdMap = 1;
                 t.chain = 2;
                 cands.push_back(t);
+                // Grouped levels (the other levels at this team width), direct or graph launches.
+                if (groupable) {
+                    for (uint32_t useGraph = 0; useGraph < 2; useGraph++) {
+                        t.chain = 3;
+                        t.useGraph = useGraph;
+                        cands.push_back(t);
+                    }
+                }
             }
             // Wide-team shapes: 512-thread blocks and 8 loads in flight, direct launches.
             if (tileTunable(levelVb(geo, tw, es), tw) && tc != kW16 && tc != kW8 && tc != kX8) {

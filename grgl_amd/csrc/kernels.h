@@ -50,6 +50,9 @@ template <> struct RawVec<16> { using type = uint4; };
 // two 16-byte accesses.
 struct Raw32 { uint4 a, b; };
 template <> struct RawVec<32> { using type = Raw32; };
+// A whole 64-byte row held by one lane (grouped levels): four 16-byte accesses.
+struct Raw64 { uint4 a, b, c, d; };
+template <> struct RawVec<64> { using type = Raw64; };
 
 template <typename T, int VB> struct Pack {
     static constexpr int N = VB / (int)sizeof(T);
@@ -477,6 +480,175 @@ __global__ __launch_bounds__(kBlock) void pull_tail_kernel(TailArgs t) {
         }
         __syncthreads(); // the next level reads rows written by other waves of this block
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// Grouped levels (grouping.h): block b runs its groups through levels [levelBegin, levelEnd)
+// by itself, one segment (one level of its rows) after another with a block barrier in
+// between; no block waits for another.  Each lane holds a whole row of RB bytes (rows of at
+// most 64 bytes).  The block keeps its rows' values in LDS slots.
+//   Phase 0 loads everything the levels need, every load of the block independent of the
+//   others: the row offsets of the block's slot lists, the slot lists, its segment table and
+//   heavy rows' prefix lengths; the import slots (HBM rows that a list reads after its first
+//   in-group entry); and each row's leading entries, which read HBM rows (earlier levels or
+//   seed rows, complete before the launch), summed: a light row into its slot, starting from
+//   the row's own value with node initialisation; a heavy row (longer than the heavy
+//   threshold) into 64 partial slots, entry i to partial (i mod 64) as in pull_heavy.
+//   Phase 1 continues each row's sum over its slot list, segment by segment, reading LDS
+//   only: light rows one per lane; heavy rows one per wavefront, lane q finishing partial q,
+//   then the fixed tree (strides 32..1) on lane shuffles and the row's own value added last,
+//   as in pull_heavy.  So every row is the same sequence of adds as in the level kernels.
+//   Rows are written to HBM once, after the last segment, so the barriers wait for no store
+//   to memory.
+// LDS of a block: value slots (its rows, 64 partials per heavy row, its imports) of RB bytes,
+// 16-byte aligned, then 32-bit words: row offsets into the slot lists (rows + 1), the slot
+// lists, 3 words per segment (first row, heavy rows, ordinal of its first heavy row) and
+// each heavy row's prefix length (the layout groupLdsBytes counts).
+struct GroupArgs {
+    char* rows;
+    uint64_t ld;
+    const uint32_t* blockRow;    // blocks + 1: first group row of each block
+    const uint32_t* blockSeg;    // blocks + 1: first segment of each block
+    const uint32_t* blockHeavy;  // blocks + 1: first entry of heavyRow of each block
+    const uint32_t* blockImport; // blocks + 1: first entry of importRow of each block
+    const uint32_t* segRow;      // segments + 1: first group row of each segment
+    const uint32_t* segHeavy;    // per segment: heavy rows at its front
+    const uint32_t* segHeavyOrd; // per segment: block-relative ordinal of its first heavy row
+    const uint32_t* row;         // per group row: the row it computes
+    const uint32_t* off;         // group rows + 1: list bounds in src
+    const uint32_t* pre;         // per group row: leading entries that read HBM rows
+    const uint32_t* src;         // row indices (in-group ones tagged; only the prefix is read)
+    const uint32_t* remOff;      // group rows + 1: slot list bounds in rem
+    const uint32_t* rem;         // slot lists (block-relative slots)
+    const uint32_t* heavyRow;    // heavy group rows of each block, in segment order
+    const uint32_t* importRow;   // HBM row of each import slot, block after block
+    uint32_t heavyFanin;
+    uint32_t readSelf;
+};
+
+// acc += rows[src[i]], rows[src[i + step]], ... below e, in that order, with UNR loads issued
+// before the adds of each batch; the last, partial batch loads clamped (in-list) entries and
+// adds only the real ones.
+template <typename T, int RB, bool XOR, int UNR>
+__device__ __forceinline__ void group_prefix_sum(const GroupArgs& a, Pack<T, RB>& acc, uint32_t i, uint32_t e,
+                                                 uint32_t step) {
+    using P = Pack<T, RB>;
+    for (; i + (uint32_t)(UNR - 1) * step < e; i += (uint32_t)UNR * step) {
+        P x[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) x[u] = load_pack<T, RB>(a.rows + (uint64_t)a.src[i + (uint32_t)u * step] * a.ld);
+#pragma unroll
+        for (int u = 0; u < UNR; u++) acc_pack<T, RB, XOR>(acc, x[u]);
+    }
+    if (i < e) {
+        P x[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const uint32_t j = i + (uint32_t)u * step;
+            x[u] = load_pack<T, RB>(a.rows + (uint64_t)a.src[j < e ? j : i] * a.ld);
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; u++)
+            if (i + (uint32_t)u * step < e) acc_pack<T, RB, XOR>(acc, x[u]);
+    }
+}
+
+// acc += slot[list[j]], slot[list[j + step]], ... below e, in that order (LDS only).
+template <typename T, int RB, bool XOR>
+__device__ __forceinline__ void group_slot_sum(const char* slots, const uint32_t* list, Pack<T, RB>& acc, uint32_t j,
+                                               uint32_t e, uint32_t step) {
+    using P = Pack<T, RB>;
+    constexpr int U = 4;
+    for (; j + (uint32_t)(U - 1) * step < e; j += (uint32_t)U * step) {
+        P x[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) x[u] = load_pack<T, RB>(slots + (size_t)list[j + (uint32_t)u * step] * RB);
+#pragma unroll
+        for (int u = 0; u < U; u++) acc_pack<T, RB, XOR>(acc, x[u]);
+    }
+    for (; j < e; j += step) acc_pack<T, RB, XOR>(acc, load_pack<T, RB>(slots + (size_t)list[j] * RB));
+}
+
+template <typename T, int RB, bool XOR>
+__global__ __launch_bounds__(kBlock) void pull_group_kernel(GroupArgs a) {
+    extern __shared__ __align__(16) char slots[];
+    using P = Pack<T, RB>;
+    constexpr int UNR = RB <= 16 ? 8 : (RB == 32 ? 4 : 2);
+    constexpr uint32_t kWaves = kBlock / 64;
+    const uint32_t b = blockIdx.x;
+    const uint32_t r0 = a.blockRow[b], r1 = a.blockRow[b + 1];
+    const uint32_t h0 = a.blockHeavy[b], h1 = a.blockHeavy[b + 1];
+    const uint32_t s0 = a.blockSeg[b], s1 = a.blockSeg[b + 1];
+    const uint32_t i0 = a.blockImport[b], i1 = a.blockImport[b + 1];
+    const uint32_t nRows = r1 - r0, nHeavy = h1 - h0, nSeg = s1 - s0, nImport = i1 - i0;
+    const uint32_t remBase = a.remOff[r0], nRem = a.remOff[r1] - remBase;
+    const uint32_t importBase = nRows + 64 * nHeavy;
+    const uint32_t valueBytes = ((importBase + nImport) * RB + 15) / 16 * 16;
+    uint32_t* remOffL = reinterpret_cast<uint32_t*>(slots + valueBytes);
+    uint32_t* remL = remOffL + nRows + 1;
+    uint32_t* segL = remL + nRem;
+    uint32_t* heavyPreL = segL + 3 * nSeg;
+    const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    // Phase 0.
+    for (uint32_t t = threadIdx.x; t <= nRows; t += kBlock) remOffL[t] = a.remOff[r0 + t] - remBase;
+    for (uint32_t t = threadIdx.x; t < nRem; t += kBlock) remL[t] = a.rem[remBase + t];
+    for (uint32_t t = threadIdx.x; t < nSeg; t += kBlock) {
+        segL[3 * t] = a.segRow[s0 + t] - r0;
+        segL[3 * t + 1] = a.segHeavy[s0 + t];
+        segL[3 * t + 2] = a.segHeavyOrd[s0 + t];
+    }
+    for (uint32_t t = threadIdx.x; t < nHeavy; t += kBlock) heavyPreL[t] = a.pre[a.heavyRow[h0 + t]];
+    for (uint32_t t = threadIdx.x; t < nImport; t += kBlock)
+        store_pack<T, RB>(slots + (size_t)(importBase + t) * RB,
+                          load_pack<T, RB>(a.rows + (uint64_t)a.importRow[i0 + t] * a.ld));
+    for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+        const uint32_t lb = a.off[i], le = a.off[i + 1];
+        if (le - lb > a.heavyFanin) continue;
+        P acc = a.readSelf ? load_pack<T, RB>(a.rows + (uint64_t)a.row[i] * a.ld) : zero_pack<T, RB>();
+        group_prefix_sum<T, RB, XOR, UNR>(a, acc, lb, lb + a.pre[i], 1);
+        store_pack<T, RB>(slots + (size_t)(i - r0) * RB, acc);
+    }
+    for (uint32_t k = wave; k < nHeavy; k += kWaves) {
+        const uint32_t i = a.heavyRow[h0 + k];
+        const uint32_t lb = a.off[i];
+        P acc = zero_pack<T, RB>();
+        group_prefix_sum<T, RB, XOR, UNR>(a, acc, lb + lane, lb + a.pre[i], 64);
+        store_pack<T, RB>(slots + (size_t)(nRows + 64 * k + lane) * RB, acc);
+    }
+    __syncthreads();
+    // Phase 1.
+    for (uint32_t s = 0; s < nSeg; s++) {
+        const uint32_t g0 = segL[3 * s], g1 = s + 1 < nSeg ? segL[3 * (s + 1)] : nRows;
+        const uint32_t hc = segL[3 * s + 1], hord = segL[3 * s + 2];
+        for (uint32_t h = wave; h < hc; h += kWaves) { // wave-uniform
+            const uint32_t i = g0 + h, k = hord + h;
+            P acc = load_pack<T, RB>(slots + (size_t)(nRows + 64 * k + lane) * RB);
+            // This partial's slot-list entries: list index pre + j with (pre + j) mod 64 == lane.
+            const uint32_t j0 = (lane - heavyPreL[k]) & 63U;
+            group_slot_sum<T, RB, XOR>(slots, remL, acc, remOffL[i] + j0, remOffL[i + 1], 64);
+#pragma unroll
+            for (int st = kHeavyPartials / 2; st >= 1; st /= 2) {
+                const P other = shfl_down_pack<T, RB>(acc, (unsigned)st);
+                if (lane < (uint32_t)st) acc_pack<T, RB, XOR>(acc, other);
+            }
+            if (lane == 0) {
+                if (a.readSelf) {
+                    P self = load_pack<T, RB>(a.rows + (uint64_t)a.row[r0 + i] * a.ld);
+                    acc_pack<T, RB, XOR>(self, acc);
+                    acc = self;
+                }
+                store_pack<T, RB>(slots + (size_t)i * RB, acc);
+            }
+        }
+        for (uint32_t i = g0 + hc + threadIdx.x; i < g1; i += kBlock) {
+            P acc = load_pack<T, RB>(slots + (size_t)i * RB);
+            group_slot_sum<T, RB, XOR>(slots, remL, acc, remOffL[i], remOffL[i + 1], 1);
+            store_pack<T, RB>(slots + (size_t)i * RB, acc);
+        }
+        __syncthreads();
+    }
+    for (uint32_t i = threadIdx.x; i < nRows; i += kBlock)
+        store_pack<T, RB>(a.rows + (uint64_t)a.row[r0 + i] * a.ld, load_pack<T, RB>(slots + (size_t)i * RB));
 }
 
 // ---------------------------------------------------------------------------------------

@@ -131,16 +131,28 @@ DirPlan buildDownPlan(const HostGrg& g) {
     const uint32_t V = g.numNodes(), N = g.numSamples, M = g.numMutations();
     p.numNodes = V;
     p.firstPulledLevel = 0;
-    // Depth from the roots: a node's row needs every parent's row first.
+    // Depth from the roots: a node's row needs every parent's row first.  Nodes without
+    // children (the samples, mostly) are read by no other row, so they all go to one last
+    // level after every other node: the nodes above them then form many small connected
+    // groups (a sample joins the trees of all its parents), which the grouped launches
+    // traverse level after level inside one block, and the samples make one large level.
+    // A row's gather list does not depend on its level, so neither do its bits.
     std::vector<uint32_t> d(V, 0);
-    uint32_t maxD = 0;
+    uint32_t maxInner = 0;
+    bool anyInner = false;
     for (uint32_t v = V; v-- > 0;) {
         for (uint64_t i = g.downOff[v]; i < g.downOff[v + 1]; i++) {
             const uint32_t c = g.downChild[i];
             d[c] = std::max(d[c], d[v] + 1);
         }
-        maxD = std::max(maxD, d[v]);
+        if (g.downOff[v + 1] > g.downOff[v]) {
+            maxInner = std::max(maxInner, d[v]);
+            anyInner = true;
+        }
     }
+    const uint32_t maxD = anyInner ? maxInner + 1 : 0;
+    for (uint32_t v = 0; v < V; v++)
+        if (g.downOff[v + 1] == g.downOff[v]) d[v] = maxD;
     // Gather lists in node space: seeds (tuple order) then parents (descending ID).
     std::vector<uint64_t> cnt(V, 0);
     const bool hasMiss = !g.mutMiss.empty();

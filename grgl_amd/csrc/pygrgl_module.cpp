@@ -21,6 +21,7 @@
 #include "construct.h"
 #include "fast_build.h"
 #include "host_graph.h"
+#include "grouping.h"
 #include "levelize.h"
 #include "stacking.h"
 
@@ -900,6 +901,50 @@ PYBIND11_MODULE(_grgl_amd, m) {
         },
         py::arg("grg"), py::arg("direction"),
         "The level-bucketed gather plan the device library builds for one direction (for inspection).");
+
+    m.def(
+        "_group_plans",
+        [](PyGRGPtr& g, Direction dir, uint32_t heavyFanin, uint32_t ldsBytes, uint32_t maxEntries, uint32_t maxBlocks) {
+            const DirPlan p = dir == Direction::UP ? buildUpPlan(g->H()) : buildDownPlan(g->H());
+            GroupLimits lim;
+            lim.ldsBytes = ldsBytes;
+            lim.maxEntries = maxEntries;
+            lim.maxBlocks = maxBlocks;
+            const uint32_t regionEnd = groupRegionEnd(p, dir == Direction::DOWN);
+            std::vector<GroupPlan> plans = buildGroupPlans(p, regionEnd, heavyFanin, lim);
+            py::list out;
+            for (const GroupPlan& gp : plans) {
+                validateGroupPlan(p, gp, lim);
+                py::dict d;
+                d["slot_bytes"] = gp.slotBytes;
+                d["heavy_fanin"] = gp.heavyFanin;
+                d["level_begin"] = gp.levelBegin;
+                d["level_end"] = gp.levelEnd;
+                d["blocks"] = gp.blocks;
+                d["lds_bytes"] = gp.ldsBytes;
+                d["block_row"] = toArray(gp.blockRow);
+                d["block_seg"] = toArray(gp.blockSeg);
+                d["block_heavy"] = toArray(gp.blockHeavy);
+                d["block_import"] = toArray(gp.blockImport);
+                d["seg_row"] = toArray(gp.segRow);
+                d["seg_heavy"] = toArray(gp.segHeavy);
+                d["seg_heavy_ord"] = toArray(gp.segHeavyOrd);
+                d["row"] = toArray(gp.row);
+                d["off"] = toArray(gp.off);
+                d["pre"] = toArray(gp.pre);
+                d["src"] = toArray(gp.src);
+                d["rem_off"] = toArray(gp.remOff);
+                d["rem"] = toArray(gp.rem);
+                d["heavy_row"] = toArray(gp.heavyRow);
+                d["import_row"] = toArray(gp.importRow);
+                out.append(d);
+            }
+            return out;
+        },
+        py::arg("grg"), py::arg("direction"), py::arg("heavy_fanin") = 64, py::arg("lds_bytes") = 65536,
+        py::arg("max_entries") = 16384, py::arg("max_blocks") = 2048,
+        "The grouped-launch plans (one per row class: 4, 8, 16, 32, 64-byte slots) the device library "
+        "builds for one direction, each validated (for inspection).");
 
     m.def(
         "_stack_offsets",
