@@ -286,10 +286,10 @@ template <typename T, bool XOR> const void* tailKernelT(uint32_t vb, uint32_t tw
     throw std::logic_error("no tail kernel for this access width");
 }
 
-// Grouped-level kernel for slots of 4 << cls bytes (a whole row per lane), or null when a
-// row of that size cannot hold one element of the type.
-template <typename T, bool XOR> const void* groupKernelT(uint32_t cls) {
-#define GRGL_GROUP(RB) return reinterpret_cast<const void*>(&pull_group_kernel<T, RB, XOR>)
+// Grouped-level kernel for slots of 4 << cls bytes (a whole row per lane) and blocks of `blk`
+// threads (256 or 1024), or null when a row of that size cannot hold one element of the type.
+template <typename T, bool XOR, int BLK> const void* groupKernelB(uint32_t cls) {
+#define GRGL_GROUP(RB) return reinterpret_cast<const void*>(&pull_group_kernel<T, RB, XOR, BLK>)
     switch (cls) {
     case 0:
         if constexpr (sizeof(T) <= 4) GRGL_GROUP(4);
@@ -302,15 +302,24 @@ template <typename T, bool XOR> const void* groupKernelT(uint32_t cls) {
 #undef GRGL_GROUP
 }
 
-const void* groupKernel(TypeClass tc, uint32_t cls) {
+template <typename T, bool XOR> const void* groupKernelT(uint32_t cls, uint32_t blk) {
+    if (blk == 256) return groupKernelB<T, XOR, 256>(cls);
+    // Wrapping 8-bit adds on 64-byte rows need more registers than 1024-thread blocks leave.
+    if constexpr (sizeof(T) == 1 && !XOR) {
+        if (cls >= 4) return nullptr;
+    }
+    return groupKernelB<T, XOR, 1024>(cls);
+}
+
+const void* groupKernel(TypeClass tc, uint32_t cls, uint32_t blk) {
     switch (tc) {
-    case kF64: return groupKernelT<double, false>(cls);
-    case kF32: return groupKernelT<float, false>(cls);
-    case kW64: return groupKernelT<uint64_t, false>(cls);
-    case kW32: return groupKernelT<uint32_t, false>(cls);
-    case kW16: return groupKernelT<uint16_t, false>(cls);
-    case kW8: return groupKernelT<uint8_t, false>(cls);
-    case kX8: return groupKernelT<uint8_t, true>(cls);
+    case kF64: return groupKernelT<double, false>(cls, blk);
+    case kF32: return groupKernelT<float, false>(cls, blk);
+    case kW64: return groupKernelT<uint64_t, false>(cls, blk);
+    case kW32: return groupKernelT<uint32_t, false>(cls, blk);
+    case kW16: return groupKernelT<uint16_t, false>(cls, blk);
+    case kW8: return groupKernelT<uint8_t, false>(cls, blk);
+    case kX8: return groupKernelT<uint8_t, true>(cls, blk);
     }
     throw std::logic_error("bad type class");
 }
@@ -420,7 +429,7 @@ uint32_t residentBlocks(int device, const void* func) {
 // malformed dispatch reaches the device.
 void checkLaunch(dim3 grid, uint32_t block, const char* what) {
     const bool ok = grid.x >= 1 && grid.y >= 1 && grid.z == 1 && grid.x <= 0x7fffffffU && grid.y <= 65535U &&
-                    (block == 256 || block == 512);
+                    (block == 256 || block == 512 || block == 1024);
     if (!ok)
         throw DeviceError(std::string("internal error: malformed ") + what + " launch (grid " + std::to_string(grid.x) +
                           " x " + std::to_string(grid.y) + " x " + std::to_string(grid.z) + ", block " +
@@ -708,7 +717,16 @@ struct LevelLaunch {
 
 // Grouped levels replace the per-level launches of their levels unless a traversal asks for
 // one launch per level (chain = 1) or a chained launch (chain = 2).
-constexpr bool kGroupByDefault = false;
+constexpr bool kGroupByDefault = true;
+
+// Threads per block of a grouped launch: block_threads 256 or 1024 when the tuning names one,
+// else 1024 for slots of 4 and 8 bytes (the largest groups, whose rows then spread over more
+// lanes in phase 0 and per level) and 256 for wider ones (more blocks resident); 256 where the
+// kernel has no 1024-thread form.
+uint32_t groupBlockThreads(const Tuning& t, TypeClass tc, uint32_t cls) {
+    const uint32_t want = (t.blockThreads == 256 || t.blockThreads == 1024) ? t.blockThreads : (cls <= 1 ? 1024 : 256);
+    return (want == 1024 && groupKernel(tc, cls, 1024)) ? 1024 : 256;
+}
 
 // Slot class of a row of `rowBytes` (at most 64): slots of 4 << class bytes.
 uint32_t groupClassOf(uint32_t rowBytes) {
@@ -726,7 +744,7 @@ const DeviceGroupPlan* groupFor(const DeviceDirPlan& plan, const LaunchTable& ta
     if (!want || timing || orderPreserving || geo.rowBytes > 64) return nullptr;
     const uint32_t cls = groupClassOf(geo.rowBytes);
     const DeviceGroupPlan& gp = plan.groups[cls];
-    if (gp.empty() || gp.heavyFanin != tab.heavyFanin || !groupKernel(tc, cls)) return nullptr;
+    if (gp.empty() || gp.heavyFanin != tab.heavyFanin || !groupKernel(tc, cls, 256)) return nullptr;
     return &gp;
 }
 
@@ -754,7 +772,7 @@ std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTa
         LevelLaunch L;
         L.group = true;
         L.grid = dim3(gp->blocks);
-        L.block = kBlock;
+        L.block = groupBlockThreads(tuning, tc, groupClassOf(geo.rowBytes));
         L.shmem = gp->ldsBytes;
         GroupArgs& a = L.groupArgs;
         a.rows = rows;
@@ -776,7 +794,7 @@ std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTa
         a.importRow = gp->importRow;
         a.heavyFanin = gp->heavyFanin;
         a.readSelf = readSelf ? 1 : 0;
-        L.func = groupKernel(tc, groupClassOf(geo.rowBytes));
+        L.func = groupKernel(tc, groupClassOf(geo.rowBytes), L.block);
         out.push_back(L);
         continue;
       }
@@ -980,7 +998,7 @@ void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometr
                                      (int)readSelf * 2 + (int)orderPreserving + 4 * (int)(t.xcdMap == 1) +
                                          8 * (int)(t.blockThreads == 512) + 16 * (int)(t.loadsInFlight == 8) +
                                          32 * (int)tab->heavyFanin,
-                                     (uintptr_t)rows, tab->chunkEntries, t.chain);
+                                     (uintptr_t)rows, tab->chunkEntries, t.chain | (t.blockThreads << 8));
     std::shared_ptr<CapturedLevels> cap;
     {
         std::lock_guard<std::mutex> lk(g->mu);
@@ -1899,8 +1917,9 @@ GRGL_EXPORT grgl_status grgl_hip_set_tuning(grgl_graph* g, grgl_direction direct
     t.xcdMap = tuning->xcd_map;
     if (tuning->chain > 3) throw MisuseError("chain must be 0, 1, 2 or 3");
     t.chain = tuning->chain;
-    if (tuning->block_threads != 0 && tuning->block_threads != 256 && tuning->block_threads != 512)
-        throw MisuseError("block_threads must be 0, 256 or 512");
+    if (tuning->block_threads != 0 && tuning->block_threads != 256 && tuning->block_threads != 512 &&
+        tuning->block_threads != 1024)
+        throw MisuseError("block_threads must be 0, 256, 512 or 1024");
     if (tuning->loads_in_flight != 0 && tuning->loads_in_flight != 4 && tuning->loads_in_flight != 8)
         throw MisuseError("loads_in_flight must be 0, 4 or 8");
     t.blockThreads = tuning->block_threads;
@@ -2039,7 +2058,7 @@ GRGL_EXPORT grgl_status grgl_hip_tuning_load(grgl_graph* g, const char* path) {
             if (!(f >> dir >> tc >> rows >> tw >> ug >> xcd) || (version >= 2 && !(f >> chain)) ||
                 (version >= 3 && !(f >> blk >> unr)) || (dir != GRGL_DIRECTION_UP && dir != GRGL_DIRECTION_DOWN) ||
                 tc < 0 || tc > (int)kX8 || rows == 0 || rows > 65536 || tw > 64 || ug > 1 || xcd > 2 || chain > 3 ||
-                (blk != 0 && blk != 256 && blk != 512) || (unr != 0 && unr != 4 && unr != 8))
+                (blk != 0 && blk != 256 && blk != 512 && blk != 1024) || (unr != 0 && unr != 4 && unr != 8))
                 throw MisuseError("malformed tuning file (entry)");
             Tuning t;
             t.teamLanes = (uint32_t)tw;
@@ -2163,9 +2182,12 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
                 // Grouped levels (the other levels at this team width), direct or graph launches.
                 if (groupable) {
                     for (uint32_t useGraph = 0; useGraph < 2; useGraph++) {
-                        t.chain = 3;
-                        t.useGraph = useGraph;
-                        cands.push_back(t);
+                        for (uint32_t blk : {256U, 1024U}) {
+                            t.chain = 3;
+                            t.useGraph = useGraph;
+                            t.blockThreads = blk;
+                            cands.push_back(t);
+                        }
                     }
                 }
             }

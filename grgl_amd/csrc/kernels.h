@@ -569,12 +569,12 @@ __device__ __forceinline__ void group_slot_sum(const char* slots, const uint32_t
     for (; j < e; j += step) acc_pack<T, RB, XOR>(acc, load_pack<T, RB>(slots + (size_t)list[j] * RB));
 }
 
-template <typename T, int RB, bool XOR>
-__global__ __launch_bounds__(kBlock) void pull_group_kernel(GroupArgs a) {
+template <typename T, int RB, bool XOR, int BLK>
+__global__ __launch_bounds__(BLK) void pull_group_kernel(GroupArgs a) {
     extern __shared__ __align__(16) char slots[];
     using P = Pack<T, RB>;
     constexpr int UNR = RB <= 16 ? 8 : (RB == 32 ? 4 : 2);
-    constexpr uint32_t kWaves = kBlock / 64;
+    constexpr uint32_t kWaves = BLK / 64;
     const uint32_t b = blockIdx.x;
     const uint32_t r0 = a.blockRow[b], r1 = a.blockRow[b + 1];
     const uint32_t h0 = a.blockHeavy[b], h1 = a.blockHeavy[b + 1];
@@ -590,18 +590,18 @@ __global__ __launch_bounds__(kBlock) void pull_group_kernel(GroupArgs a) {
     uint32_t* heavyPreL = segL + 3 * nSeg;
     const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     // Phase 0.
-    for (uint32_t t = threadIdx.x; t <= nRows; t += kBlock) remOffL[t] = a.remOff[r0 + t] - remBase;
-    for (uint32_t t = threadIdx.x; t < nRem; t += kBlock) remL[t] = a.rem[remBase + t];
-    for (uint32_t t = threadIdx.x; t < nSeg; t += kBlock) {
+    for (uint32_t t = threadIdx.x; t <= nRows; t += BLK) remOffL[t] = a.remOff[r0 + t] - remBase;
+    for (uint32_t t = threadIdx.x; t < nRem; t += BLK) remL[t] = a.rem[remBase + t];
+    for (uint32_t t = threadIdx.x; t < nSeg; t += BLK) {
         segL[3 * t] = a.segRow[s0 + t] - r0;
         segL[3 * t + 1] = a.segHeavy[s0 + t];
         segL[3 * t + 2] = a.segHeavyOrd[s0 + t];
     }
-    for (uint32_t t = threadIdx.x; t < nHeavy; t += kBlock) heavyPreL[t] = a.pre[a.heavyRow[h0 + t]];
-    for (uint32_t t = threadIdx.x; t < nImport; t += kBlock)
+    for (uint32_t t = threadIdx.x; t < nHeavy; t += BLK) heavyPreL[t] = a.pre[a.heavyRow[h0 + t]];
+    for (uint32_t t = threadIdx.x; t < nImport; t += BLK)
         store_pack<T, RB>(slots + (size_t)(importBase + t) * RB,
                           load_pack<T, RB>(a.rows + (uint64_t)a.importRow[i0 + t] * a.ld));
-    for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+    for (uint32_t i = r0 + threadIdx.x; i < r1; i += BLK) {
         const uint32_t lb = a.off[i], le = a.off[i + 1];
         if (le - lb > a.heavyFanin) continue;
         P acc = a.readSelf ? load_pack<T, RB>(a.rows + (uint64_t)a.row[i] * a.ld) : zero_pack<T, RB>();
@@ -640,14 +640,14 @@ __global__ __launch_bounds__(kBlock) void pull_group_kernel(GroupArgs a) {
                 store_pack<T, RB>(slots + (size_t)i * RB, acc);
             }
         }
-        for (uint32_t i = g0 + hc + threadIdx.x; i < g1; i += kBlock) {
+        for (uint32_t i = g0 + hc + threadIdx.x; i < g1; i += BLK) {
             P acc = load_pack<T, RB>(slots + (size_t)i * RB);
             group_slot_sum<T, RB, XOR>(slots, remL, acc, remOffL[i], remOffL[i + 1], 1);
             store_pack<T, RB>(slots + (size_t)i * RB, acc);
         }
         __syncthreads();
     }
-    for (uint32_t i = threadIdx.x; i < nRows; i += kBlock)
+    for (uint32_t i = threadIdx.x; i < nRows; i += BLK)
         store_pack<T, RB>(a.rows + (uint64_t)a.row[r0 + i] * a.ld, load_pack<T, RB>(slots + (size_t)i * RB));
 }
 

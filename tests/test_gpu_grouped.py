@@ -79,12 +79,11 @@ def test_grouped_equals_level_launches(gpu, name, dtype):
             ref = _run(cg, dt, x, K, direction, cin, cout, sp)
             if np.dtype(dtype).kind in "iub":
                 assert bits_equal(ref, og.matmul(x.cpu().numpy(), oracle.UP if direction == capi.UP else oracle.DOWN))
-            for tw in (1, 4):
-                for use_graph in (0, 1):
-                    cg.set_tuning(direction, dt, K, team_lanes=tw, chain=3, use_graph=use_graph)
-                    for rep in range(2):
-                        got = _run(cg, dt, x, K, direction, cin, cout, sp)
-                        assert bits_equal(got, ref), (name, direction, K, tw, use_graph, rep)
+            for tw, use_graph, blk in ((1, 0, 0), (4, 1, 0), (4, 0, 256)):
+                cg.set_tuning(direction, dt, K, team_lanes=tw, chain=3, use_graph=use_graph, block_threads=blk)
+                for rep in range(2):
+                    got = _run(cg, dt, x, K, direction, cin, cout, sp)
+                    assert bits_equal(got, ref), (name, direction, K, tw, use_graph, blk, rep)
             cg.set_tuning(direction, dt, K)
     cg.close()
 
