@@ -1017,6 +1017,11 @@ def main():
     # The practical streaming limit of this GPU (one 16-byte access per thread), measured now.
     ceiling = capi.stream_ceiling()
     log(f"streaming ceiling: read {ceiling['read_GBps']:.0f} GB/s, copy {ceiling['copy_GBps']:.0f} GB/s")
+    # The practical random-gather rate (4-byte values, the access of a traversal at K = 1):
+    # tables of 2 MB (a K = 1 row buffer of this graph's size, cache resident) and 256 MB.
+    gather = {f"table_{tb >> 20}MB_lanes_{lpr}": round(capi.gather_ceiling(table_bytes=tb, lanes_per_row=lpr) / 1e9, 1)
+              for tb in (2 << 20, 256 << 20) for lpr in (1, 4)}
+    log(f"random-gather ceiling (G gathers/s): {gather}")
     roofline = {
         "bound": "hbm",
         "kernel": "pull_level_kernel<float,16,*,false> (level gather, f32, 511 stacked RHS)",
@@ -1091,6 +1096,7 @@ def main():
         "step_alg_GBps": step_alg_bytes / (t_total / args.steps) / 1e9,
         "plain": plain,
         "plain_i32": plain_i32,
+        "gather_ceiling_G_per_s": gather,
         "batched_vs_plain_f32": round(plain_ms_total / batched_f32_ms, 3) if plain_ms_total else None,
         "roofline": roofline,
         # autotuned level-kernel tile per (dtype, direction, stacked rows): team width, block

@@ -85,6 +85,7 @@ EXPORTED_SYMBOLS = [
     "grgl_hip_host_alloc",
     "grgl_hip_host_free",
     "grgl_hip_stream_ceiling",
+    "grgl_hip_gather_ceiling",
 ]
 
 
@@ -183,6 +184,8 @@ def lib():
     L.grgl_hip_host_free.argtypes = [vp]
     L.grgl_hip_stream_ceiling.restype = ci
     L.grgl_hip_stream_ceiling.argtypes = [ci, sz, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.grgl_hip_gather_ceiling.restype = ci
+    L.grgl_hip_gather_ceiling.argtypes = [ci, sz, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
     L.grgl_hip_graph_create.restype = ci
     L.grgl_hip_graph_create.argtypes = [ctypes.POINTER(GraphDesc), ci, ctypes.POINTER(vp)]
     L.grgl_hip_graph_destroy.restype = ci
@@ -266,6 +269,13 @@ def stream_ceiling(device=0, nbytes=2 << 30):
     r, c = ctypes.c_double(), ctypes.c_double()
     check(lib().grgl_hip_stream_ceiling(device, nbytes, ctypes.byref(r), ctypes.byref(c)))
     return {"read_GBps": r.value, "copy_GBps": c.value}
+
+
+def gather_ceiling(device=0, table_bytes=2 << 20, lanes_per_row=1):
+    """grgl_hip_gather_ceiling: measured random 4-byte gathers per second of the device."""
+    g = ctypes.c_double()
+    check(lib().grgl_hip_gather_ceiling(device, table_bytes, lanes_per_row, ctypes.byref(g)))
+    return g.value
 
 
 def _desc(arrays):

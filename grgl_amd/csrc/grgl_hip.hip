@@ -640,6 +640,10 @@ std::vector<TravRequest> stackable(const std::vector<TravRequest>& reqs) {
     return sorted;
 }
 
+// Grouped levels replace the per-level launches of their levels unless a traversal asks for
+// one launch per level (chain = 1) or a chained launch (chain = 2).
+constexpr bool kGroupByDefault = true;
+
 Tuning chooseTuning(grgl_graph* g, grgl_direction dir, TypeClass tc, uint32_t K, const RowGeometry& geo) {
     Tuning t;
     {
@@ -671,7 +675,8 @@ Tuning chooseTuning(grgl_graph* g, grgl_direction dir, TypeClass tc, uint32_t K,
             tw = std::min<uint32_t>(16, pow2ceil(geo.lanes));
     }
     t.teamLanes = std::min<uint32_t>(maxTeamLanes(geo), pow2ceil(tw));
-    if (t.chain == 0) t.chain = 1; // level-by-level launches unless chosen
+    // Grouped levels (where the direction has them for this row size), else one launch per level.
+    if (t.chain == 0) t.chain = kGroupByDefault ? 3 : 1;
     // Rows of 128 B - 1 KB (FP32 K = 32..256): 8 row loads in flight per batch; the autotuner
     // picks it for these on the 200k x 1M graph (DOWN K = 32: 0.47 -> 0.41 ms, K = 64: 0.75 ->
     // 0.70 ms).  Wider rows keep 4 (the 511-RHS traversal measures no gain).
@@ -715,9 +720,6 @@ struct LevelLaunch {
     const char* what() const { return tail ? "tail level" : (group ? "grouped levels" : "level"); }
 };
 
-// Grouped levels replace the per-level launches of their levels unless a traversal asks for
-// one launch per level (chain = 1) or a chained launch (chain = 2).
-constexpr bool kGroupByDefault = true;
 
 // Threads per block of a grouped launch: block_threads 256 or 1024 when the tuning names one,
 // else 1024 for slots of 4 and 8 bytes (the largest groups, whose rows then spread over more

@@ -14,9 +14,10 @@ namespace grgl_amd {
 namespace {
 
 // Union-find over the examined rows.  Per set: value slots `a` (rows, 64 partials per heavy
-// row, one per slot-list entry: an upper bound on the imports), 32-bit words `b` (a list
-// offset per row, a word per heavy row, one per slot-list entry) and gather entries; a
-// block's LDS for class c is then at most (4 << c) * a + 4 * b plus its segment table.
+// row, one per slot-list entry that reads a row outside the grouped levels: an upper bound on
+// the imports), 32-bit words `b` (a list offset per row, a word per heavy row, one per
+// slot-list entry) and gather entries; a block's LDS for class c is then at most
+// (4 << c) * a + 4 * b plus its segment table.
 struct Groups {
     std::vector<uint32_t> parent;
     std::vector<uint64_t> a, b, entries;
@@ -132,12 +133,12 @@ void addLevel(const Examined& x, uint32_t L, Groups& g, std::vector<uint32_t>& p
         const uint32_t r = x.edgeRow[k];
         const uint32_t root = g.unite(r, x.edgeSrc[k]);
         const uint32_t idx = x.edgeIdx[k];
-        if (idx < pre[r]) {
-            const uint64_t d = pre[r] - idx;
-            pre[r] = idx;
-            g.a[root] += d;
-            g.b[root] += d;
-        }
+        // The entry becomes an in-group slot (no import); the slot list grows to start at it.
+        const uint64_t d = idx < pre[r] ? pre[r] - idx : 0;
+        if (d) pre[r] = idx;
+        g.a[root] += d;
+        g.a[root] -= 1;
+        g.b[root] += d;
         touch(root);
     }
 }

@@ -348,6 +348,12 @@ grgl_status grgl_hip_host_free(void* ptr);
  * bytes counted).  Allocates and frees two buffers of `bytes`.  No counterpart in the
  * reference: it gives the practical ceiling the level kernels' HBM traffic is compared with. */
 grgl_status grgl_hip_stream_ceiling(int device, size_t bytes, double* read_gbps, double* copy_gbps);
+/* Measured random-gather rate of `device`, in 4-byte gathers per second: rows of 16 list
+ * entries, each entry an index (read in order) of a value gathered from a table of
+ * `table_bytes` at a random position, one working lane in `lanes_per_row` (1 or 4).  The
+ * access of a traversal at one column (4-byte rows), whose levels are bound by it.  No
+ * counterpart in the reference. */
+grgl_status grgl_hip_gather_ceiling(int device, size_t table_bytes, uint32_t lanes_per_row, double* gathers_per_s);
 
 #ifdef __cplusplus
 }
