@@ -295,19 +295,22 @@ template <typename T, bool XOR, int BLK> const void* groupKernelB(uint32_t cls) 
         if constexpr (sizeof(T) <= 4) GRGL_GROUP(4);
         return nullptr;
     case 1: GRGL_GROUP(8);
-    case 2: GRGL_GROUP(16);
-    case 3: GRGL_GROUP(32);
-    default: GRGL_GROUP(64);
+    default:
+        if constexpr (BLK == 256) {
+            if (cls == 2) GRGL_GROUP(16);
+            if (cls == 3) GRGL_GROUP(32);
+            GRGL_GROUP(64);
+        }
+        return nullptr;
     }
 #undef GRGL_GROUP
 }
 
 template <typename T, bool XOR> const void* groupKernelT(uint32_t cls, uint32_t blk) {
     if (blk == 256) return groupKernelB<T, XOR, 256>(cls);
-    // Wrapping 8-bit adds on 64-byte rows need more registers than 1024-thread blocks leave.
-    if constexpr (sizeof(T) == 1 && !XOR) {
-        if (cls >= 4) return nullptr;
-    }
+    // 1024-thread blocks for slots of 4 and 8 bytes only: wider rows need more registers
+    // than such blocks leave (and run faster in 256-thread blocks).
+    if (cls > 1) return nullptr;
     return groupKernelB<T, XOR, 1024>(cls);
 }
 
@@ -723,8 +726,7 @@ struct LevelLaunch {
 
 // Threads per block of a grouped launch: block_threads 256 or 1024 when the tuning names one,
 // else 1024 for slots of 4 and 8 bytes (the largest groups, whose rows then spread over more
-// lanes in phase 0 and per level) and 256 for wider ones (more blocks resident); 256 where the
-// kernel has no 1024-thread form.
+// lanes in phase 0 and per level); 256 for wider slots, which have no 1024-thread form.
 uint32_t groupBlockThreads(const Tuning& t, TypeClass tc, uint32_t cls) {
     const uint32_t want = (t.blockThreads == 256 || t.blockThreads == 1024) ? t.blockThreads : (cls <= 1 ? 1024 : 256);
     return (want == 1024 && groupKernel(tc, cls, 1024)) ? 1024 : 256;
@@ -2185,6 +2187,7 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
                 if (groupable) {
                     for (uint32_t useGraph = 0; useGraph < 2; useGraph++) {
                         for (uint32_t blk : {256U, 1024U}) {
+                            if (blk == 1024 && geo.rowBytes > 8) continue; // 256 threads only
                             t.chain = 3;
                             t.useGraph = useGraph;
                             t.blockThreads = blk;

@@ -233,9 +233,9 @@ typedef struct grgl_tuning {
                                 another block.  The levels before run one launch each. */
     uint32_t block_threads;  /* threads per block: of the level kernels with wide teams (16-byte
                                 accesses, 8-32 lanes per row) of 4- and 8-byte types, 256 (default) or
-                                512; of grouped launches, 256 or 1024 (default 1024 for rows of at
-                                most 8 bytes, 256 for wider ones).  A value a launch does not take
-                                leaves it at its default.  Never changes results. */
+                                512; of grouped launches with rows of at most 8 bytes, 1024 (default)
+                                or 256 (wider rows: 256).  A value a launch does not take leaves it at
+                                its default.  Never changes results. */
     uint32_t loads_in_flight; /* row loads issued before the first add in each batch of such kernels:
                                 0 = default (4), 4 or 8.  Never changes results. */
 } grgl_tuning;
