@@ -128,7 +128,8 @@ void initGroups(const DirPlan& p, const Examined& x, uint32_t heavyFanin, Groups
 // Adds the entries of level L's bucket: unions, and each reader row's slot list growing to
 // start at its first in-group entry.  `touch` sees every changed root.
 template <typename Touch>
-void addLevel(const Examined& x, uint32_t L, Groups& g, std::vector<uint32_t>& pre, Touch touch) {
+void addLevel(const Examined& x, uint32_t L, Groups& g, std::vector<uint32_t>& pre, Touch touch,
+              uint64_t* totalA = nullptr, uint64_t* totalB = nullptr) {
     for (uint64_t k = x.bucket[L - x.lowest]; k < x.bucket[L - x.lowest + 1]; k++) {
         const uint32_t r = x.edgeRow[k];
         const uint32_t root = g.unite(r, x.edgeSrc[k]);
@@ -139,6 +140,8 @@ void addLevel(const Examined& x, uint32_t L, Groups& g, std::vector<uint32_t>& p
         g.a[root] += d;
         g.a[root] -= 1;
         g.b[root] += d;
+        if (totalA) *totalA += d - 1;
+        if (totalB) *totalB += d;
         touch(root);
     }
 }
@@ -321,23 +324,30 @@ std::vector<GroupPlan> buildGroupPlans(const DirPlan& p, uint32_t regionEnd, uin
     const uint32_t n = x.rowEnd - x.rowBegin;
     // Pass 1: the largest group of levels [L, regionEnd) for each L, in LDS bytes per class
     // and in gather entries.
+    // Also the total of all groups' slots and words, whose LDS must fit the resident blocks.
     std::vector<std::vector<uint64_t>> maxBytes(kGroupClasses, std::vector<uint64_t>(nl));
-    std::vector<uint64_t> maxEntries(nl);
+    std::vector<uint64_t> maxEntries(nl), totalA(nl), totalB(nl);
     {
         Groups g(n);
         std::vector<uint32_t> pre;
         initGroups(p, x, heavyFanin, g, pre);
         std::vector<uint64_t> mb(kGroupClasses, 0);
-        uint64_t me = 0;
+        uint64_t me = 0, ta = 0, tb = 0;
         auto touch = [&](uint32_t root) {
             for (uint32_t c = 0; c < kGroupClasses; c++) mb[c] = std::max(mb[c], g.bytes(root, groupClassBytes(c)));
             me = std::max(me, g.entries[root]);
         };
         for (uint32_t L = regionEnd; L-- > x.lowest;) {
-            for (uint32_t r = p.levelStart[L]; r < p.levelStart[L + 1]; r++) touch(r - x.rowBegin);
-            addLevel(x, L, g, pre, touch);
+            for (uint32_t r = p.levelStart[L]; r < p.levelStart[L + 1]; r++) {
+                touch(r - x.rowBegin);
+                ta += g.a[r - x.rowBegin];
+                tb += g.b[r - x.rowBegin];
+            }
+            addLevel(x, L, g, pre, touch, &ta, &tb);
             for (uint32_t c = 0; c < kGroupClasses; c++) maxBytes[c][L - x.lowest] = mb[c];
             maxEntries[L - x.lowest] = me;
+            totalA[L - x.lowest] = ta;
+            totalB[L - x.lowest] = tb;
         }
     }
     // First grouped level per class: the lowest L whose groups all fit.
@@ -347,8 +357,12 @@ std::vector<GroupPlan> buildGroupPlans(const DirPlan& p, uint32_t regionEnd, uin
         uint32_t l0 = regionEnd;
         while (l0 > x.lowest) {
             const uint32_t L = l0 - 1;
+            const uint64_t cap = lim.ldsBytes - blockReserve(regionEnd - L);
+            const uint64_t total = totalA[L - x.lowest] * groupClassBytes(c) + 4 * totalB[L - x.lowest];
+            // Packing leaves blocks partly empty: the groups' total must fit 85% of the resident
+            // blocks' LDS.
             if (maxBytes[c][L - x.lowest] + blockReserve(regionEnd - L) > lim.ldsBytes ||
-                maxEntries[L - x.lowest] > lim.maxEntries)
+                maxEntries[L - x.lowest] > lim.maxEntries || total * 100 > cap * lim.maxBlocks * 85)
                 break;
             l0 = L;
         }

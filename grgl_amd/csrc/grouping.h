@@ -34,7 +34,10 @@ constexpr uint32_t kGroupPartials = 64; // partial slots of a heavy row (the lev
 struct GroupLimits {
     uint32_t ldsBytes = 64U << 10;   // LDS of one block at most
     uint32_t maxEntries = 16384;     // gather entries of one group at most
-    uint32_t maxBlocks = 2048;       // groups share blocks beyond this many
+    // Blocks of a grouped launch at most: as many as are resident at once, so that no block
+    // waits for a compute unit (MI355X: 256 compute units of 160 KB LDS, two 64 KB blocks
+    // each).  The grouped levels stop where their groups would need more LDS than that.
+    uint32_t maxBlocks = 512;
     uint64_t maxScanEntries = 1ULL << 26; // gather entries examined per direction at most
 };
 

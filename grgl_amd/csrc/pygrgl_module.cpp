@@ -942,7 +942,7 @@ PYBIND11_MODULE(_grgl_amd, m) {
             return out;
         },
         py::arg("grg"), py::arg("direction"), py::arg("heavy_fanin") = 64, py::arg("lds_bytes") = 65536,
-        py::arg("max_entries") = 16384, py::arg("max_blocks") = 2048,
+        py::arg("max_entries") = 16384, py::arg("max_blocks") = 512,
         "The grouped-launch plans (one per row class: 4, 8, 16, 32, 64-byte slots) the device library "
         "builds for one direction, each validated (for inspection).");
 
