@@ -110,7 +110,10 @@ struct LaunchTable {
 // A grouped-launch plan (grouping.h) on the device: its arrays in one allocation.
 struct DeviceGroupPlan {
     uint32_t slotBytes = 0, heavyFanin = 0, levelBegin = 0, levelEnd = 0, blocks = 0;
-    uint32_t ldsBytes = 0; // LDS of the largest block
+    uint32_t ldsBytes = 0;  // LDS of the largest block
+    uint32_t numRows = 0;   // rows of the grouped levels
+    uint32_t numHeavy = 0;  // heavy rows among them (64 partial prefix sums each in the row buffer's tail)
+    uint64_t prefixEntries = 0; // leading list entries that read HBM rows, over all rows
     uint32_t* mem = nullptr;
     const uint32_t *blockRow = nullptr, *blockSeg = nullptr, *blockHeavy = nullptr, *blockImport = nullptr,
                    *segRow = nullptr, *segHeavy = nullptr, *segHeavyOrd = nullptr, *row = nullptr, *off = nullptr,

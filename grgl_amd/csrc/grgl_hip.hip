@@ -160,11 +160,26 @@ std::shared_ptr<const LaunchTable> launchTable(grgl_graph* g, grgl_direction dir
     return (dir == GRGL_DIRECTION_UP ? g->up : g->down).table;
 }
 
-size_t rowsBufferBytes(const grgl_graph* g, grgl_direction dir, uint32_t Ktotal, size_t elemSize) {
-    const RowGeometry geo = rowGeometry(Ktotal, elemSize);
+// Byte offset of the grouped launches' scratch area in a row buffer: after the node and seed
+// rows, 256-byte aligned.
+size_t groupScratchOffset(const grgl_graph* g, grgl_direction dir, const RowGeometry& geo) {
     uint64_t rows = g->V;
     if (dir == GRGL_DIRECTION_DOWN) rows += (uint64_t)g->M * (g->hasMiss ? 2 : 1);
-    return (size_t)(rows * geo.ld) + 256;
+    return (size_t)((rows * geo.ld + 255) / 256 * 256);
+}
+
+size_t rowsBufferBytes(const grgl_graph* g, grgl_direction dir, uint32_t Ktotal, size_t elemSize) {
+    const RowGeometry geo = rowGeometry(Ktotal, elemSize);
+    size_t bytes = groupScratchOffset(g, dir, geo);
+    // Rows of at most 8 bytes may take grouped levels with a prefix launch: 64 partial sums per
+    // heavy grouped row.
+    if (geo.rowBytes <= 8) {
+        uint32_t c = 0;
+        while ((4U << c) < geo.rowBytes) c++;
+        const DeviceGroupPlan& gp = (dir == GRGL_DIRECTION_UP ? g->up : g->down).groups[c];
+        if (!gp.empty()) bytes += (size_t)gp.numHeavy * 64 * gp.slotBytes;
+    }
+    return bytes + 256;
 }
 
 void validateShapes(const grgl_graph* g, grgl_direction dir, size_t inCols, size_t rows, size_t outSize,
@@ -312,6 +327,33 @@ template <typename T, bool XOR> const void* groupKernelT(uint32_t cls, uint32_t 
     // than such blocks leave (and run faster in 256-thread blocks).
     if (cls > 1) return nullptr;
     return groupKernelB<T, XOR, 1024>(cls);
+}
+
+template <typename T, bool XOR> const void* groupPrefixKernelT(uint32_t cls) {
+#define GRGL_PREFIX(RB) return reinterpret_cast<const void*>(&group_prefix_kernel<T, RB, XOR>)
+    switch (cls) {
+    case 0:
+        if constexpr (sizeof(T) <= 4) GRGL_PREFIX(4);
+        return nullptr;
+    case 1: GRGL_PREFIX(8);
+    case 2: GRGL_PREFIX(16);
+    case 3: GRGL_PREFIX(32);
+    default: GRGL_PREFIX(64);
+    }
+#undef GRGL_PREFIX
+}
+
+const void* groupPrefixKernel(TypeClass tc, uint32_t cls) {
+    switch (tc) {
+    case kF64: return groupPrefixKernelT<double, false>(cls);
+    case kF32: return groupPrefixKernelT<float, false>(cls);
+    case kW64: return groupPrefixKernelT<uint64_t, false>(cls);
+    case kW32: return groupPrefixKernelT<uint32_t, false>(cls);
+    case kW16: return groupPrefixKernelT<uint16_t, false>(cls);
+    case kW8: return groupPrefixKernelT<uint8_t, false>(cls);
+    case kX8: return groupPrefixKernelT<uint8_t, true>(cls);
+    }
+    throw std::logic_error("bad type class");
 }
 
 const void* groupKernel(TypeClass tc, uint32_t cls, uint32_t blk) {
@@ -714,13 +756,16 @@ struct LevelLaunch {
     uint32_t block = kBlock;
     uint32_t shmem = 0;  // dynamic LDS bytes
     bool tail = false;
-    bool group = false;  // a grouped-levels launch (GroupArgs)
+    bool group = false;  // a grouped-levels launch or its prefix launch (GroupArgs)
+    bool prefix = false; // the prefix launch
     PullArgs pull{};
     TailArgs tailArgs{};
     GroupArgs groupArgs{};
     double bytes = 0.0;
     void* argPtr() { return tail ? (void*)&tailArgs : (group ? (void*)&groupArgs : (void*)&pull); }
-    const char* what() const { return tail ? "tail level" : (group ? "grouped levels" : "level"); }
+    const char* what() const {
+        return tail ? "tail level" : (group ? (prefix ? "grouped prefix" : "grouped levels") : "level");
+    }
 };
 
 
@@ -756,7 +801,8 @@ const DeviceGroupPlan* groupFor(const DeviceDirPlan& plan, const LaunchTable& ta
 // one tail launch).  With `timing`, every level gets a launch of its own and its bytes.
 std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTable& tab, TypeClass tc,
                                        const RowGeometry& geo, const Tuning& tuning, bool readSelf,
-                                       bool orderPreserving, uint32_t K, size_t elemSize, char* rows, bool timing) {
+                                       bool orderPreserving, uint32_t K, size_t elemSize, char* rows, bool timing,
+                                       size_t groupScratch) {
     std::vector<LevelLaunch> out;
     const uint32_t tw = tuning.teamLanes;
     const bool xcdMap = tuning.xcdMap == 1;
@@ -775,7 +821,6 @@ std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTa
         if (l != gp->levelBegin) continue;
         LevelLaunch L;
         L.group = true;
-        L.grid = dim3(gp->blocks);
         L.block = groupBlockThreads(tuning, tc, groupClassOf(geo.rowBytes));
         L.shmem = gp->ldsBytes;
         GroupArgs& a = L.groupArgs;
@@ -798,6 +843,27 @@ std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTa
         a.importRow = gp->importRow;
         a.heavyFanin = gp->heavyFanin;
         a.readSelf = readSelf ? 1 : 0;
+        a.scratch = rows + groupScratch;
+        a.numRows = gp->numRows;
+        a.numHeavy = gp->numHeavy;
+        a.lightBlocks = (gp->numRows + kPrefixRowsPerBlock - 1) / kPrefixRowsPerBlock;
+        // Rows of at most 8 bytes with long prefixes (4 entries a row or more on average: UP,
+        // whose grouped rows read many samples first): the prefix sums over the whole device
+        // first, so that no block sums long lists a few rows per thread; otherwise (DOWN, whose
+        // prefixes are a row's few mutation seeds, and wider rows) in the grouped launch.
+        // UP K = 1 on the 200k x 1M graph: 0.130 -> 0.124 ms; DOWN K = 2 on the constructed
+        // 100k x 500k graph: 0.153 -> 0.166 ms with a prefix launch.
+        a.prefixDone = (geo.rowBytes <= 8 && gp->prefixEntries >= 4ULL * gp->numRows) ? 1 : 0;
+        if (a.prefixDone) {
+            LevelLaunch P = L;
+            P.prefix = true;
+            P.grid = dim3(a.lightBlocks + (gp->numHeavy + kBlock / 64 - 1) / (kBlock / 64));
+            P.block = kBlock;
+            P.shmem = 0;
+            P.func = groupPrefixKernel(tc, groupClassOf(geo.rowBytes));
+            out.push_back(P);
+        }
+        L.grid = dim3(gp->blocks);
         L.func = groupKernel(tc, groupClassOf(geo.rowBytes), L.block);
         out.push_back(L);
         continue;
@@ -994,7 +1060,8 @@ void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometr
     }
     if (direct) {
         std::vector<LevelLaunch> launches =
-            levelLaunches(plan, *tab, tc, geo, t, readSelf, orderPreserving, K, es, rows, timing);
+            levelLaunches(plan, *tab, tc, geo, t, readSelf, orderPreserving, K, es, rows, timing,
+                          groupScratchOffset(g, dir, geo));
         launchDirect(g, launches, stream, timing);
         return;
     }
@@ -1013,7 +1080,8 @@ void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometr
         auto fresh = std::make_shared<CapturedLevels>();
         GRGL_HIP_CHECK(hipEventCreateWithFlags(&fresh->lastLaunch, hipEventDisableTiming));
         std::vector<LevelLaunch> launches =
-            levelLaunches(plan, *tab, tc, geo, t, readSelf, orderPreserving, K, es, rows, false);
+            levelLaunches(plan, *tab, tc, geo, t, readSelf, orderPreserving, K, es, rows, false,
+                          groupScratchOffset(g, dir, geo));
         buildLevelGraph(launches, *fresh);
         std::lock_guard<std::mutex> lk(g->mu);
         if (g->graphs.size() >= kMaxCapturedGraphs) {
@@ -1374,6 +1442,10 @@ void uploadGroups(DeviceDirPlan& d, const std::vector<GroupPlan>& plans, uint64_
         dg.levelEnd = gp.levelEnd;
         dg.blocks = gp.blocks;
         dg.ldsBytes = gp.ldsBytes;
+        dg.numRows = (uint32_t)gp.row.size();
+        dg.numHeavy = (uint32_t)gp.heavyRow.size();
+        dg.prefixEntries = 0;
+        for (uint32_t x : gp.pre) dg.prefixEntries += x;
     }
 }
 

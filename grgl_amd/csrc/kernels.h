@@ -522,8 +522,12 @@ struct GroupArgs {
     const uint32_t* rem;         // slot lists (block-relative slots)
     const uint32_t* heavyRow;    // heavy group rows of each block, in segment order
     const uint32_t* importRow;   // HBM row of each import slot, block after block
+    char* scratch;               // heavy rows' 64 partial prefix sums, in heavyRow order
     uint32_t heavyFanin;
     uint32_t readSelf;
+    uint32_t numRows, numHeavy;  // group rows and heavy rows of the plan (prefix launch)
+    uint32_t lightBlocks;        // prefix launch: blocks of light rows before the heavy ones
+    uint32_t prefixDone;         // 1: group_prefix_kernel ran before; 0: phase 0 sums the prefixes
 };
 
 // acc += rows[src[i]], rows[src[i + step]], ... below e, in that order, with UNR loads issued
@@ -569,6 +573,40 @@ __device__ __forceinline__ void group_slot_sum(const char* slots, const uint32_t
     for (; j < e; j += step) acc_pack<T, RB, XOR>(acc, load_pack<T, RB>(slots + (size_t)list[j] * RB));
 }
 
+// The leading HBM entries of every grouped row, summed over the whole device just before the
+// grouped launch: one lane per light row, whose sum (from the row's own value with node
+// initialisation) goes to the row's own place in the row buffer, which nothing reads before
+// the grouped launch overwrites it; one wavefront per heavy row, whose 64 partial sums (entry i
+// to partial i mod 64) go to `scratch`.  The adds are those pull_group_kernel would do.
+// Light rows take one working lane in four (64 rows per block), as the level kernels' narrow
+// teams do: a latency-bound pass over short lists runs faster spread over more wavefronts.
+constexpr uint32_t kPrefixRowsPerBlock = kBlock / 4;
+
+template <typename T, int RB, bool XOR>
+__global__ __launch_bounds__(kBlock) void group_prefix_kernel(GroupArgs a) {
+    using P = Pack<T, RB>;
+    constexpr int UNR = RB <= 16 ? 8 : (RB == 32 ? 4 : 2);
+    if (blockIdx.x < a.lightBlocks) {
+        if (threadIdx.x % 4 != 0) return;
+        const uint32_t i = blockIdx.x * kPrefixRowsPerBlock + threadIdx.x / 4;
+        if (i >= a.numRows) return;
+        const uint32_t lb = a.off[i], le = a.off[i + 1];
+        if (le - lb > a.heavyFanin) return;
+        char* self = a.rows + (uint64_t)a.row[i] * a.ld;
+        P acc = a.readSelf ? load_pack<T, RB>(self) : zero_pack<T, RB>();
+        group_prefix_sum<T, RB, XOR, UNR>(a, acc, lb, lb + a.pre[i], 1);
+        store_pack<T, RB>(self, acc);
+        return;
+    }
+    const uint32_t h = (blockIdx.x - a.lightBlocks) * (kBlock / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (h >= a.numHeavy) return;
+    const uint32_t i = a.heavyRow[h];
+    const uint32_t lb = a.off[i];
+    P acc = zero_pack<T, RB>();
+    group_prefix_sum<T, RB, XOR, UNR>(a, acc, lb + lane, lb + a.pre[i], 64);
+    store_pack<T, RB>(a.scratch + ((uint64_t)h * 64 + lane) * RB, acc);
+}
+
 template <typename T, int RB, bool XOR, int BLK>
 __global__ __launch_bounds__(BLK) void pull_group_kernel(GroupArgs a) {
     extern __shared__ __align__(16) char slots[];
@@ -601,19 +639,39 @@ __global__ __launch_bounds__(BLK) void pull_group_kernel(GroupArgs a) {
     for (uint32_t t = threadIdx.x; t < nImport; t += BLK)
         store_pack<T, RB>(slots + (size_t)(importBase + t) * RB,
                           load_pack<T, RB>(a.rows + (uint64_t)a.importRow[i0 + t] * a.ld));
-    for (uint32_t i = r0 + threadIdx.x; i < r1; i += BLK) {
-        const uint32_t lb = a.off[i], le = a.off[i + 1];
-        if (le - lb > a.heavyFanin) continue;
-        P acc = a.readSelf ? load_pack<T, RB>(a.rows + (uint64_t)a.row[i] * a.ld) : zero_pack<T, RB>();
-        group_prefix_sum<T, RB, XOR, UNR>(a, acc, lb, lb + a.pre[i], 1);
-        store_pack<T, RB>(slots + (size_t)(i - r0) * RB, acc);
-    }
-    for (uint32_t k = wave; k < nHeavy; k += kWaves) {
-        const uint32_t i = a.heavyRow[h0 + k];
-        const uint32_t lb = a.off[i];
-        P acc = zero_pack<T, RB>();
-        group_prefix_sum<T, RB, XOR, UNR>(a, acc, lb + lane, lb + a.pre[i], 64);
-        store_pack<T, RB>(slots + (size_t)(nRows + 64 * k + lane) * RB, acc);
+    if (a.prefixDone) {
+        // The prefix sums group_prefix_kernel left: a light row's in its own place (a heavy
+        // row's slot is written only when the row completes, so copying its place too is
+        // harmless), a heavy row's partials in the scratch area.  Four rows in flight per thread.
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += 4 * BLK) {
+            P v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (i + (uint32_t)u * BLK < r1)
+                    v[u] = load_pack<T, RB>(a.rows + (uint64_t)a.row[i + (uint32_t)u * BLK] * a.ld);
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (i + (uint32_t)u * BLK < r1) store_pack<T, RB>(slots + (size_t)(i + (uint32_t)u * BLK - r0) * RB, v[u]);
+        }
+        for (uint32_t k = wave; k < nHeavy; k += kWaves)
+            store_pack<T, RB>(slots + (size_t)(nRows + 64 * k + lane) * RB,
+                              load_pack<T, RB>(a.scratch + ((uint64_t)(h0 + k) * 64 + lane) * RB));
+    } else {
+        // The same sums here, block by block (wide rows: few rows per block).
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += BLK) {
+            const uint32_t lb = a.off[i], le = a.off[i + 1];
+            if (le - lb > a.heavyFanin) continue;
+            P acc = a.readSelf ? load_pack<T, RB>(a.rows + (uint64_t)a.row[i] * a.ld) : zero_pack<T, RB>();
+            group_prefix_sum<T, RB, XOR, UNR>(a, acc, lb, lb + a.pre[i], 1);
+            store_pack<T, RB>(slots + (size_t)(i - r0) * RB, acc);
+        }
+        for (uint32_t k = wave; k < nHeavy; k += kWaves) {
+            const uint32_t i = a.heavyRow[h0 + k];
+            const uint32_t lb = a.off[i];
+            P acc = zero_pack<T, RB>();
+            group_prefix_sum<T, RB, XOR, UNR>(a, acc, lb + lane, lb + a.pre[i], 64);
+            store_pack<T, RB>(slots + (size_t)(nRows + 64 * k + lane) * RB, acc);
+        }
     }
     __syncthreads();
     // Phase 1.
