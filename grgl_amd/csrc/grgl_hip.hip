@@ -777,6 +777,12 @@ uint32_t groupBlockThreads(const Tuning& t, TypeClass tc, uint32_t cls) {
     return (want == 1024 && groupKernel(tc, cls, 1024)) ? 1024 : 256;
 }
 
+// The level-sequence form a call with tuning `t` takes (grouped levels fall back to one
+// launch per level where the direction has none for this row size), and for grouped levels
+// the grouped launch's block size.
+void reportChain(const DeviceDirPlan& plan, const LaunchTable& tab, TypeClass tc, const RowGeometry& geo,
+                 const Tuning& t, grgl_tuning& out);
+
 // Slot class of a row of `rowBytes` (at most 64): slots of 4 << class bytes.
 uint32_t groupClassOf(uint32_t rowBytes) {
     uint32_t c = 0;
@@ -795,6 +801,17 @@ const DeviceGroupPlan* groupFor(const DeviceDirPlan& plan, const LaunchTable& ta
     const DeviceGroupPlan& gp = plan.groups[cls];
     if (gp.empty() || gp.heavyFanin != tab.heavyFanin || !groupKernel(tc, cls, 256)) return nullptr;
     return &gp;
+}
+
+void reportChain(const DeviceDirPlan& plan, const LaunchTable& tab, TypeClass tc, const RowGeometry& geo,
+                 const Tuning& t, grgl_tuning& out) {
+    out.chain = t.chain;
+    if (t.chain == 3) {
+        if (groupFor(plan, tab, tc, geo, t, false, false))
+            out.block_threads = groupBlockThreads(t, tc, groupClassOf(geo.rowBytes));
+        else
+            out.chain = 1;
+    }
 }
 
 // The launches of every pulled level of one direction, in order (the trailing small levels as
@@ -2039,9 +2056,9 @@ GRGL_EXPORT grgl_status grgl_hip_get_tuning(grgl_graph* g, grgl_direction direct
     out->heavy_fanin = plan.table->heavyFanin;
     out->use_graph = t.useGraph;
     out->xcd_map = t.xcdMap;
-    out->chain = t.chain;
     const RowGeometry geo = rowGeometry(rows, dtypeSize(dtype));
     tileShape(tc, levelVb(geo, t.teamLanes, dtypeSize(dtype)), t.teamLanes, t, out->block_threads, out->loads_in_flight);
+    reportChain(plan, *plan.table, tc, geo, t, *out);
     out->max_launch_entries = (uint32_t)plan.table->chunkEntries;
     GRGL_API_END
 }
@@ -2321,9 +2338,9 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
         chosen->heavy_fanin = tab->heavyFanin;
         chosen->use_graph = best.useGraph;
         chosen->xcd_map = best.xcdMap;
-        chosen->chain = best.chain;
         tileShape(tc, levelVb(geo, best.teamLanes, es), best.teamLanes, best, chosen->block_threads,
                   chosen->loads_in_flight);
+        reportChain(direction == GRGL_DIRECTION_UP ? g->up : g->down, *tab, tc, geo, best, *chosen);
         chosen->max_launch_entries = (uint32_t)tab->chunkEntries;
     }
     if (best_ms) *best_ms = bestMs;

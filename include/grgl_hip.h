@@ -254,8 +254,10 @@ grgl_status grgl_hip_chain_control(grgl_graph* graph, int abort, uint32_t wait_u
 grgl_status grgl_hip_autotune(grgl_graph* graph, grgl_direction direction, grgl_dtype dtype, uint32_t rows,
                               grgl_tuning* chosen, float* best_ms);
 /* The tuning a call of (direction, dtype, rows) uses now: the stored choice with library
- * defaults filled in (team_lanes as launched, use_graph, xcd_map) and the direction's
- * heavy_fanin and max_launch_entries. */
+ * defaults filled in (team_lanes as launched, use_graph, xcd_map; chain as taken: 3 only where
+ * the direction has grouped levels for this row size, else 1; block_threads of the grouped
+ * launch when chain is 3, else of the level kernels) and the direction's heavy_fanin and
+ * max_launch_entries. */
 grgl_status grgl_hip_get_tuning(grgl_graph* graph, grgl_direction direction, grgl_dtype dtype, uint32_t rows,
                                 grgl_tuning* out);
 /* Tuning cache file (text).  save writes every stored (direction, dtype class, rows) choice

@@ -175,3 +175,23 @@ def test_grouped_batch_host_buffers_and_order_preserving(gpu):
         assert bits_equal(cg.matmul_host(xh, direction), a)
         assert bits_equal(cg.matmul_host(xh, direction, flags=capi.FLAG_ORDER_PRESERVING), b)
     cg.close()
+
+
+def test_grouped_tuning_report(gpu):
+    """get_tuning names the form a call takes: grouped levels (and their block size) where the
+    direction has them for the row size, one launch per level otherwise."""
+    arrays = _graphs()["tree"]
+    cg = capi.Graph(arrays)
+    for direction in (capi.UP, capi.DOWN):
+        t = cg.get_tuning(direction, capi.F32, 1)
+        assert t["chain"] == 3 and t["block_threads"] == 1024, t
+        assert cg.get_tuning(direction, capi.F32, 16)["chain"] == 3
+        assert cg.get_tuning(direction, capi.F32, 16)["block_threads"] == 256
+        assert cg.get_tuning(direction, capi.F32, 32)["chain"] == 1  # 128-byte rows: none
+        cg.set_tuning(direction, capi.F32, 1, block_threads=256)
+        assert cg.get_tuning(direction, capi.F32, 1)["block_threads"] == 256
+        cg.set_tuning(direction, capi.F32, 32, chain=3)
+        assert cg.get_tuning(direction, capi.F32, 32)["chain"] == 1
+        cg.set_tuning(direction, capi.F32, 1, chain=1)
+        assert cg.get_tuning(direction, capi.F32, 1)["chain"] == 1
+    cg.close()
