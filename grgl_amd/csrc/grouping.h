@@ -38,7 +38,9 @@ struct GroupLimits {
     // waits for a compute unit (MI355X: 256 compute units of 160 KB LDS, two 64 KB blocks
     // each).  The grouped levels stop where their groups would need more LDS than that.
     uint32_t maxBlocks = 512;
-    uint64_t maxScanEntries = 1ULL << 26; // gather entries examined per direction at most
+    // Gather entries examined per direction at most: bounds the plans' host time and device
+    // memory (each class's list copies, 8 bytes an entry) on graphs of billions of edges.
+    uint64_t maxScanEntries = 1ULL << 24;
 };
 
 // LDS of one block, in bytes: value slots (rows, 64 partials per heavy row, imports) of
