@@ -689,6 +689,12 @@ std::vector<TravRequest> stackable(const std::vector<TravRequest>& reqs) {
 // one launch per level (chain = 1) or a chained launch (chain = 2).
 constexpr bool kGroupByDefault = true;
 
+// Rows of up to this many bytes take grouped levels by default; wider rows when the tuning
+// asks for them.  UP: 512 (FP32 K <= 128); on the 200k x 1M graph 64-byte slices save 7-9%
+// at K = 17-32 and 3-6% at K = 48-128.  DOWN: 64 (FP32 K <= 16); slices save 1-4% at K = 17-32
+// there but lose 3.6% at K = 32 on the constructed 100k x 500k graph, and 1-8% at K >= 48.
+uint32_t groupDefaultMaxRowBytes(const DeviceDirPlan& plan) { return plan.down ? 64 : 512; }
+
 Tuning chooseTuning(grgl_graph* g, grgl_direction dir, TypeClass tc, uint32_t K, const RowGeometry& geo) {
     Tuning t;
     {
@@ -720,8 +726,12 @@ Tuning chooseTuning(grgl_graph* g, grgl_direction dir, TypeClass tc, uint32_t K,
             tw = std::min<uint32_t>(16, pow2ceil(geo.lanes));
     }
     t.teamLanes = std::min<uint32_t>(maxTeamLanes(geo), pow2ceil(tw));
-    // Grouped levels (where the direction has them for this row size), else one launch per level.
-    if (t.chain == 0) t.chain = kGroupByDefault ? 3 : 1;
+    // Grouped levels (where the direction has them for this row size, up to its default width),
+    // else one launch per level.
+    if (t.chain == 0)
+        t.chain = kGroupByDefault && geo.rowBytes <= groupDefaultMaxRowBytes(dir == GRGL_DIRECTION_UP ? g->up : g->down)
+                      ? 3
+                      : 1;
     // Rows of 128 B - 1 KB (FP32 K = 32..256): 8 row loads in flight per batch; the autotuner
     // picks it for these on the 200k x 1M graph (DOWN K = 32: 0.47 -> 0.41 ms, K = 64: 0.75 ->
     // 0.70 ms).  Wider rows keep 4 (the 511-RHS traversal measures no gain).
@@ -783,20 +793,28 @@ uint32_t groupBlockThreads(const Tuning& t, TypeClass tc, uint32_t cls) {
 void reportChain(const DeviceDirPlan& plan, const LaunchTable& tab, TypeClass tc, const RowGeometry& geo,
                  const Tuning& t, grgl_tuning& out);
 
-// Slot class of a row of `rowBytes` (at most 64): slots of 4 << class bytes.
+// Slot class of a row of `rowBytes`: slots of 4 << class bytes (64 for wider rows, which
+// run as 64-byte column slices).
 uint32_t groupClassOf(uint32_t rowBytes) {
     uint32_t c = 0;
-    while ((4U << c) < rowBytes) c++;
+    while ((4U << c) < rowBytes && c + 1 < kDeviceGroupClasses) c++;
     return c;
 }
 
-// The grouped-launch plan a traversal takes, or null: rows of at most 64 bytes, the launch
-// table's heavy threshold equal to the plan's, not order-preserving (whose lists are all
-// light) and no per-launch timing.
+// Column slices of a grouped launch: one for rows of at most 64 bytes, else one per 64 bytes.
+uint32_t groupSlices(const RowGeometry& geo) { return geo.rowBytes <= 64 ? 1 : (geo.rowBytes + 63) / 64; }
+
+
+// The grouped-launch plan a traversal takes, or null: the launch table's heavy threshold
+// equal to the plan's, not order-preserving (whose lists are all light), no per-launch
+// timing, and rows of at most groupDefaultMaxRowBytes unless the tuning asks for grouped
+// levels (chain 3) whatever the width; wider rows run as 64-byte slices, each inside the row.
 const DeviceGroupPlan* groupFor(const DeviceDirPlan& plan, const LaunchTable& tab, TypeClass tc,
                                 const RowGeometry& geo, const Tuning& t, bool orderPreserving, bool timing) {
-    const bool want = t.chain == 3 || (t.chain == 0 && kGroupByDefault);
-    if (!want || timing || orderPreserving || geo.rowBytes > 64) return nullptr;
+    const bool want =
+        t.chain == 3 || (t.chain == 0 && kGroupByDefault && geo.rowBytes <= groupDefaultMaxRowBytes(plan));
+    if (!want || timing || orderPreserving) return nullptr;
+    if (geo.rowBytes > 64 && (uint64_t)groupSlices(geo) * 64 > geo.ld) return nullptr;
     const uint32_t cls = groupClassOf(geo.rowBytes);
     const DeviceGroupPlan& gp = plan.groups[cls];
     if (gp.empty() || gp.heavyFanin != tab.heavyFanin || !groupKernel(tc, cls, 256)) return nullptr;
@@ -880,7 +898,7 @@ std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTa
             P.func = groupPrefixKernel(tc, groupClassOf(geo.rowBytes));
             out.push_back(P);
         }
-        L.grid = dim3(gp->blocks);
+        L.grid = dim3(gp->blocks, groupSlices(geo));
         L.func = groupKernel(tc, groupClassOf(geo.rowBytes), L.block);
         out.push_back(L);
         continue;
@@ -1524,6 +1542,7 @@ std::unique_ptr<grgl_graph> makeDeviceGraph(LevelizedGraph lg, int device) {
         buildGroups(lg, upGroups, downGroups);
         uploadPlan(g->up, lg.up, g->deviceBytes);
         uploadPlan(g->down, lg.down, g->deviceBytes);
+        g->down.down = true;
         uploadGroups(g->up, upGroups, g->deviceBytes);
         uploadGroups(g->down, downGroups, g->deviceBytes);
         void* ctl = nullptr;
@@ -2247,13 +2266,16 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
         GRGL_HIP_CHECK(hipEventCreate(&e1));
         GRGL_HIP_CHECK(hipMemsetAsync(ws->ptr, 0, rowsBufferBytes(g, direction, rows, es), s));
         // Candidates: every team width with direct or graph launches and both block orders
-        // (the XCD-aware order only where it differs), and, for rows of at most 64 bytes, the
-        // chained single launch and the grouped levels at every team width.
+        // (the XCD-aware order only where it differs), for rows of at most 64 bytes the
+        // chained single launch, and for rows of at most 512 bytes the grouped levels at every
+        // team width (rows over 64 bytes as 64-byte slices; wider rows need more slices than
+        // the slices save launches).
         std::vector<Tuning> cands;
         Tuning grouped;
         grouped.chain = 3;
-        const bool groupable = groupFor(direction == GRGL_DIRECTION_UP ? g->up : g->down, *launchTable(g, direction), tc,
-                                        geo, grouped, false, false) != nullptr;
+        const bool groupable = geo.rowBytes <= 512 &&
+                               groupFor(direction == GRGL_DIRECTION_UP ? g->up : g->down, *launchTable(g, direction),
+                                        tc, geo, grouped, false, false) != nullptr;
         for (uint32_t tw : tws) {
             for (uint32_t useGraph = 0; useGraph < 2; useGraph++) {
                 for (uint32_t xcd = 1; xcd <= 2; xcd++) {
@@ -2272,16 +2294,19 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
                 t.xcdMap = 1;
                 t.chain = 2;
                 cands.push_back(t);
-                // Grouped levels (the other levels at this team width), direct or graph launches.
-                if (groupable) {
-                    for (uint32_t useGraph = 0; useGraph < 2; useGraph++) {
-                        for (uint32_t blk : {256U, 1024U}) {
-                            if (blk == 1024 && geo.rowBytes > 8) continue; // 256 threads only
-                            t.chain = 3;
-                            t.useGraph = useGraph;
-                            t.blockThreads = blk;
-                            cands.push_back(t);
-                        }
+            }
+            // Grouped levels (the other levels at this team width), direct or graph launches.
+            if (groupable) {
+                for (uint32_t useGraph = 0; useGraph < 2; useGraph++) {
+                    for (uint32_t blk : {256U, 1024U}) {
+                        if (blk == 1024 && geo.rowBytes > 8) continue; // 256 threads only
+                        Tuning t;
+                        t.teamLanes = tw;
+                        t.xcdMap = 1;
+                        t.chain = 3;
+                        t.useGraph = useGraph;
+                        t.blockThreads = blk;
+                        cands.push_back(t);
                     }
                 }
             }

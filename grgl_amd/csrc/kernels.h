@@ -486,7 +486,8 @@ __global__ __launch_bounds__(kBlock) void pull_tail_kernel(TailArgs t) {
 // Grouped levels (grouping.h): block b runs its groups through levels [levelBegin, levelEnd)
 // by itself, one segment (one level of its rows) after another with a block barrier in
 // between; no block waits for another.  Each lane holds a whole row of RB bytes (rows of at
-// most 64 bytes).  The block keeps its rows' values in LDS slots.
+// most 64 bytes; wider rows as slices of 64 bytes, one per grid row).  The block keeps its
+// rows' values in LDS slots.
 //   Phase 0 loads everything the levels need, every load of the block independent of the
 //   others: the row offsets of the block's slot lists, the slot lists, its segment table and
 //   heavy rows' prefix lengths; the import slots (HBM rows that a list reads after its first
@@ -611,6 +612,9 @@ template <typename T, int RB, bool XOR, int BLK>
 __global__ __launch_bounds__(BLK) void pull_group_kernel(GroupArgs a) {
     extern __shared__ __align__(16) char slots[];
     using P = Pack<T, RB>;
+    // Rows wider than RB bytes run as column slices of RB bytes, one per grid row: the columns
+    // of a row never meet, so a slice is the same traversal over other bytes.
+    a.rows += (uint64_t)blockIdx.y * RB;
     constexpr int UNR = RB <= 16 ? 8 : (RB == 32 ? 4 : 2);
     constexpr uint32_t kWaves = BLK / 64;
     const uint32_t b = blockIdx.x;

@@ -218,19 +218,21 @@ typedef struct grgl_tuning {
                                     launches.  Never changes results; like heavy_fanin it applies to
                                     every later call in that direction. */
     uint32_t chain;          /* form of the level sequence; never changes results.
-                                0 = default: grouped levels (3) where the graph has them, else 1.
+                                0 = default: grouped levels (3) where the graph has them for rows of
+                                at most 64 bytes (DOWN) or 512 bytes (UP), else 1.
                                 1 = one launch per level.
                                 2 = the whole level sequence as ONE chained launch where the shape
                                 allows it (rows of at most 64 bytes of a 4- or 8-byte type, at most
                                 120 levels): blocks take the levels' work in order from a ticket
                                 counter and wait on per-level completion counters, with no grid
                                 barrier.  Its waits are bounded (grgl_hip_chain_control).
-                                3 = grouped levels: rows of at most 64 bytes, not order-preserving,
-                                with the default heavy_fanin.  The later levels of the direction are
-                                cut, when the graph is created, into connected groups of rows that
-                                read no row of those levels outside the group; ONE launch runs them,
-                                each block its groups level after level from LDS, with no wait on
-                                another block.  The levels before run one launch each. */
+                                3 = grouped levels: not order-preserving, with the default
+                                heavy_fanin.  The later levels of the direction are cut, when the
+                                graph is created, into connected groups of rows that read no row of
+                                those levels outside the group; ONE launch runs them, each block its
+                                groups level after level from LDS, with no wait on another block
+                                (rows wider than 64 bytes as 64-byte column slices, one grid row
+                                each).  The levels before run one launch each. */
     uint32_t block_threads;  /* threads per block: of the level kernels with wide teams (16-byte
                                 accesses, 8-32 lanes per row) of 4- and 8-byte types, 256 (default) or
                                 512; of grouped launches with rows of at most 8 bytes, 1024 (default)

@@ -1,8 +1,8 @@
 """Grouped levels (grgl_tuning::chain = 3: the later levels of a direction cut into connected
 groups that one block runs by itself, values held in LDS): bit-identical to the level-by-level
-launches for every dtype and row class (4-64 bytes), every option that changes how rows start
-or are read out, stacked batches, host buffers and hipGraph replay; integer results equal the
-oracle."""
+launches for every dtype and row class (4-64 bytes), wider rows as 64-byte column slices, every
+option that changes how rows start or are read out, stacked batches, host buffers and hipGraph
+replay; integer results equal the oracle."""
 
 import numpy as np
 import pytest
@@ -72,7 +72,10 @@ def test_grouped_equals_level_launches(gpu, name, dtype):
         assert _grouped_classes(arrays, direction), (name, direction)
         cin = cg.num_samples if direction == capi.UP else cg.num_mutations
         cout = cg.num_mutations if direction == capi.UP else cg.num_samples
-        for K in [k for k in (1, 2, 3, 4, 8, 12, 16, 24, 32, 64) if k * es <= 64]:
+        # Rows of 4-64 bytes (one slot class each), then two wider ones (64-byte column slices,
+        # the last one partly padding).
+        wide = [k for k in (9, 17, 40, 100, 130) if 64 < k * es <= 640][:2]
+        for K in [k for k in (1, 2, 3, 4, 8, 12, 16, 24, 32, 64) if k * es <= 64] + wide:
             print(f"grouped: {name} {np.dtype(dtype).name} dir={direction} K={K}", flush=True)
             x = torch.from_numpy(make_input(rng, dtype, (K, cin))).cuda()
             cg.set_tuning(direction, dt, K, chain=1)
@@ -91,7 +94,8 @@ def test_grouped_equals_level_launches(gpu, name, dtype):
 @pytest.mark.parametrize("dtype", [np.float32, np.int32, np.float64])
 def test_grouped_options(gpu, dtype):
     """Node initialisation (vector, matrix, xtx), emit_all_nodes, by_individual, miss and
-    accumulation through grouped levels equal the level-by-level launches."""
+    accumulation through grouped levels equal the level-by-level launches (K = 20: column
+    slices)."""
     import torch
 
     arrays = _graphs()["heavy"]
@@ -101,7 +105,7 @@ def test_grouped_options(gpu, dtype):
     sp = torch.cuda.current_stream().cuda_stream
     V, N, Mu = cg.num_nodes, cg.num_samples, cg.num_mutations
     for direction in (capi.UP, capi.DOWN):
-        for K in (1, 3):
+        for K in (1, 3, 20):
             cin = N if direction == capi.UP else Mu
             x = torch.from_numpy(make_input(rng, dtype, (K, cin))).cuda()
             vec = torch.from_numpy(make_input(rng, dtype, (K,))).cuda()
@@ -187,11 +191,16 @@ def test_grouped_tuning_report(gpu):
         assert t["chain"] == 3 and t["block_threads"] == 1024, t
         assert cg.get_tuning(direction, capi.F32, 16)["chain"] == 3
         assert cg.get_tuning(direction, capi.F32, 16)["block_threads"] == 256
-        assert cg.get_tuning(direction, capi.F32, 32)["chain"] == 1  # 128-byte rows: none
+        # 128-byte rows: 64-byte slices by default UP (rows up to 512 bytes), not DOWN (64).
+        assert cg.get_tuning(direction, capi.F32, 32)["chain"] == (3 if direction == capi.UP else 1)
+        assert cg.get_tuning(direction, capi.F32, 128)["chain"] == (3 if direction == capi.UP else 1)
+        assert cg.get_tuning(direction, capi.F32, 129)["chain"] == 1
         cg.set_tuning(direction, capi.F32, 1, block_threads=256)
         assert cg.get_tuning(direction, capi.F32, 1)["block_threads"] == 256
         cg.set_tuning(direction, capi.F32, 32, chain=3)
-        assert cg.get_tuning(direction, capi.F32, 32)["chain"] == 1
+        assert cg.get_tuning(direction, capi.F32, 32)["chain"] == 3
+        cg.set_tuning(direction, capi.F32, 511, chain=3)
+        assert cg.get_tuning(direction, capi.F32, 511)["chain"] == 3
         cg.set_tuning(direction, capi.F32, 1, chain=1)
         assert cg.get_tuning(direction, capi.F32, 1)["chain"] == 1
     cg.close()

@@ -264,8 +264,9 @@ def test_tile_shapes_give_the_same_bits(gpu, name, dtype):
                 for blk in (256, 512):
                     for unr in (4, 8):
                         for use_graph in ((0, 1) if (blk, unr) == (512, 8) else (0,)):
+                            # One launch per level (block_threads reports the level kernels').
                             cg.set_tuning(direction, dt, K, team_lanes=tw, block_threads=blk, loads_in_flight=unr,
-                                          use_graph=use_graph)
+                                          use_graph=use_graph, chain=1)
                             t = cg.get_tuning(direction, dt, K)
                             if t["team_lanes"] == tw and K * np.dtype(dtype).itemsize >= 16 * tw:
                                 assert (t["block_threads"], t["loads_in_flight"]) == (blk, unr), t
