@@ -75,7 +75,7 @@ def test_config2_constructed_graph(gpu, tmp_path):
     rng = np.random.default_rng(SEED + 11)
     for d, cols in ((grgl_amd.UP, g.num_samples), (grgl_amd.DOWN, g.num_mutations)):
         od = oracle.UP if d == grgl_amd.UP else oracle.DOWN
-        for K in (1, 16):
+        for K in (1, 16, 40):  # 40: 160-byte rows, grouped as 64-byte column slices UP by default
             xi = rng.integers(-9, 10, size=(K, cols)).astype(np.int32)
             yi = grgl_amd.matmul(g, xi, d)
             assert bits_equal(yi, og.matmul(xi, od)), (d, K)
@@ -127,6 +127,36 @@ def test_config3_batched_ladder(gpu, dtype):
             rep = fp_bounds.fp32_report(og, x[:7], _odir(direction), y[:7], with_reference=False)
             assert rep["bound_ratio"] <= 1.0, rep
         cg.set_tuning(direction, cdt, Ktot)
+
+
+@pytest.mark.parametrize("K", [1, 40])
+def test_config3_grouped_levels(gpu, K):
+    """Grouped levels at the benchmark graph's size (K = 40: 160-byte rows as 64-byte column
+    slices): int32 equal to the oracle, FP32 bit-identical to one launch per level."""
+    import torch
+
+    a, cg, og = _config3()
+    rng = np.random.default_rng(SEED + 3 + K)
+    sp = torch.cuda.current_stream().cuda_stream
+    for direction in (capi.UP, capi.DOWN):
+        cin = cg.num_samples if direction == capi.UP else cg.num_mutations
+        cout = cg.num_mutations if direction == capi.UP else cg.num_samples
+        for dt, x in ((capi.I32, rng.integers(-9, 10, size=(K, cin)).astype(np.int32)),
+                      (capi.F32, rng.standard_normal((K, cin)).astype(np.float32))):
+            xd = torch.from_numpy(x).cuda()
+            outs = []
+            for chain in (3, 1):
+                print(f"config3 grouped: K={K} dir={direction} dtype={dt} chain={chain}", flush=True)
+                cg.set_tuning(direction, dt, K, chain=chain)
+                assert cg.get_tuning(direction, dt, K)["chain"] == chain
+                yd = torch.zeros((K, cout), dtype=xd.dtype, device="cuda")
+                cg.matmul_ptr(dt, xd.data_ptr(), cin, K, direction, yd.data_ptr(), yd.numel(), stream=sp)
+                torch.cuda.synchronize()
+                outs.append(yd.cpu().numpy())
+            cg.set_tuning(direction, dt, K)
+            assert bits_equal(outs[0], outs[1]), (K, direction, dt)
+            if dt == capi.I32:
+                assert bits_equal(outs[0], oracle_rows(og, x, _odir(direction))), (K, direction)
 
 
 def test_config4_pca_full_size(gpu):
