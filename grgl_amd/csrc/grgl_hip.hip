@@ -832,12 +832,22 @@ void reportChain(const DeviceDirPlan& plan, const LaunchTable& tab, TypeClass tc
     }
 }
 
+// Output fusion of a traversal (PullArgs::out): the last level's rows written straight to
+// their output elements.
+struct OutFusion {
+    char* out;
+    const uint32_t* outCol;
+    uint32_t outCols;
+};
+
 // The launches of every pulled level of one direction, in order (the trailing small levels as
-// one tail launch).  With `timing`, every level gets a launch of its own and its bytes.
+// one tail launch).  With `timing`, every level gets a launch of its own and its bytes.  With
+// `fuse`, the last level's level-kernel launches write its rows to the output when each row is
+// one lane access (*fused is then set; a last level in the tail launch is not fused).
 std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTable& tab, TypeClass tc,
                                        const RowGeometry& geo, const Tuning& tuning, bool readSelf,
                                        bool orderPreserving, uint32_t K, size_t elemSize, char* rows, bool timing,
-                                       size_t groupScratch) {
+                                       size_t groupScratch, const OutFusion* fuse = nullptr, bool* fused = nullptr) {
     std::vector<LevelLaunch> out;
     const uint32_t tw = tuning.teamLanes;
     const bool xcdMap = tuning.xcdMap == 1;
@@ -931,6 +941,12 @@ std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTa
         a.levelEnd = end;
         a.rowBytes = geo.rowBytes;
         a.readSelf = readSelf ? 1 : 0;
+        if (fuse && l + 1 == plan.numLevels && vb == geo.rowBytes && tiles == 1) {
+            a.out = fuse->out;
+            a.outCol = fuse->outCol;
+            a.outCols = fuse->outCols;
+            if (fused) *fused = true;
+        }
         const uint32_t bxCount = a.heavyBlocks + (light + teamsPerBlock - 1) / teamsPerBlock;
         a.xcdTiles = (xcdMap && tiles % 8 == 0) ? tiles : 0;
         L.grid = a.xcdTiles ? dim3(bxCount * tiles, 1) : dim3(bxCount, tiles);
@@ -1076,15 +1092,16 @@ bool launchChained(grgl_graph* g, const DeviceDirPlan& plan, const LaunchTable& 
 // Level sequence, either launched directly or replayed from a hipGraph of it.  A graph bakes
 // in the row buffer address, the row geometry, the team width and the launch table, so the
 // cache key holds all of them; different workspaces get different graphs.
-void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometry& geo, const Tuning& t,
+// Returns whether the last level's rows went to the output (`fuse`: direct launches only).
+bool runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometry& geo, const Tuning& t,
                bool readSelf, bool orderPreserving, uint32_t K, size_t es, char* rows, hipStream_t stream,
-               bool launchOnly = false, uint32_t* chainState = nullptr) {
+               bool launchOnly = false, uint32_t* chainState = nullptr, const OutFusion* fuse = nullptr) {
     const DeviceDirPlan& plan = dir == GRGL_DIRECTION_UP ? g->up : g->down;
     const std::shared_ptr<const LaunchTable> tab = launchTable(g, dir);
     const bool timing = g->timing && !launchOnly;
     if (t.chain == 2 && !timing && !launchOnly &&
         launchChained(g, plan, *tab, tc, geo, t.teamLanes, readSelf, orderPreserving, es, rows, chainState, stream))
-        return;
+        return false;
     bool direct = !t.useGraph || timing || launchOnly;
     if (!direct && stream != nullptr) {
         // A stream the caller is capturing gets direct launches (they become part of the
@@ -1094,11 +1111,12 @@ void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometr
         direct = cs != hipStreamCaptureStatusNone;
     }
     if (direct) {
+        bool fused = false;
         std::vector<LevelLaunch> launches =
             levelLaunches(plan, *tab, tc, geo, t, readSelf, orderPreserving, K, es, rows, timing,
-                          groupScratchOffset(g, dir, geo));
+                          groupScratchOffset(g, dir, geo), fuse, &fused);
         launchDirect(g, launches, stream, timing);
-        return;
+        return fused;
     }
     const auto key = std::make_tuple((int)dir, (int)tc, geo.vb, t.teamLanes, geo.ld, geo.rowBytes,
                                      (int)readSelf * 2 + (int)orderPreserving + 4 * (int)(t.xcdMap == 1) +
@@ -1133,6 +1151,7 @@ void runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometr
     }
     GRGL_HIP_CHECK(hipGraphLaunch(cap->exec, stream));
     GRGL_HIP_CHECK(hipEventRecord(cap->lastLaunch, stream));
+    return false;
 }
 
 } // namespace
@@ -1241,8 +1260,25 @@ void enqueueTraversal(grgl_graph* g, grgl_dtype dt, grgl_direction dir, const st
     const Tuning tuning = chooseTuning(g, dir, tc, Ktotal, geo);
     // ORDER_PRESERVING: every list is summed by one team front to back, i.e. in the
     // reference's order.
-    runLevels(g, dir, tc, geo, tuning, hasInit, opt.orderPreserving, Ktotal, es, rows, stream, opt.launchOnly,
-              opt.chainState);
+    // One-column DOWN traversals whose output is the samples' rows as they are (4- and 8-byte
+    // types, no accumulation, no per-individual sums): the last level, which holds every
+    // sample and which no row reads, writes the samples' values straight to the output (0 +
+    // value, as the read-out does), and there is no read-out launch.  On the 200k x 1M graph
+    // that is the sample level and saves the 5 us read-out of a 112 us FP32 K = 1 call.
+    OutFusion fo{};
+    const bool fuse = dir == GRGL_DIRECTION_DOWN && reqs.size() == 1 && Ktotal == 1 && geo.rowBytes == es &&
+                      !opt.emitAll && !opt.byIndividual && !opt.accumulate && plan.samplesLast;
+    if (fuse) {
+        fo.out = static_cast<char*>(reqs[0].out);
+        fo.outCol = plan.order;
+        fo.outCols = g->N;
+    }
+    const bool fusedOut = runLevels(g, dir, tc, geo, tuning, hasInit, opt.orderPreserving, Ktotal, es, rows, stream,
+                                    opt.launchOnly, opt.chainState, fuse ? &fo : nullptr);
+    if (fusedOut) {
+        GRGL_HIP_CHECK(hipGetLastError());
+        return;
+    }
     // 4. Read-out.  Several requests: one stacked launch for the main outputs.
     const bool stackOut = !stacked.empty() && !(dir == GRGL_DIRECTION_UP && opt.miss && plan.missRow && !opt.emitAll);
     if (stackOut) {
@@ -1543,6 +1579,11 @@ std::unique_ptr<grgl_graph> makeDeviceGraph(LevelizedGraph lg, int device) {
         uploadPlan(g->up, lg.up, g->deviceBytes);
         uploadPlan(g->down, lg.down, g->deviceBytes);
         g->down.down = true;
+        // Every sample in the last DOWN level (samples have no children, so always in a GRG):
+        // the condition for writing that level's rows straight to the output.
+        g->down.samplesLast = lg.down.numLevels > 0;
+        for (uint32_t sIdx = 0; sIdx < lg.N && g->down.samplesLast; sIdx++)
+            g->down.samplesLast = lg.down.outRow[sIdx] >= lg.down.levelStart[lg.down.numLevels - 1];
         uploadGroups(g->up, upGroups, g->deviceBytes);
         uploadGroups(g->down, downGroups, g->deviceBytes);
         void* ctl = nullptr;

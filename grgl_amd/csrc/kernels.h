@@ -134,7 +134,30 @@ struct PullArgs {
     uint32_t readSelf;    // 1: start from the row's current value (node initialisation)
     uint32_t xcdTiles;    // 0: 2-D grid (x = rows, y = column slices); else a 1-D grid over
                           // xcdTiles slices (a multiple of 8), see pull_block_coords
+    // Output fusion (set only for the last DOWN level of a one-column traversal, whose rows
+    // nothing reads afterwards): a row whose outCol entry is below outCols is written to the
+    // output element outCol[row] (rowBytes apart) instead of its row, as 0 + value like the
+    // read-out it replaces; other rows go to the row buffer.
+    char* out = nullptr;
+    const uint32_t* outCol = nullptr;
+    uint32_t outCols = 0;
 };
+
+// Stores row `row`'s finished value (its slice at colByte) where it goes: the row buffer, or
+// with output fusion the output element of its column.
+template <typename T, int VB, bool XOR>
+__device__ __forceinline__ void store_row(const PullArgs& a, uint32_t row, uint64_t colByte, const Pack<T, VB>& v) {
+    if (a.outCol) {
+        const uint32_t c = a.outCol[row];
+        if (c < a.outCols) {
+            Pack<T, VB> z = zero_pack<T, VB>();
+            acc_pack<T, VB, XOR>(z, v);
+            store_pack<T, VB>(a.out + (uint64_t)c * a.rowBytes + colByte, z);
+            return;
+        }
+    }
+    store_pack<T, VB>(a.rows + (uint64_t)row * a.ld + colByte, v);
+}
 
 // Workgroups are handed to the 8 XCDs round-robin by linear ID.  With a 1-D grid,
 // block b runs on XCD b % 8; giving it column slice (b % 8) + 8 * ((b / 8) % (tiles / 8))
@@ -185,7 +208,7 @@ __device__ __forceinline__ void pull_light_range(const PullArgs& a, uint32_t row
         for (int u = 0; u < UNR; u++)
             if (i + u < e) acc_pack<T, VB, XOR>(acc, x[u]);
     }
-    store_pack<T, VB>(a.rows + (uint64_t)row * a.ld + colByte, acc);
+    store_row<T, VB, XOR>(a, row, colByte, acc);
 }
 
 template <typename T, int VB, int TW, bool XOR, int UNR, typename Idx>
@@ -271,7 +294,7 @@ __device__ __forceinline__ void pull_heavy(const PullArgs& a, uint32_t row, uint
                     acc_pack<T, VB, XOR>(self, mine);
                     mine = self;
                 }
-                store_pack<T, VB>(a.rows + (uint64_t)row * a.ld + colByte, mine);
+                store_row<T, VB, XOR>(a, row, colByte, mine);
             }
         } else {
             store_pack<T, VB>(lds + ((uint64_t)team * TW + lane) * VB, part[0]);
@@ -296,7 +319,7 @@ __device__ __forceinline__ void pull_heavy(const PullArgs& a, uint32_t row, uint
             acc_pack<T, VB, XOR>(self, res);
             res = self;
         }
-        store_pack<T, VB>(a.rows + (uint64_t)row * a.ld + colByte, res);
+        store_row<T, VB, XOR>(a, row, colByte, res);
     }
 }
 
