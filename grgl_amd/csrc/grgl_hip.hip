@@ -838,12 +838,13 @@ struct OutFusion {
     char* out;
     const uint32_t* outCol;
     uint32_t outCols;
+    uint32_t K;
 };
 
 // The launches of every pulled level of one direction, in order (the trailing small levels as
 // one tail launch).  With `timing`, every level gets a launch of its own and its bytes.  With
-// `fuse`, the last level's level-kernel launches write its rows to the output when each row is
-// one lane access (*fused is then set; a last level in the tail launch is not fused).
+// `fuse`, the last level's level-kernel launches write its rows to the output when a row is one
+// column slice (*fused is then set; a last level in the tail launch is not fused).
 std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTable& tab, TypeClass tc,
                                        const RowGeometry& geo, const Tuning& tuning, bool readSelf,
                                        bool orderPreserving, uint32_t K, size_t elemSize, char* rows, bool timing,
@@ -941,10 +942,11 @@ std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTa
         a.levelEnd = end;
         a.rowBytes = geo.rowBytes;
         a.readSelf = readSelf ? 1 : 0;
-        if (fuse && l + 1 == plan.numLevels && vb == geo.rowBytes && tiles == 1) {
+        if (fuse && l + 1 == plan.numLevels && tiles == 1) {
             a.out = fuse->out;
             a.outCol = fuse->outCol;
             a.outCols = fuse->outCols;
+            a.outK = fuse->K;
             if (fused) *fused = true;
         }
         const uint32_t bxCount = a.heavyBlocks + (light + teamsPerBlock - 1) / teamsPerBlock;
@@ -1260,18 +1262,20 @@ void enqueueTraversal(grgl_graph* g, grgl_dtype dt, grgl_direction dir, const st
     const Tuning tuning = chooseTuning(g, dir, tc, Ktotal, geo);
     // ORDER_PRESERVING: every list is summed by one team front to back, i.e. in the
     // reference's order.
-    // One-column DOWN traversals whose output is the samples' rows as they are (4- and 8-byte
-    // types, no accumulation, no per-individual sums): the last level, which holds every
-    // sample and which no row reads, writes the samples' values straight to the output (0 +
-    // value, as the read-out does), and there is no read-out launch.  On the 200k x 1M graph
-    // that is the sample level and saves the 5 us read-out of a 112 us FP32 K = 1 call.
+    // DOWN traversals of one request with rows of at most 16 bytes (FP32 K <= 4) whose output
+    // is the samples' rows as they are (no accumulation, no per-individual sums): the last
+    // level, which holds every sample and which no row reads, writes the samples' values
+    // straight to their output columns (0 + value, as the read-out does), and there is no
+    // read-out launch.  On the 200k x 1M graph that is the sample level and saves the 4-5 us
+    // read-out of a FP32 K = 1 call (0.112 -> 0.107 ms).
     OutFusion fo{};
-    const bool fuse = dir == GRGL_DIRECTION_DOWN && reqs.size() == 1 && Ktotal == 1 && geo.rowBytes == es &&
+    const bool fuse = dir == GRGL_DIRECTION_DOWN && reqs.size() == 1 && reqs[0].kOffset == 0 && geo.rowBytes <= 16 &&
                       !opt.emitAll && !opt.byIndividual && !opt.accumulate && plan.samplesLast;
     if (fuse) {
         fo.out = static_cast<char*>(reqs[0].out);
         fo.outCol = plan.order;
         fo.outCols = g->N;
+        fo.K = reqs[0].K;
     }
     const bool fusedOut = runLevels(g, dir, tc, geo, tuning, hasInit, opt.orderPreserving, Ktotal, es, rows, stream,
                                     opt.launchOnly, opt.chainState, fuse ? &fo : nullptr);

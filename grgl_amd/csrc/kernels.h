@@ -134,17 +134,19 @@ struct PullArgs {
     uint32_t readSelf;    // 1: start from the row's current value (node initialisation)
     uint32_t xcdTiles;    // 0: 2-D grid (x = rows, y = column slices); else a 1-D grid over
                           // xcdTiles slices (a multiple of 8), see pull_block_coords
-    // Output fusion (set only for the last DOWN level of a one-column traversal, whose rows
-    // nothing reads afterwards): a row whose outCol entry is below outCols is written to the
-    // output element outCol[row] (rowBytes apart) instead of its row, as 0 + value like the
-    // read-out it replaces; other rows go to the row buffer.
+    // Output fusion (set only for the last DOWN level of a traversal of narrow rows, one
+    // column slice, whose rows nothing reads afterwards): a row whose outCol entry c is below
+    // outCols is written to column c of the K x outCols output (element k at k * outCols + c,
+    // k < outK) instead of to its row, as 0 + value like the read-out it replaces; other rows
+    // go to the row buffer.
     char* out = nullptr;
     const uint32_t* outCol = nullptr;
     uint32_t outCols = 0;
+    uint32_t outK = 0;
 };
 
 // Stores row `row`'s finished value (its slice at colByte) where it goes: the row buffer, or
-// with output fusion the output element of its column.
+// with output fusion the output elements of its column.
 template <typename T, int VB, bool XOR>
 __device__ __forceinline__ void store_row(const PullArgs& a, uint32_t row, uint64_t colByte, const Pack<T, VB>& v) {
     if (a.outCol) {
@@ -152,7 +154,11 @@ __device__ __forceinline__ void store_row(const PullArgs& a, uint32_t row, uint6
         if (c < a.outCols) {
             Pack<T, VB> z = zero_pack<T, VB>();
             acc_pack<T, VB, XOR>(z, v);
-            store_pack<T, VB>(a.out + (uint64_t)c * a.rowBytes + colByte, z);
+            const uint32_t k0 = (uint32_t)(colByte / sizeof(T));
+            T* out = reinterpret_cast<T*>(a.out);
+#pragma unroll
+            for (int e = 0; e < Pack<T, VB>::N; e++)
+                if (k0 + (uint32_t)e < a.outK) out[(uint64_t)(k0 + (uint32_t)e) * a.outCols + c] = z.v[e];
             return;
         }
     }

@@ -1,8 +1,9 @@
-"""One-column DOWN traversals write their last level (every sample's row) straight to the
-output instead of through a read-out launch: bit-identical to the traversal with a read-out
-(hipGraph replay and accumulation never take the fused form), integers equal to the oracle,
-sums of -0.0 read out as +0.0 as the read-out does, heavy sample rows (block and wavefront
-forms) included, and childless nodes that are not samples left in the row buffer."""
+"""DOWN traversals of rows of at most 16 bytes write their last level (every sample's row)
+straight to the output columns instead of through a read-out launch: bit-identical to the
+traversal with a read-out (hipGraph replay and accumulation never take the fused form), for
+every dtype and K up to 16 bytes a row (and just past it), integers equal to the oracle, rows
+of -0.0 read out as +0.0 as the read-out does, heavy sample rows (block and wavefront forms)
+included, and childless nodes that are not samples left in the row buffer."""
 
 import numpy as np
 import pytest
@@ -48,14 +49,16 @@ def _graphs():
 def _down(cg, dt, x, stream, **kw):
     import torch
 
-    y = torch.full((1, cg.num_samples), 7, dtype=x.dtype, device="cuda")
-    cg.matmul_ptr(dt, x.data_ptr(), cg.num_mutations, 1, capi.DOWN, y.data_ptr(), y.numel(), stream=stream, **kw)
+    K = x.shape[0]
+    y = torch.full((K, cg.num_samples), 7, dtype=x.dtype, device="cuda")
+    cg.matmul_ptr(dt, x.data_ptr(), cg.num_mutations, K, capi.DOWN, y.data_ptr(), y.numel(), stream=stream, **kw)
     torch.cuda.synchronize()
     return y.cpu().numpy()
 
 
 @pytest.mark.parametrize("name", ["tree", "wide", "one_heavy"])
-@pytest.mark.parametrize("dtype", [np.float32, np.int32, np.float64, np.int64, np.uint32])
+@pytest.mark.parametrize("dtype", [np.float32, np.int32, np.float64, np.int64, np.uint32, np.int16, np.uint8,
+                                   np.bool_])
 def test_fused_output_equals_readout(gpu, name, dtype):
     import torch
 
@@ -63,34 +66,38 @@ def test_fused_output_equals_readout(gpu, name, dtype):
     cg = capi.Graph(arrays)
     og = oracle.OracleGraph(arrays)
     dt = capi.NUMPY_TO_DTYPE[np.dtype(dtype)]
+    es = np.dtype(dtype).itemsize
     sp = torch.cuda.current_stream().cuda_stream
     rng = np.random.default_rng(31)
-    x = torch.from_numpy(make_input(rng, dtype, (1, cg.num_mutations))).cuda()
-    vec = torch.from_numpy(make_input(rng, dtype, (1,))).cuda()
-    for chain in (0, 1, 3):
-        print(f"fused out: {name} {np.dtype(dtype).name} chain={chain}", flush=True)
-        cg.set_tuning(capi.DOWN, dt, 1, chain=chain)
-        fused = _down(cg, dt, x, sp)
-        fused_init = _down(cg, dt, x, sp, init_ptr=vec.data_ptr(), init_mode=capi.INIT_VECTOR)
-        cg.set_tuning(capi.DOWN, dt, 1, chain=chain, use_graph=1)  # replayed graphs keep the read-out
-        for rep in range(2):
-            assert bits_equal(fused, _down(cg, dt, x, sp)), (name, chain, rep)
-            assert bits_equal(fused_init, _down(cg, dt, x, sp, init_ptr=vec.data_ptr(),
-                                                init_mode=capi.INIT_VECTOR)), (name, chain, rep)
+    for K in [k for k in (1, 2, 3, 4, 5, 8, 16, 17) if k * es <= 20]:
+        x = torch.from_numpy(make_input(rng, dtype, (K, cg.num_mutations))).cuda()
+        vec = torch.from_numpy(make_input(rng, dtype, (K,))).cuda()
+        for chain in (0, 1, 3):
+            print(f"fused out: {name} {np.dtype(dtype).name} K={K} chain={chain}", flush=True)
+            cg.set_tuning(capi.DOWN, dt, K, chain=chain)
+            init = {} if dtype == np.bool_ else dict(init_ptr=vec.data_ptr(), init_mode=capi.INIT_VECTOR)
+            fused = _down(cg, dt, x, sp)
+            fused_init = _down(cg, dt, x, sp, **init)
+            cg.set_tuning(capi.DOWN, dt, K, chain=chain, use_graph=1)  # replayed graphs keep the read-out
+            for rep in range(2):
+                assert bits_equal(fused, _down(cg, dt, x, sp)), (name, K, chain, rep)
+                assert bits_equal(fused_init, _down(cg, dt, x, sp, **init)), (name, K, chain, rep)
+            if np.dtype(dtype).kind in "iub":
+                assert bits_equal(fused, og.matmul(x.cpu().numpy(), oracle.DOWN)), (name, K, chain)
+        cg.set_tuning(capi.DOWN, dt, K)
+        if dtype == np.bool_:
+            continue
+        # Accumulation keeps the read-out (it adds to what the output holds).
+        base = torch.from_numpy(make_input(rng, dtype, (K, cg.num_samples))).cuda()
+        y = base.clone()
+        cg.matmul_ptr(dt, x.data_ptr(), cg.num_mutations, K, capi.DOWN, y.data_ptr(), y.numel(), stream=sp,
+                      flags=capi.FLAG_ACCUMULATE)
+        torch.cuda.synchronize()
+        plain = _down(cg, dt, x, sp)
         if np.dtype(dtype).kind in "iu":
-            assert bits_equal(fused, og.matmul(x.cpu().numpy(), oracle.DOWN)), (name, chain)
-    # Accumulation keeps the read-out (it adds to what the output holds).
-    cg.set_tuning(capi.DOWN, dt, 1)
-    base = torch.from_numpy(make_input(rng, dtype, (1, cg.num_samples))).cuda()
-    y = base.clone()
-    cg.matmul_ptr(dt, x.data_ptr(), cg.num_mutations, 1, capi.DOWN, y.data_ptr(), y.numel(), stream=sp,
-                  flags=capi.FLAG_ACCUMULATE)
-    torch.cuda.synchronize()
-    plain = _down(cg, dt, x, sp)
-    if np.dtype(dtype).kind in "iu":
-        assert bits_equal(y.cpu().numpy(), (base.cpu().numpy() + plain).astype(dtype))
-    else:
-        assert np.array_equal(y.cpu().numpy(), base.cpu().numpy() + plain)
+            assert bits_equal(y.cpu().numpy(), (base.cpu().numpy() + plain).astype(dtype)), K
+        else:
+            assert np.array_equal(y.cpu().numpy(), base.cpu().numpy() + plain), K
     cg.close()
 
 
