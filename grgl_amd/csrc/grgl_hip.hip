@@ -804,7 +804,6 @@ uint32_t groupClassOf(uint32_t rowBytes) {
 // Column slices of a grouped launch: one for rows of at most 64 bytes, else one per 64 bytes.
 uint32_t groupSlices(const RowGeometry& geo) { return geo.rowBytes <= 64 ? 1 : (geo.rowBytes + 63) / 64; }
 
-
 // The grouped-launch plan a traversal takes, or null: the launch table's heavy threshold
 // equal to the plan's, not order-preserving (whose lists are all light), no per-launch
 // timing, and rows of at most groupDefaultMaxRowBytes unless the tuning asks for grouped
