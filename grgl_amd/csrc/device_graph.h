@@ -126,6 +126,7 @@ constexpr uint32_t kDeviceGroupClasses = 5; // slots of 4, 8, 16, 32, 64 bytes
 struct DeviceDirPlan {
     bool down = false;             // the DOWN plan (else UP)
     bool samplesLast = false;      // DOWN: every sample's row is in the last level
+    bool firstLevelSeedsOnly = false; // the first pulled level's lists read only seed rows
     uint32_t numNodes = 0, numLevels = 0, firstPulledLevel = 0, maxFanIn = 0;
     uint64_t numEntries = 0;
     std::vector<uint32_t> levelStart;

@@ -1,4 +1,5 @@
 // libgrgl_hip.so: device graph upload, the traversal driver and the C ABI of grgl_hip.h.
+#include <functional>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -840,15 +841,28 @@ struct OutFusion {
     uint32_t K;
 };
 
+// Seed fold of a one-column traversal (PullArgs::readRows, copy*): the first pulled level reads
+// its seed rows (all its lists read) from the input, and its launch copies the input to the
+// seed rows [seedBegin, seedBegin + bytes / ld) for the later levels, so there is no seed launch.
+struct SeedFold {
+    const char* in;
+    uint32_t seedBegin;
+    uint64_t bytes;
+};
+
 // The launches of every pulled level of one direction, in order (the trailing small levels as
 // one tail launch).  With `timing`, every level gets a launch of its own and its bytes.  With
 // `fuse`, the last level's level-kernel launches write its rows to the output when a row is one
-// column slice (*fused is then set; a last level in the tail launch is not fused).
+// column slice (*fused is then set; a last level in the tail launch is not fused).  With `fold`,
+// the first pulled level's level-kernel launches read the seeds from the input and the first
+// of them copies them (*folded is then set; a first level in a grouped or tail launch is not).
 std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTable& tab, TypeClass tc,
                                        const RowGeometry& geo, const Tuning& tuning, bool readSelf,
                                        bool orderPreserving, uint32_t K, size_t elemSize, char* rows, bool timing,
-                                       size_t groupScratch, const OutFusion* fuse = nullptr, bool* fused = nullptr) {
+                                       size_t groupScratch, const OutFusion* fuse = nullptr, bool* fused = nullptr,
+                                       const SeedFold* fold = nullptr, bool* folded = nullptr) {
     std::vector<LevelLaunch> out;
+    bool copyAttached = false;
     const uint32_t tw = tuning.teamLanes;
     const bool xcdMap = tuning.xcdMap == 1;
     const uint32_t tiles = levelTiles(geo, tw, elemSize);
@@ -922,6 +936,7 @@ std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTa
         LevelLaunch L;
         PullArgs& a = L.pull;
         a.rows = rows;
+        a.readRows = rows;
         a.ld = geo.ld;
         a.off = plan.off;
         a.src = plan.src + ck.entryBegin;
@@ -951,6 +966,21 @@ std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTa
         const uint32_t bxCount = a.heavyBlocks + (light + teamsPerBlock - 1) / teamsPerBlock;
         a.xcdTiles = (xcdMap && tiles % 8 == 0) ? tiles : 0;
         L.grid = a.xcdTiles ? dim3(bxCount * tiles, 1) : dim3(bxCount, tiles);
+        if (fold && l == plan.firstPulledLevel && tiles == 1 && geo.ld == elemSize) {
+            // Seed row r of the input sits at in + (r - seedBegin) * ld: read it there.
+            a.readRows = reinterpret_cast<const char*>(reinterpret_cast<uintptr_t>(fold->in) -
+                                                       (uintptr_t)fold->seedBegin * geo.ld);
+            if (!copyAttached) { // the level's first launch copies the seeds
+                copyAttached = true;
+                a.copySrc = fold->in;
+                a.copyDst = rows + (uint64_t)fold->seedBegin * geo.ld;
+                a.copyBytes = fold->bytes;
+                a.copyFirstBlock = bxCount;
+                const uint64_t words = fold->bytes / 4, perBlock = (uint64_t)blk * 16;
+                L.grid.x = bxCount + (uint32_t)std::min<uint64_t>(1024, (words + perBlock - 1) / perBlock);
+            }
+            if (folded) *folded = true;
+        }
         L.func = pullKernel(tc, vb, tw, h32, blk, unr);
         L.block = blk;
         if (timing) {
@@ -1094,15 +1124,21 @@ bool launchChained(grgl_graph* g, const DeviceDirPlan& plan, const LaunchTable& 
 // in the row buffer address, the row geometry, the team width and the launch table, so the
 // cache key holds all of them; different workspaces get different graphs.
 // Returns whether the last level's rows went to the output (`fuse`: direct launches only).
+// `seeds`, when given, launches the seed rows: it runs before the levels unless `fold` applies
+// (direct launches whose first pulled level is a level-kernel launch), which replaces it.
 bool runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometry& geo, const Tuning& t,
                bool readSelf, bool orderPreserving, uint32_t K, size_t es, char* rows, hipStream_t stream,
-               bool launchOnly = false, uint32_t* chainState = nullptr, const OutFusion* fuse = nullptr) {
+               bool launchOnly = false, uint32_t* chainState = nullptr, const OutFusion* fuse = nullptr,
+               const SeedFold* fold = nullptr, const std::function<void()>& seeds = {}) {
     const DeviceDirPlan& plan = dir == GRGL_DIRECTION_UP ? g->up : g->down;
     const std::shared_ptr<const LaunchTable> tab = launchTable(g, dir);
     const bool timing = g->timing && !launchOnly;
-    if (t.chain == 2 && !timing && !launchOnly &&
-        launchChained(g, plan, *tab, tc, geo, t.teamLanes, readSelf, orderPreserving, es, rows, chainState, stream))
-        return false;
+    if (t.chain == 2 && !timing && !launchOnly) {
+        if (seeds) seeds();
+        if (launchChained(g, plan, *tab, tc, geo, t.teamLanes, readSelf, orderPreserving, es, rows, chainState,
+                          stream))
+            return false;
+    }
     bool direct = !t.useGraph || timing || launchOnly;
     if (!direct && stream != nullptr) {
         // A stream the caller is capturing gets direct launches (they become part of the
@@ -1111,14 +1147,17 @@ bool runLevels(grgl_graph* g, grgl_direction dir, TypeClass tc, const RowGeometr
         GRGL_HIP_CHECK(hipStreamIsCapturing(stream, &cs));
         direct = cs != hipStreamCaptureStatusNone;
     }
+    const bool seeded = t.chain == 2 && !timing && !launchOnly;
     if (direct) {
-        bool fused = false;
+        bool fused = false, folded = false;
         std::vector<LevelLaunch> launches =
             levelLaunches(plan, *tab, tc, geo, t, readSelf, orderPreserving, K, es, rows, timing,
-                          groupScratchOffset(g, dir, geo), fuse, &fused);
+                          groupScratchOffset(g, dir, geo), fuse, &fused, seeded ? nullptr : fold, &folded);
+        if (!folded && !seeded && seeds) seeds();
         launchDirect(g, launches, stream, timing);
         return fused;
     }
+    if (!seeded && seeds) seeds();
     const auto key = std::make_tuple((int)dir, (int)tc, geo.vb, t.teamLanes, geo.ld, geo.rowBytes,
                                      (int)readSelf * 2 + (int)orderPreserving + 4 * (int)(t.xcdMap == 1) +
                                          8 * (int)(t.blockThreads == 512) + 16 * (int)(t.loadsInFlight == 8) +
@@ -1208,55 +1247,74 @@ void enqueueTraversal(grgl_graph* g, grgl_dtype dt, grgl_direction dir, const st
     // zero fill of the missingness seed rows for all columns).
     const std::vector<TravRequest> stacked = stackable(reqs);
     const bool stackSeeds = !stacked.empty() && !hasInit && (dir == GRGL_DIRECTION_UP || !opt.miss);
-    if (stackSeeds) {
-        if (dir == GRGL_DIRECTION_UP) {
-            const uint32_t per = opt.byIndividual ? g->P : 1;
-            const uint64_t inCols = opt.byIndividual ? g->N / g->P : g->N;
-            stackedTranspose(tc, es, true, stacked, false, geo, rows, 0, g->N, inCols, per, nullptr, false, stream);
-        } else {
-            stackedTranspose(tc, es, true, stacked, false, geo, rows, g->V, g->M, g->M, 1, nullptr, false, stream);
-            if (g->hasMiss) {
-                InitArgs a{};
-                a.rows = rows;
-                a.ld = geo.ld;
-                a.rowBegin = g->V + g->M;
-                a.rowEnd = g->V + 2 * g->M;
-                a.kOffset = 0;
-                a.K = (uint32_t)(geo.rowBytes / es);
-                a.kind = kInitZero;
-                launchInit(tc, a, stream);
-            }
-        }
-    }
-    for (const auto& r : reqs) {
-        if (stackSeeds) break;
-        if (dir == GRGL_DIRECTION_UP) {
-            // Samples are rows 0..N-1 (grg.h:1456-1469).
-            const uint32_t per = opt.byIndividual ? g->P : 1;
-            const uint64_t inCols = opt.byIndividual ? g->N / g->P : g->N;
-            seedRows(tc, es, r.in, inCols, r.K, rows, geo.ld, 0, r.kOffset, g->N, per, nullptr, hasInit, stream);
-        } else {
-            // Mutation seed rows V..V+M-1; missingness seed rows V+M..V+2M-1.
-            seedRows(tc, es, r.in, g->M, r.K, rows, geo.ld, g->V, r.kOffset, g->M, 1, nullptr, false, stream);
-            if (g->hasMiss) {
-                if (opt.miss) {
-                    seedRows(tc, es, opt.miss, g->M, r.K, rows, geo.ld, g->V + g->M, r.kOffset, g->M, 1, nullptr, false,
-                             stream);
-                } else {
+    auto launchSeeds = [&]() {
+        if (stackSeeds) {
+            if (dir == GRGL_DIRECTION_UP) {
+                const uint32_t per = opt.byIndividual ? g->P : 1;
+                const uint64_t inCols = opt.byIndividual ? g->N / g->P : g->N;
+                stackedTranspose(tc, es, true, stacked, false, geo, rows, 0, g->N, inCols, per, nullptr, false, stream);
+            } else {
+                stackedTranspose(tc, es, true, stacked, false, geo, rows, g->V, g->M, g->M, 1, nullptr, false, stream);
+                if (g->hasMiss) {
                     InitArgs a{};
                     a.rows = rows;
                     a.ld = geo.ld;
                     a.rowBegin = g->V + g->M;
                     a.rowEnd = g->V + 2 * g->M;
-                    a.kOffset = r.kOffset;
-                    a.K = r.K;
+                    a.kOffset = 0;
+                    a.K = (uint32_t)(geo.rowBytes / es);
                     a.kind = kInitZero;
                     launchInit(tc, a, stream);
                 }
             }
         }
+        for (const auto& r : reqs) {
+            if (stackSeeds) break;
+            if (dir == GRGL_DIRECTION_UP) {
+                // Samples are rows 0..N-1 (grg.h:1456-1469).
+                const uint32_t per = opt.byIndividual ? g->P : 1;
+                const uint64_t inCols = opt.byIndividual ? g->N / g->P : g->N;
+                seedRows(tc, es, r.in, inCols, r.K, rows, geo.ld, 0, r.kOffset, g->N, per, nullptr, hasInit, stream);
+            } else {
+                // Mutation seed rows V..V+M-1; missingness seed rows V+M..V+2M-1.
+                seedRows(tc, es, r.in, g->M, r.K, rows, geo.ld, g->V, r.kOffset, g->M, 1, nullptr, false, stream);
+                if (g->hasMiss) {
+                    if (opt.miss) {
+                        seedRows(tc, es, opt.miss, g->M, r.K, rows, geo.ld, g->V + g->M, r.kOffset, g->M, 1, nullptr,
+                                 false, stream);
+                    } else {
+                        InitArgs a{};
+                        a.rows = rows;
+                        a.ld = geo.ld;
+                        a.rowBegin = g->V + g->M;
+                        a.rowEnd = g->V + 2 * g->M;
+                        a.kOffset = r.kOffset;
+                        a.K = r.K;
+                        a.kind = kInitZero;
+                        launchInit(tc, a, stream);
+                    }
+                }
+            }
+        }
+        GRGL_HIP_CHECK(hipGetLastError());
+    };
+    // A one-column request of a 4- or 8-byte type without node initialisation, on a plan whose
+    // first pulled level reads only seed rows (UP: level 1 reads only the samples, which are all
+    // of level 0; DOWN: level 0, the roots, reads only mutation seeds, with no missingness rows):
+    // the input row is the seed rows' layout, so the first level reads it there and its launch
+    // copies it for the later levels (runLevels; the seed launch runs only where that does not
+    // apply).  Saves the seed launch of a K = 1 call (4.5-4.8 us on the 200k x 1M graph).
+    SeedFold sf{};
+    const bool foldable = reqs.size() == 1 && Ktotal == 1 && reqs[0].kOffset == 0 && !hasInit && es >= 4 &&
+                          geo.ld == es && plan.firstLevelSeedsOnly &&
+                          (dir == GRGL_DIRECTION_UP ? !opt.byIndividual : !g->hasMiss);
+    if (foldable) {
+        sf.in = static_cast<const char*>(reqs[0].in);
+        sf.seedBegin = dir == GRGL_DIRECTION_UP ? 0 : g->V;
+        sf.bytes = (uint64_t)(dir == GRGL_DIRECTION_UP ? g->N : g->M) * es;
+    } else {
+        launchSeeds();
     }
-    GRGL_HIP_CHECK(hipGetLastError());
     // 3. Levels.
     const Tuning tuning = chooseTuning(g, dir, tc, Ktotal, geo);
     // ORDER_PRESERVING: every list is summed by one team front to back, i.e. in the
@@ -1276,8 +1334,10 @@ void enqueueTraversal(grgl_graph* g, grgl_dtype dt, grgl_direction dir, const st
         fo.outCols = g->N;
         fo.K = reqs[0].K;
     }
-    const bool fusedOut = runLevels(g, dir, tc, geo, tuning, hasInit, opt.orderPreserving, Ktotal, es, rows, stream,
-                                    opt.launchOnly, opt.chainState, fuse ? &fo : nullptr);
+    const bool fusedOut =
+        runLevels(g, dir, tc, geo, tuning, hasInit, opt.orderPreserving, Ktotal, es, rows, stream, opt.launchOnly,
+                  opt.chainState, fuse ? &fo : nullptr, foldable ? &sf : nullptr,
+                  foldable ? std::function<void()>(launchSeeds) : std::function<void()>());
     if (fusedOut) {
         GRGL_HIP_CHECK(hipGetLastError());
         return;
@@ -1561,6 +1621,18 @@ HostGrg hostGrgOf(const grgl_graph_desc& desc) {
     return h;
 }
 
+// Whether every gather entry of level `lvl` of `p` reads a row in [lo, hi).
+bool levelReadsOnly(const DirPlan& p, uint32_t lvl, uint64_t lo, uint64_t hi) {
+    uint64_t pos = 0;
+    for (uint32_t r = 0; r < p.levelStart[lvl]; r++) pos += (uint32_t)(p.off[r + 1] - p.off[r]);
+    for (uint32_t r = p.levelStart[lvl]; r < p.levelStart[lvl + 1]; r++) {
+        const uint64_t end = pos + (uint32_t)(p.off[r + 1] - p.off[r]);
+        for (; pos < end; pos++)
+            if (p.src[pos] < lo || p.src[pos] >= hi) return false;
+    }
+    return true;
+}
+
 // A device graph from both plans: upload, control words, clock rate.
 std::unique_ptr<grgl_graph> makeDeviceGraph(LevelizedGraph lg, int device) {
     int ndev = 0;
@@ -1587,6 +1659,11 @@ std::unique_ptr<grgl_graph> makeDeviceGraph(LevelizedGraph lg, int device) {
         g->down.samplesLast = lg.down.numLevels > 0;
         for (uint32_t sIdx = 0; sIdx < lg.N && g->down.samplesLast; sIdx++)
             g->down.samplesLast = lg.down.outRow[sIdx] >= lg.down.levelStart[lg.down.numLevels - 1];
+        // First pulled level reading only seed rows (UP: the samples, all of level 0; DOWN:
+        // mutation seeds): the condition for reading a one-column input in place.
+        g->up.firstLevelSeedsOnly = lg.up.numLevels > 1 && lg.up.levelStart[1] == lg.N &&
+                                    levelReadsOnly(lg.up, 1, 0, lg.N);
+        g->down.firstLevelSeedsOnly = lg.down.numLevels > 0 && levelReadsOnly(lg.down, 0, lg.V, lg.V + lg.M);
         uploadGroups(g->up, upGroups, g->deviceBytes);
         uploadGroups(g->down, downGroups, g->deviceBytes);
         void* ctl = nullptr;

@@ -117,6 +117,9 @@ template <typename T, int VB> __device__ __forceinline__ Pack<T, VB> shfl_down_p
 
 struct PullArgs {
     char* rows;
+    const char* readRows; // rows the lists are read from: `rows`, or for a first level whose
+                          // lists read only seed rows, the input itself (a one-column input is
+                          // the seed rows' layout), based so that seed row r is at r * ld
     uint64_t ld;          // row stride, bytes
     const uint32_t* off;  // list offsets modulo 2^32
     const uint32_t* src;  // the launch's first gather entry
@@ -143,6 +146,13 @@ struct PullArgs {
     const uint32_t* outCol = nullptr;
     uint32_t outCols = 0;
     uint32_t outK = 0;
+    // Seed copy folded into a first level that reads its seeds from the input (readRows):
+    // blocks from copyFirstBlock on copy copyBytes (a multiple of 4) from copySrc to copyDst,
+    // the seed rows the later levels read.
+    const char* copySrc = nullptr;
+    char* copyDst = nullptr;
+    uint64_t copyBytes = 0;
+    uint32_t copyFirstBlock = 0;
 };
 
 // Stores row `row`'s finished value (its slice at colByte) where it goes: the row buffer, or
@@ -199,7 +209,7 @@ __device__ __forceinline__ void pull_light_range(const PullArgs& a, uint32_t row
     for (; i + UNR <= e; i += UNR) {
         P x[UNR];
 #pragma unroll
-        for (int u = 0; u < UNR; u++) x[u] = load_pack<T, VB>(a.rows + (uint64_t)idx(i + u) * a.ld + colByte);
+        for (int u = 0; u < UNR; u++) x[u] = load_pack<T, VB>(a.readRows + (uint64_t)idx(i + u) * a.ld + colByte);
 #pragma unroll
         for (int u = 0; u < UNR; u++) acc_pack<T, VB, XOR>(acc, x[u]);
     }
@@ -208,7 +218,7 @@ __device__ __forceinline__ void pull_light_range(const PullArgs& a, uint32_t row
 #pragma unroll
         for (int u = 0; u < UNR; u++) {
             const uint32_t j = i + u < e ? i + u : e - 1;
-            x[u] = load_pack<T, VB>(a.rows + (uint64_t)idx(j) * a.ld + colByte);
+            x[u] = load_pack<T, VB>(a.readRows + (uint64_t)idx(j) * a.ld + colByte);
         }
 #pragma unroll
         for (int u = 0; u < UNR; u++)
@@ -256,7 +266,7 @@ __device__ __forceinline__ void pull_heavy(const PullArgs& a, uint32_t row, uint
             for (int st = 0; st < STEPS; st++)
 #pragma unroll
                 for (int q = 0; q < PPT; q++)
-                    x[st][q] = load_pack<T, VB>(a.rows + (uint64_t)a.src[i + (st * PPT + q) * A] * a.ld + colByte);
+                    x[st][q] = load_pack<T, VB>(a.readRows + (uint64_t)a.src[i + (st * PPT + q) * A] * a.ld + colByte);
 #pragma unroll
             for (int st = 0; st < STEPS; st++)
 #pragma unroll
@@ -269,7 +279,7 @@ __device__ __forceinline__ void pull_heavy(const PullArgs& a, uint32_t row, uint
 #pragma unroll
                 for (int q = 0; q < PPT; q++) {
                     const uint32_t j = i + (uint32_t)(st * PPT + q) * A;
-                    x[st][q] = load_pack<T, VB>(a.rows + (uint64_t)a.src[j < e ? j : e - 1] * a.ld + colByte);
+                    x[st][q] = load_pack<T, VB>(a.readRows + (uint64_t)a.src[j < e ? j : e - 1] * a.ld + colByte);
                 }
 #pragma unroll
             for (int st = 0; st < STEPS; st++)
@@ -454,6 +464,15 @@ __device__ __forceinline__ bool pull_level_block(const PullArgs& a, uint32_t bx,
 // occupancy of the levels without heavy rows, so they use the plain variant.
 template <typename T, int VB, int TW, bool XOR, bool H32 = false, int BLK = kBlock, int UNRW = 4>
 __global__ __launch_bounds__(BLK) void pull_level_kernel(PullArgs a) {
+    if (a.copyBytes != 0 && blockIdx.x >= a.copyFirstBlock) {
+        // The folded seed copy (a one-column grid: blockIdx.x is the block's only coordinate).
+        const uint64_t n = a.copyBytes / 4, stride = (uint64_t)(gridDim.x - a.copyFirstBlock) * BLK;
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(a.copySrc);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(a.copyDst);
+        for (uint64_t i = (uint64_t)(blockIdx.x - a.copyFirstBlock) * BLK + threadIdx.x; i < n; i += stride)
+            dst[i] = src[i];
+        return;
+    }
     uint32_t bx, by;
     pull_block_coords(a, bx, by);
     (void)pull_level_block<T, VB, TW, XOR, H32, BLK, UNRW>(a, bx, by, NoWait{});
@@ -488,6 +507,7 @@ __global__ __launch_bounds__(kBlock) void pull_tail_kernel(TailArgs t) {
     for (uint32_t l = 0; l < t.numLevels; l++) {
         PullArgs a;
         a.rows = t.rows;
+        a.readRows = t.rows;
         a.ld = t.ld;
         a.off = t.off;
         a.src = t.src;
@@ -886,6 +906,7 @@ __global__ __launch_bounds__(kBlock) void pull_chain_kernel(ChainArgs c) {
         const uint32_t l = lo;
         PullArgs a;
         a.rows = c.rows;
+        a.readRows = c.rows;
         a.ld = c.ld;
         a.off = c.off;
         a.src = c.src;
