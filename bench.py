@@ -797,7 +797,7 @@ def pca_benchmark():
 
 
 def latency_trace(args):
-    """One request per traversal, FP32, K = 1, 4, 16, DOWN then UP per K, library defaults:
+    """One request per traversal, FP32, K = 1, 4, 16, 64, DOWN then UP per K, library defaults:
     3 untimed calls, then 20 timed calls per shape; one JSON line of milliseconds per call."""
     import torch
 
@@ -810,7 +810,7 @@ def latency_trace(args):
     N, M = g.num_samples, g.num_mutations
     out = {"workload": f"{args.haplotypes} haplotypes x {args.sites} sites, {args.trees} trees; FP32, one request "
                        "per traversal", "ms": {}}
-    for K in (1, 4, 16):
+    for K in (1, 4, 16, 64):
         for d, dn, cin, cout in ((capi.DOWN, "DOWN", M, N), (capi.UP, "UP", N, M)):
             x = torch.randn((K, cin), device="cuda")
             y = torch.zeros((K, cout), device="cuda")
