@@ -122,6 +122,7 @@ struct DeviceGroupPlan {
     bool empty() const { return levelBegin >= levelEnd; }
 };
 constexpr uint32_t kDeviceGroupClasses = 5; // slots of 4, 8, 16, 32, 64 bytes
+constexpr uint32_t kLateLevelRows = 8192;
 
 struct DeviceDirPlan {
     bool down = false;             // the DOWN plan (else UP)
@@ -141,6 +142,9 @@ struct DeviceDirPlan {
     // Grouped-launch plans per row class (built with the default heavy threshold; used only
     // while the launch table has that threshold).  Immutable after graph creation.
     DeviceGroupPlan groups[kDeviceGroupClasses];
+    // DOWN: 64-byte slots over only the small levels before the samples (levels of at most
+    // kLateLevelRows rows), for rows wider than 64 bytes run as column slices.
+    DeviceGroupPlan lateWide;
     // Current launch table; replaced (never modified) under grgl_graph::mu.
     std::shared_ptr<const LaunchTable> table;
     std::shared_ptr<const LaunchTable> buildTable(uint32_t heavyFanin, uint64_t chunkEntries) const;

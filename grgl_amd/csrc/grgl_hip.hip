@@ -692,9 +692,11 @@ constexpr bool kGroupByDefault = true;
 
 // Rows of up to this many bytes take grouped levels by default; wider rows when the tuning
 // asks for them.  UP: 512 (FP32 K <= 128); on the 200k x 1M graph 64-byte slices save 7-9%
-// at K = 17-32 and 3-6% at K = 48-128.  DOWN: 64 (FP32 K <= 16); slices save 1-4% at K = 17-32
-// there but lose 3.6% at K = 32 on the constructed 100k x 500k graph, and 1-8% at K >= 48.
-uint32_t groupDefaultMaxRowBytes(const DeviceDirPlan& plan) { return plan.down ? 64 : 512; }
+// at K = 17-32 and 3-6% at K = 48-128.  DOWN: 256 (FP32 K <= 64), rows over 64 bytes over its
+// small levels only (DeviceDirPlan::lateWide): 2-4% saved at K = 17-64 on the 200k x 1M graph,
+// 0.5-3% on the constructed 100k x 500k graph, none at K = 128-256 (over all its levels the
+// slices lost 1-8% at K >= 48).
+uint32_t groupDefaultMaxRowBytes(const DeviceDirPlan& plan) { return plan.down ? 256 : 512; }
 
 Tuning chooseTuning(grgl_graph* g, grgl_direction dir, TypeClass tc, uint32_t K, const RowGeometry& geo) {
     Tuning t;
@@ -816,7 +818,9 @@ const DeviceGroupPlan* groupFor(const DeviceDirPlan& plan, const LaunchTable& ta
     if (!want || timing || orderPreserving) return nullptr;
     if (geo.rowBytes > 64 && (uint64_t)groupSlices(geo) * 64 > geo.ld) return nullptr;
     const uint32_t cls = groupClassOf(geo.rowBytes);
-    const DeviceGroupPlan& gp = plan.groups[cls];
+    // DOWN rows over 64 bytes: only the small levels (the larger ones move wide rows faster
+    // one launch each).
+    const DeviceGroupPlan& gp = plan.down && geo.rowBytes > 64 ? plan.lateWide : plan.groups[cls];
     if (gp.empty() || gp.heavyFanin != tab.heavyFanin || !groupKernel(tc, cls, 256)) return nullptr;
     return &gp;
 }
@@ -1520,13 +1524,19 @@ void uploadPlan(DeviceDirPlan& d, const DirPlan& p, uint64_t& bytes) {
     d.table = d.buildTable(kDefaultHeavyFanin, kChunkEntries);
 }
 
-// Grouped-launch plans of both directions (built on two host threads).
+// Grouped-launch plans of both directions (built on two host threads).  DOWN gets one more
+// after its five classes: 64-byte slots over its levels of at most kLateLevelRows rows.
 void buildGroups(const LevelizedGraph& lg, std::vector<GroupPlan>& up, std::vector<GroupPlan>& down) {
     const GroupLimits lim;
     std::exception_ptr err;
     std::thread t([&] {
         try {
             down = buildGroupPlans(lg.down, groupRegionEnd(lg.down, true), kDefaultHeavyFanin, lim);
+            GroupLimits late;
+            late.maxLevelRows = kLateLevelRows;
+            late.classMask = 1U << (kGroupClasses - 1);
+            down.push_back(
+                buildGroupPlans(lg.down, groupRegionEnd(lg.down, true), kDefaultHeavyFanin, late)[kGroupClasses - 1]);
         } catch (...) {
             err = std::current_exception();
         }
@@ -1552,9 +1562,9 @@ uint64_t groupDeviceBytes(const GroupPlan& gp) {
 }
 
 void uploadGroups(DeviceDirPlan& d, const std::vector<GroupPlan>& plans, uint64_t& bytes) {
-    for (uint32_t c = 0; c < kDeviceGroupClasses && c < plans.size(); c++) {
+    for (uint32_t c = 0; c <= kDeviceGroupClasses && c < plans.size(); c++) {
         const GroupPlan& gp = plans[c];
-        DeviceGroupPlan& dg = d.groups[c];
+        DeviceGroupPlan& dg = c < kDeviceGroupClasses ? d.groups[c] : d.lateWide;
         dg = DeviceGroupPlan();
         dg.slotBytes = gp.slotBytes;
         dg.heavyFanin = gp.heavyFanin;
@@ -1592,6 +1602,7 @@ void freePlan(DeviceDirPlan& d) {
     freeDev(d.missRow);
     freeDev(d.coalByRow);
     for (auto& dg : d.groups) freeDev(dg.mem);
+    freeDev(d.lateWide.mem);
     d = DeviceDirPlan();
 }
 

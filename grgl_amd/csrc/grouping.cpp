@@ -354,6 +354,7 @@ std::vector<GroupPlan> buildGroupPlans(const DirPlan& p, uint32_t regionEnd, uin
     std::vector<uint32_t> first(kGroupClasses, regionEnd);
     uint32_t lowestWanted = regionEnd;
     for (uint32_t c = 0; c < kGroupClasses; c++) {
+        if (!((lim.classMask >> c) & 1U)) continue;
         uint32_t l0 = regionEnd;
         while (l0 > x.lowest) {
             const uint32_t L = l0 - 1;
@@ -362,7 +363,8 @@ std::vector<GroupPlan> buildGroupPlans(const DirPlan& p, uint32_t regionEnd, uin
             // Packing leaves blocks partly empty: the groups' total must fit 85% of the resident
             // blocks' LDS.
             if (maxBytes[c][L - x.lowest] + blockReserve(regionEnd - L) > lim.ldsBytes ||
-                maxEntries[L - x.lowest] > lim.maxEntries || total * 100 > cap * lim.maxBlocks * 85)
+                maxEntries[L - x.lowest] > lim.maxEntries || total * 100 > cap * lim.maxBlocks * 85 ||
+                p.levelStart[L + 1] - p.levelStart[L] > lim.maxLevelRows)
                 break;
             l0 = L;
         }
@@ -389,6 +391,8 @@ void validateGroupPlan(const DirPlan& p, const GroupPlan& gp, const GroupLimits&
     if (gp.empty()) return;
     if (gp.levelEnd > p.numLevels || gp.levelBegin < p.firstPulledLevel || gp.levelEnd < gp.levelBegin + 2)
         bad("levels");
+    for (uint32_t l = gp.levelBegin; l < gp.levelEnd; l++)
+        if (p.levelStart[l + 1] - p.levelStart[l] > lim.maxLevelRows) bad("a level over the row limit");
     const uint32_t Rb = p.levelStart[gp.levelBegin], R1 = p.levelStart[gp.levelEnd], m = R1 - Rb;
     const uint32_t B = gp.blocks;
     if (gp.row.size() != m || gp.off.size() != (size_t)m + 1 || gp.pre.size() != m ||

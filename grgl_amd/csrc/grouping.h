@@ -41,6 +41,12 @@ struct GroupLimits {
     // Gather entries examined per direction at most: bounds the plans' host time and device
     // memory (each class's list copies, 8 bytes an entry) on graphs of billions of edges.
     uint64_t maxScanEntries = 1ULL << 24;
+    // Grouped levels hold at most this many rows each: a lower bound leaves the larger levels
+    // to the level kernels (which move wide rows faster) and groups only the small levels after
+    // them.
+    uint32_t maxLevelRows = 0xffffffffU;
+    // Row classes built (bit c: class c).
+    uint32_t classMask = (1U << kGroupClasses) - 1;
 };
 
 // LDS of one block, in bytes: value slots (rows, 64 partials per heavy row, imports) of
@@ -89,7 +95,7 @@ std::vector<GroupPlan> buildGroupPlans(const DirPlan& p, uint32_t regionEnd, uin
 // Throws std::runtime_error unless `gp` is a grouping of levels [levelBegin, levelEnd) of
 // `p`: every row of those levels exactly once, level-ordered segments, lists equal to the
 // plan's with in-group entries as slots of the same block, slot lists and imports that
-// read the same rows, heavy rows and LDS as stated.
+// read the same rows, heavy rows and LDS as stated, levels within lim.maxLevelRows.
 void validateGroupPlan(const DirPlan& p, const GroupPlan& gp, const GroupLimits& lim);
 
 } // namespace grgl_amd

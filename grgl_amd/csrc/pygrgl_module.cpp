@@ -904,12 +904,14 @@ PYBIND11_MODULE(_grgl_amd, m) {
 
     m.def(
         "_group_plans",
-        [](PyGRGPtr& g, Direction dir, uint32_t heavyFanin, uint32_t ldsBytes, uint32_t maxEntries, uint32_t maxBlocks) {
+        [](PyGRGPtr& g, Direction dir, uint32_t heavyFanin, uint32_t ldsBytes, uint32_t maxEntries, uint32_t maxBlocks,
+           uint32_t maxLevelRows) {
             const DirPlan p = dir == Direction::UP ? buildUpPlan(g->H()) : buildDownPlan(g->H());
             GroupLimits lim;
             lim.ldsBytes = ldsBytes;
             lim.maxEntries = maxEntries;
             lim.maxBlocks = maxBlocks;
+            lim.maxLevelRows = maxLevelRows;
             const uint32_t regionEnd = groupRegionEnd(p, dir == Direction::DOWN);
             std::vector<GroupPlan> plans = buildGroupPlans(p, regionEnd, heavyFanin, lim);
             py::list out;
@@ -942,7 +944,7 @@ PYBIND11_MODULE(_grgl_amd, m) {
             return out;
         },
         py::arg("grg"), py::arg("direction"), py::arg("heavy_fanin") = 64, py::arg("lds_bytes") = 65536,
-        py::arg("max_entries") = 16384, py::arg("max_blocks") = 512,
+        py::arg("max_entries") = 16384, py::arg("max_blocks") = 512, py::arg("max_level_rows") = 0xffffffffU,
         "The grouped-launch plans (one per row class: 4, 8, 16, 32, 64-byte slots) the device library "
         "builds for one direction, each validated (for inspection).");
 

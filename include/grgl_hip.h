@@ -219,7 +219,7 @@ typedef struct grgl_tuning {
                                     every later call in that direction. */
     uint32_t chain;          /* form of the level sequence; never changes results.
                                 0 = default: grouped levels (3) where the graph has them for rows of
-                                at most 64 bytes (DOWN) or 512 bytes (UP), else 1.
+                                at most 256 bytes (DOWN) or 512 bytes (UP), else 1.
                                 1 = one launch per level.
                                 2 = the whole level sequence as ONE chained launch where the shape
                                 allows it (rows of at most 64 bytes of a 4- or 8-byte type, at most

@@ -1,7 +1,8 @@
 // Grouped-level plans under random graphs and limits (run by tests/test_grouping_fuzz.py,
 // built with the address and undefined-behaviour sanitizers): for every direction of a random
 // ordered graph and random limits (LDS per block, entries per group, blocks, heavy threshold,
-// entries examined), every plan buildGroupPlans returns must pass validateGroupPlan, fit the
+// entries examined, rows per level, classes built), every plan buildGroupPlans returns must
+// pass validateGroupPlan, fit the
 // LDS limit, and compute — executed on the CPU the way pull_group_kernel does it (leading HBM
 // entries first, then each level's rows from LDS slots, heavy rows as 64 partials and a fixed
 // tree) — the same float bits as the level plan executed the way the level kernels do it.
@@ -163,6 +164,8 @@ int main(int argc, char** argv) {
                 lim.maxEntries = 4U << (rng() % 13);       // 4 .. 16384
                 lim.maxBlocks = 1 + (uint32_t)(rng() % 600);
                 lim.maxScanEntries = rng() % 3 ? (1ULL << 26) : (uint64_t)(rng() % 400);
+                lim.maxLevelRows = rng() % 3 ? 0xffffffffU : 1 + (uint32_t)(rng() % 200);
+                lim.classMask = rng() % 4 ? (1U << kGroupClasses) - 1 : (uint32_t)(rng() % (1U << kGroupClasses));
                 const uint32_t heavy = 2 + (uint32_t)(rng() % 70);
                 const std::vector<GroupPlan> plans = buildGroupPlans(p, groupRegionEnd(p, down), heavy, lim);
                 if (plans.size() != kGroupClasses) throw std::runtime_error("class count");

@@ -191,8 +191,10 @@ def test_grouped_tuning_report(gpu):
         assert t["chain"] == 3 and t["block_threads"] == 1024, t
         assert cg.get_tuning(direction, capi.F32, 16)["chain"] == 3
         assert cg.get_tuning(direction, capi.F32, 16)["block_threads"] == 256
-        # 128-byte rows: 64-byte slices by default UP (rows up to 512 bytes), not DOWN (64).
-        assert cg.get_tuning(direction, capi.F32, 32)["chain"] == (3 if direction == capi.UP else 1)
+        # Rows over 64 bytes: 64-byte slices by default up to 512 bytes UP, 256 bytes DOWN (over
+        # DOWN's small levels only).
+        assert cg.get_tuning(direction, capi.F32, 32)["chain"] == 3
+        assert cg.get_tuning(direction, capi.F32, 64)["chain"] == 3
         assert cg.get_tuning(direction, capi.F32, 128)["chain"] == (3 if direction == capi.UP else 1)
         assert cg.get_tuning(direction, capi.F32, 129)["chain"] == 1
         cg.set_tuning(direction, capi.F32, 1, block_threads=256)

@@ -171,6 +171,37 @@ def test_group_plans_compute_the_level_plan_bitwise(name, arrays, heavy, lds, ma
         assert np.array_equal(out, og.matmul(xi, d))
 
 
+@pytest.mark.parametrize("name,arrays", list(_graphs()))
+def test_level_row_limit(name, arrays):
+    """A row limit per level (the DOWN plan for wide rows takes only levels of at most 8,192
+    rows) starts the grouped levels after the last larger level, ends them where the unlimited
+    plan ends, and still computes the level plan bit for bit."""
+    g = _graph(arrays)
+    rng = np.random.default_rng(9)
+    for direction in (grgl_amd.UP, grgl_amd.DOWN):
+        plan = M._level_plan(g, direction)
+        ls, first, L = plan["level_start"], plan["first_pulled_level"], plan["num_levels"]
+        sizes = np.diff(ls)
+        full = M._group_plans(g, direction, max_blocks=16)
+        cols = g.num_samples if direction == grgl_amd.UP else g.num_mutations
+        x = rng.standard_normal((2, cols)).astype(np.float32)
+        ref = _rows0(g, plan, direction, x)
+        _run_levels(ref, plan, first, L, 64)
+        for limit in sorted(set(int(v) for v in sizes[first:]))[:4]:
+            limited = M._group_plans(g, direction, max_blocks=16, max_level_rows=limit)
+            for a, b in zip(full, limited):
+                if b["level_begin"] >= b["level_end"]:
+                    continue
+                assert a["level_begin"] < a["level_end"] and b["level_end"] == a["level_end"]
+                assert b["level_begin"] >= a["level_begin"]
+                assert sizes[b["level_begin"]:b["level_end"]].max() <= limit
+                rows = _rows0(g, plan, direction, x)
+                _run_levels(rows, plan, first, b["level_begin"], 64)
+                _run_groups(rows, b)
+                _run_levels(rows, plan, b["level_end"], L, 64)
+                assert np.array_equal(rows.view(np.uint32), ref.view(np.uint32)), (name, direction, limit)
+
+
 def test_down_plan_puts_childless_nodes_last():
     for arrays in (random_grg(5, num_samples=50, num_internal=200), grgl_amd.grg_from_coalescent(300, 2000, 2, seed=1).arrays()):
         g = _graph(arrays)
