@@ -979,9 +979,9 @@ std::vector<LevelLaunch> levelLaunches(const DeviceDirPlan& plan, const LaunchTa
                 a.copySrc = fold->in;
                 a.copyDst = rows + (uint64_t)fold->seedBegin * geo.ld;
                 a.copyBytes = fold->bytes;
-                a.copyFirstBlock = bxCount;
                 const uint64_t words = fold->bytes / 4, perBlock = (uint64_t)blk * 16;
-                L.grid.x = bxCount + (uint32_t)std::min<uint64_t>(1024, (words + perBlock - 1) / perBlock);
+                a.copyBlocks = (uint32_t)std::min<uint64_t>(1024, (words + perBlock - 1) / perBlock);
+                L.grid.x = a.copyBlocks + bxCount;
             }
             if (folded) *folded = true;
         }

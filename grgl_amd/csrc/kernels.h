@@ -146,13 +146,14 @@ struct PullArgs {
     const uint32_t* outCol = nullptr;
     uint32_t outCols = 0;
     uint32_t outK = 0;
-    // Seed copy folded into a first level that reads its seeds from the input (readRows):
-    // blocks from copyFirstBlock on copy copyBytes (a multiple of 4) from copySrc to copyDst,
-    // the seed rows the later levels read.
+    // Seed copy folded into a first level that reads its seeds from the input (readRows): the
+    // first copyBlocks blocks (dispatched first, so the copy overlaps the level's rows) copy
+    // copyBytes (a multiple of 4) from copySrc to copyDst, the seed rows the later levels read;
+    // the level's own blocks follow.
     const char* copySrc = nullptr;
     char* copyDst = nullptr;
     uint64_t copyBytes = 0;
-    uint32_t copyFirstBlock = 0;
+    uint32_t copyBlocks = 0;
 };
 
 // Stores row `row`'s finished value (its slice at colByte) where it goes: the row buffer, or
@@ -464,17 +465,17 @@ __device__ __forceinline__ bool pull_level_block(const PullArgs& a, uint32_t bx,
 // occupancy of the levels without heavy rows, so they use the plain variant.
 template <typename T, int VB, int TW, bool XOR, bool H32 = false, int BLK = kBlock, int UNRW = 4>
 __global__ __launch_bounds__(BLK) void pull_level_kernel(PullArgs a) {
-    if (a.copyBytes != 0 && blockIdx.x >= a.copyFirstBlock) {
+    if (blockIdx.x < a.copyBlocks) {
         // The folded seed copy (a one-column grid: blockIdx.x is the block's only coordinate).
-        const uint64_t n = a.copyBytes / 4, stride = (uint64_t)(gridDim.x - a.copyFirstBlock) * BLK;
+        const uint64_t n = a.copyBytes / 4, stride = (uint64_t)a.copyBlocks * BLK;
         const uint32_t* src = reinterpret_cast<const uint32_t*>(a.copySrc);
         uint32_t* dst = reinterpret_cast<uint32_t*>(a.copyDst);
-        for (uint64_t i = (uint64_t)(blockIdx.x - a.copyFirstBlock) * BLK + threadIdx.x; i < n; i += stride)
-            dst[i] = src[i];
+        for (uint64_t i = (uint64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += stride) dst[i] = src[i];
         return;
     }
     uint32_t bx, by;
     pull_block_coords(a, bx, by);
+    bx -= a.copyBlocks;
     (void)pull_level_block<T, VB, TW, XOR, H32, BLK, UNRW>(a, bx, by, NoWait{});
 }
 
