@@ -934,7 +934,7 @@ def main():
         for cdt, d, _, _, _ in work:
             best, ms = cg.autotune(d, cdt, Kpad)
             tuning_chosen[f"{'f32' if cdt == capi.F32 else 'i32'}_{'UP' if d == capi.UP else 'DOWN'}_K{Kpad}"] = dict(
-                best, levels_ms=round(ms, 4))
+                best, tuned_ms=round(ms, 4))
             log(f"autotune dtype={cdt} dir={d} rows={Kpad}: {best} levels {ms:.3f} ms")
 
     def step(timing_f32=False):
@@ -975,7 +975,7 @@ def main():
         for K, x, y in reqs:
             if not args.no_autotune:
                 best, ms = cg.autotune(d, cdt, K)
-                tuning_chosen[f"{'f32' if cdt == capi.F32 else 'i32'}_{name}_K{K}"] = dict(best, levels_ms=round(ms, 4))
+                tuning_chosen[f"{'f32' if cdt == capi.F32 else 'i32'}_{name}_K{K}"] = dict(best, tuned_ms=round(ms, 4))
             reps = 5
             cg.matmul_ptr(cdt, x.data_ptr(), cin, K, d, y.data_ptr(), y.numel(), stream=sp)
             torch.cuda.synchronize()

@@ -252,7 +252,11 @@ grgl_status grgl_hip_set_tuning(grgl_graph* graph, grgl_direction direction, grg
  * While marked, every matmul call on the graph fails with GRGL_ERR_DEVICE (a call with host
  * buffers also checks after its own traversal).  No counterpart in the reference. */
 grgl_status grgl_hip_chain_control(grgl_graph* graph, int abort, uint32_t wait_us, uint32_t* gave_up, int reset);
-/* Time candidate tunings on the device and keep the fastest; writes it to *chosen. */
+/* Time candidate tunings on the device and keep the fastest; writes it to *chosen and its time
+ * to *best_ms: of a whole traversal (seed, levels, read-out) of zero inputs for rows of at
+ * most 512 bytes (rows * dtype size) whose input and output fit 4 GiB, else of the level
+ * sequence alone.  While it runs, other calls
+ * of the same (direction, dtype, rows) may take any candidate (results never depend on it). */
 grgl_status grgl_hip_autotune(grgl_graph* graph, grgl_direction direction, grgl_dtype dtype, uint32_t rows,
                               grgl_tuning* chosen, float* best_ms);
 /* The tuning a call of (direction, dtype, rows) uses now: the stored choice with library
