@@ -2498,17 +2498,20 @@ GRGL_EXPORT grgl_status grgl_hip_autotune(grgl_graph* g, grgl_direction directio
                 g->tuning[tuningKey] = t; // the calls below take it (results never depend on it)
             }
             runOnce(t); // warm-up
-            float trial[3];
-            for (float& ms : trial) {
+            // Median of 3 trials; of 5 for level sequences alone (wide rows, where candidates
+            // differ by a few percent and single trials by as much).
+            const int nTrials = io ? 3 : 5;
+            float trial[5];
+            for (int k = 0; k < nTrials; k++) {
                 GRGL_HIP_CHECK(hipEventRecord(e0, s));
                 runOnce(t);
                 GRGL_HIP_CHECK(hipEventRecord(e1, s));
                 GRGL_HIP_CHECK(hipEventSynchronize(e1));
-                GRGL_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                GRGL_HIP_CHECK(hipEventElapsedTime(&trial[k], e0, e1));
             }
-            std::sort(trial, trial + 3);
-            if (trial[1] < bestMs) {
-                bestMs = trial[1];
+            std::sort(trial, trial + nTrials);
+            if (trial[nTrials / 2] < bestMs) {
+                bestMs = trial[nTrials / 2];
                 best = t;
             }
         }
